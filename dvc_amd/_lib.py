@@ -1,0 +1,151 @@
+"""ctypes binding of libdvc_accel.so (include/dvc_accel.h).
+
+Importing this module loads the library; a missing library raises instead of
+falling back to any CPU path.
+"""
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("DVC_ACCEL_LIB", PKG / "libdvc_accel.so"))
+DATA_DIR = PKG / "data"
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python -m dvc_amd.build` "
+        "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+
+lib = ctypes.CDLL(str(LIB_PATH))
+
+DVC_OK, DVC_EINVAL, DVC_EDEVICE, DVC_EIO, DVC_ENOMEM, DVC_ESTATE = 0, -1, -2, -3, -4, -5
+
+
+class DvcError(RuntimeError):
+    def __init__(self, rc, where):
+        msg = lib.dvc_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed ({rc}): {msg}")
+        self.rc = rc
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("max_iter", ctypes.c_int),
+                ("data_dir", ctypes.c_char_p), ("bp_threads", ctypes.c_int)]
+
+
+class BpStats(ctypes.Structure):
+    _fields_ = [("start_level", ctypes.c_int), ("level", ctypes.c_int), ("rate", ctypes.c_int),
+                ("requests", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("used_inverse", ctypes.c_int), ("entropy", ctypes.c_float)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class LdpcaItem(ctypes.Structure):
+    _fields_ = [("N", ctypes.c_int), ("LR", ctypes.c_void_p), ("acc", ctypes.c_void_p),
+                ("crc", ctypes.c_uint8), ("start_level", ctypes.c_int),
+                ("decoded", ctypes.c_void_p), ("stats", BpStats)]
+
+
+# Every symbol include/dvc_accel.h declares, with its ctypes signature.
+_c = ctypes
+SIGNATURES = {
+    "dvc_ctx_create": (_c.c_int, [_c.POINTER(Params), _c.POINTER(_c.c_void_p)]),
+    "dvc_ctx_destroy": (_c.c_int, [_c.c_void_p]),
+    "dvc_last_error": (_c.c_char_p, []),
+    "dvc_prepare_data": (_c.c_int, [_c.c_char_p]),
+    "dvc_ladder_info": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_void_p]),
+    "dvc_ladder_export": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_void_p, _c.c_void_p]),
+    "dvc_ldpca_decode": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_uint8,
+                                    _c.c_int, _c.c_void_p, _c.POINTER(BpStats)]),
+    "dvc_ldpca_decode_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(LdpcaItem)]),
+}
+for _name, (_res, _args) in SIGNATURES.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def _check(rc, where):
+    if rc != DVC_OK:
+        raise DvcError(rc, where)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def prepare_data(data_dir=None):
+    _check(lib.dvc_prepare_data(str(data_dir).encode() if data_dir else None), "dvc_prepare_data")
+
+
+class Context:
+    """Owns a dvc_ctx (device, both ladders and their inverses resident)."""
+
+    def __init__(self, device=0, max_iter=50, bp_threads=0, data_dir=None):
+        p = Params(device, max_iter, str(data_dir).encode() if data_dir else None, bp_threads)
+        h = ctypes.c_void_p()
+        _check(lib.dvc_ctx_create(ctypes.byref(p), ctypes.byref(h)), "dvc_ctx_create")
+        self.h = h
+        self.max_iter = max_iter
+
+    def close(self):
+        if self.h:
+            lib.dvc_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ladder_info(self, n):
+        nc, ne = ctypes.c_int(), ctypes.c_int()
+        synd = np.zeros(128, np.int32)
+        _check(lib.dvc_ladder_info(self.h, n, ctypes.byref(nc), ctypes.byref(ne), _ptr(synd)), "dvc_ladder_info")
+        return nc.value, ne.value, synd[:nc.value].copy()
+
+    def ladder_export(self, n, code):
+        nc, ne, _ = self.ladder_info(n)
+        ir = np.zeros(ne, np.int32)
+        inv = np.zeros(n * 4 * ((n + 31) // 32), np.uint8)
+        _check(lib.dvc_ladder_export(self.h, n, code, _ptr(ir), _ptr(inv)), "dvc_ladder_export")
+        return ir, inv
+
+    def ldpca_decode(self, lr, acc, crc, start_level=-1):
+        """Sequential rate search for one bitplane (host arrays).  Returns
+        (decoded uint8[N], stats dict)."""
+        lr = np.ascontiguousarray(lr, np.float32)
+        acc = np.ascontiguousarray(acc, np.uint8)
+        n = lr.size
+        out = np.zeros(n, np.uint8)
+        st = BpStats()
+        _check(lib.dvc_ldpca_decode(self.h, n, _ptr(lr), _ptr(acc), int(crc), int(start_level), _ptr(out),
+                                    ctypes.byref(st)), "dvc_ldpca_decode")
+        return out, st.as_dict()
+
+    def ldpca_decode_batch(self, lrs, accs, crcs, start_levels=None):
+        """Speculative batched decode of many bitplanes (mixed N allowed)."""
+        k = len(lrs)
+        items = (LdpcaItem * max(k, 1))()
+        keep, outs = [], []
+        for i in range(k):
+            lr = np.ascontiguousarray(lrs[i], np.float32)
+            acc = np.ascontiguousarray(accs[i], np.uint8)
+            out = np.zeros(lr.size, np.uint8)
+            keep += [lr, acc]
+            outs.append(out)
+            sl = -1 if start_levels is None else int(start_levels[i])
+            items[i] = LdpcaItem(lr.size, lr.ctypes.data, acc.ctypes.data, int(crcs[i]), sl, out.ctypes.data, BpStats())
+        _check(lib.dvc_ldpca_decode_batch(self.h, k, items), "dvc_ldpca_decode_batch")
+        return outs, [items[i].stats.as_dict() for i in range(k)]

@@ -1,0 +1,285 @@
+// capi_ldpca.cpp -- context management and the LDPCA entry points of
+// include/dvc_accel.h.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <dlfcn.h>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include "dvc_internal.h"
+#include "ldpca_kernels.h"
+
+namespace dvc {
+
+static thread_local std::string g_err;
+void set_error(const std::string &msg) { g_err = msg; }
+
+// Directory of this shared library's data files: <lib dir>/data.
+static std::string default_data_dir() {
+  Dl_info info;
+  if (dladdr((void *)&default_data_dir, &info) && info.dli_fname) {
+    std::string p(info.dli_fname);
+    size_t s = p.find_last_of('/');
+    if (s != std::string::npos) return p.substr(0, s) + "/data";
+  }
+  return "data";
+}
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  hipError_t alloc(size_t count) {
+    n = count;
+    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+};
+
+static int bp_threads_for(const dvc_ctx *ctx, int N) {
+  if (N == 1584) return 256;
+  return ctx->bp_threads_cif > 0 ? ctx->bp_threads_cif : 1024;
+}
+
+static int persistent_grid(int N, int threads) {
+  hipDeviceProp_t prop;
+  int cus = 256;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
+  // workgroups resident per CU: limited by LDS (CIF codewords use ~150 KB)
+  int per_cu = N == 6336 ? 1 : (threads <= 256 ? 4 : 2);
+  return cus * per_cu;
+}
+
+}  // namespace dvc
+
+using namespace dvc;
+
+extern "C" {
+
+const char *dvc_last_error(void) { return g_err.c_str(); }
+
+int dvc_ctx_create(const dvc_params *params, dvc_ctx **out) {
+  DVC_CHECK(out != nullptr, DVC_EINVAL, "dvc_ctx_create: out is NULL");
+  *out = nullptr;
+  dvc_params p{};
+  if (params) p = *params;
+  int ndev = 0;
+  DVC_HIP(hipGetDeviceCount(&ndev));
+  DVC_CHECK(ndev > 0 && p.device >= 0 && p.device < ndev, DVC_EDEVICE, "dvc_ctx_create: no such HIP device");
+  DVC_HIP(hipSetDevice(p.device));
+  hipDeviceProp_t prop;
+  DVC_HIP(hipGetDeviceProperties(&prop, p.device));
+  DVC_CHECK(strncmp(prop.gcnArchName, "gfx950", 6) == 0, DVC_EDEVICE,
+            std::string("dvc_ctx_create: this library is built for gfx950, device is ") + prop.gcnArchName);
+  dvc_ctx *c = new dvc_ctx();
+  c->device = p.device;
+  c->max_iter = p.max_iter > 0 ? p.max_iter : 50;
+  c->bp_threads_cif = p.bp_threads;
+  std::string dir = p.data_dir ? std::string(p.data_dir) : default_data_dir();
+  int rc;
+  if ((rc = load_ladder(dir, 1584, c->lad1584)) || (rc = load_ladder(dir, 6336, c->lad6336)) ||
+      (rc = upload_ladder(c->lad1584)) || (rc = upload_ladder(c->lad6336))) {
+    dvc_ctx_destroy(c);
+    return rc;
+  }
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { dvc_ctx_destroy(c); set_error("hipStreamCreate failed"); return DVC_EDEVICE; }
+  *out = c;
+  return DVC_OK;
+}
+
+int dvc_ctx_destroy(dvc_ctx *c) {
+  if (!c) return DVC_OK;
+  (void)hipSetDevice(c->device);
+  free_ladder(c->lad1584);
+  free_ladder(c->lad6336);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return DVC_OK;
+}
+
+int dvc_prepare_data(const char *data_dir) {
+  std::string dir = data_dir ? std::string(data_dir) : default_data_dir();
+  for (int N : {1584, 6336}) {
+    Ladder L;
+    int rc = load_ladder(dir, N, L);
+    if (rc) return rc;
+  }
+  return DVC_OK;
+}
+
+int dvc_ladder_info(dvc_ctx *c, int N, int *num_codes, int *num_edges, int *synd_length) {
+  DVC_CHECK(c, DVC_EINVAL, "dvc_ladder_info: NULL context");
+  Ladder *L = c->ladder(N);
+  DVC_CHECK(L, DVC_EINVAL, "dvc_ladder_info: N must be 1584 or 6336");
+  if (num_codes) *num_codes = L->numCodes;
+  if (num_edges) *num_edges = L->E;
+  if (synd_length) memcpy(synd_length, L->syndLength.data(), sizeof(int) * L->numCodes);
+  return DVC_OK;
+}
+
+int dvc_ladder_export(dvc_ctx *c, int N, int code, int *ir_out, uint8_t *inv_out) {
+  DVC_CHECK(c, DVC_EINVAL, "dvc_ladder_export: NULL context");
+  Ladder *L = c->ladder(N);
+  DVC_CHECK(L && code >= 0 && code < L->numCodes, DVC_EINVAL, "dvc_ladder_export: bad N or code");
+  if (ir_out) memcpy(ir_out, &L->decIr[(size_t)code * L->E], sizeof(int) * L->E);
+  if (inv_out) memcpy(inv_out, L->invH.data(), L->invH.size());
+  return DVC_OK;
+}
+
+// Shared driver for one group of bitplanes with the same code length.
+// sequential != 0: one code per launch in increasing order (the plain path,
+// which mirrors the reference's per-code beliefPropagation_CUDA call).
+static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int sequential, hipStream_t s) {
+  Ladder *L = c->ladder(N);
+  const int C = L->numCodes, nb = N / 8;
+  DevBuf<float> dLR, dLLR;
+  DevBuf<uint8_t> dAcc, dSlots;
+  DevBuf<PlaneDesc> dPlanes;
+  DevBuf<PlaneState> dState;
+  DevBuf<int4> dRec;
+  DevBuf<int> dCounters;  // [0] queue head, [1] slot count
+  const int slotCap = sequential ? 2 : std::max(4 * n, n + 64);
+  DVC_HIP(dLR.alloc((size_t)n * N));
+  DVC_HIP(dLLR.alloc((size_t)n * N));
+  DVC_HIP(dAcc.alloc((size_t)n * nb));
+  DVC_HIP(dSlots.alloc((size_t)slotCap * N));
+  DVC_HIP(dPlanes.alloc(n));
+  DVC_HIP(dState.alloc(n));
+  DVC_HIP(dRec.alloc((size_t)n * C));
+  DVC_HIP(dCounters.alloc(2));
+  std::vector<PlaneDesc> planes(n);
+  std::vector<PlaneState> state(n);
+  for (int i = 0; i < n; i++) {
+    DVC_CHECK(items[i].LR && items[i].acc && items[i].decoded, DVC_EINVAL, "dvc_ldpca: NULL item pointer");
+    DVC_HIP(hipMemcpyAsync(dLR.p + (size_t)i * N, items[i].LR, sizeof(float) * N, hipMemcpyHostToDevice, s));
+    DVC_HIP(hipMemcpyAsync(dAcc.p + (size_t)i * nb, items[i].acc, nb, hipMemcpyHostToDevice, s));
+    planes[i] = PlaneDesc{N, dLR.p + (size_t)i * N, dLLR.p + (size_t)i * N, dAcc.p + (size_t)i * nb,
+                          (uint32_t)items[i].crc, dState.p + i};
+    int st0 = items[i].start_level;
+    DVC_CHECK(st0 < C, DVC_EINVAL, "dvc_ldpca: start_level beyond the ladder");
+    state[i] = PlaneState{st0 >= 0 ? st0 : -1, C - 1, 0.0f, 0};
+  }
+  std::vector<int4> recInit((size_t)n * C, make_int4(0, REC_NOT_RUN, -1, 0));
+  DVC_HIP(hipMemcpyAsync(dPlanes.p, planes.data(), sizeof(PlaneDesc) * n, hipMemcpyHostToDevice, s));
+  DVC_HIP(hipMemcpyAsync(dState.p, state.data(), sizeof(PlaneState) * n, hipMemcpyHostToDevice, s));
+  DVC_HIP(hipMemcpyAsync(dRec.p, recInit.data(), sizeof(int4) * recInit.size(), hipMemcpyHostToDevice, s));
+  DVC_HIP(launch_ce_llr(dPlanes.p, n, 1, c->lad1584, c->lad6336, s));
+  DVC_HIP(hipMemcpyAsync(state.data(), dState.p, sizeof(PlaneState) * n, hipMemcpyDeviceToHost, s));
+  DVC_HIP(hipStreamSynchronize(s));
+
+  const int threads = bp_threads_for(c, N);
+  LadderDev ld = ladder_dev(*L);
+  std::vector<int4> rec((size_t)n * C);
+  if (sequential) {
+    // Plain path: bitplane by bitplane, one code per launch.
+    for (int i = 0; i < n; i++) {
+      int level = state[i].start;
+      for (; level < C - 1; level++) {
+        PlaneState ps{level, C - 1, state[i].entropy, 0};
+        int zero[2] = {0, 0};
+        DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
+        DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
+        BpArgs a{ld, dPlanes.p + i, 1, dCounters.p, 1, 1, c->max_iter, dSlots.p, dCounters.p + 1, slotCap,
+                 dRec.p + (size_t)i * C};
+        DVC_HIP(launch_bp_any(N, threads, a, 1, s));
+        int4 r;
+        DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + level, sizeof(r), hipMemcpyDeviceToHost, s));
+        DVC_HIP(hipStreamSynchronize(s));
+        if (r.y == REC_ACCEPTED) {
+          DVC_HIP(hipMemcpy(items[i].decoded, dSlots.p + (size_t)r.z * N, N, hipMemcpyDeviceToHost));
+          break;
+        }
+      }
+      if (level == C - 1) {
+        PlaneState ps{state[i].start, C - 1, state[i].entropy, 0};
+        int zero[2] = {0, 0};
+        DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
+        DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
+        DVC_HIP(launch_invh(dPlanes.p + i, 1, ld, dSlots.p, dCounters.p + 1, slotCap, dRec.p + (size_t)i * C, s));
+        int4 r;
+        DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + (C - 1), sizeof(r), hipMemcpyDeviceToHost, s));
+        DVC_HIP(hipStreamSynchronize(s));
+        DVC_HIP(hipMemcpy(items[i].decoded, dSlots.p + (size_t)r.z * N, N, hipMemcpyDeviceToHost));
+      }
+    }
+    DVC_HIP(hipMemcpy(rec.data(), dRec.p, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost));
+  } else {
+    int zero[2] = {0, 0};
+    DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
+    const int maxOff = C - 1;
+    BpArgs a{ld, dPlanes.p, n, dCounters.p, n * maxOff, maxOff, c->max_iter, dSlots.p, dCounters.p + 1, slotCap, dRec.p};
+    int grid = std::min(persistent_grid(N, threads), std::max(1, n * maxOff));
+    DVC_HIP(launch_bp_any(N, threads, a, grid, s));
+    DVC_HIP(launch_invh(dPlanes.p, n, ld, dSlots.p, dCounters.p + 1, slotCap, dRec.p, s));
+    int counters[2];
+    DVC_HIP(hipMemcpyAsync(rec.data(), dRec.p, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost, s));
+    DVC_HIP(hipMemcpyAsync(state.data(), dState.p, sizeof(PlaneState) * n, hipMemcpyDeviceToHost, s));
+    DVC_HIP(hipMemcpyAsync(counters, dCounters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
+    DVC_HIP(hipStreamSynchronize(s));
+    DVC_CHECK(counters[1] <= slotCap, DVC_ENOMEM, "dvc_ldpca_decode_batch: decision slots exhausted");
+    for (int i = 0; i < n; i++) {
+      int best = state[i].best;
+      const int4 r = rec[(size_t)i * C + best];
+      DVC_CHECK(r.y == REC_ACCEPTED && r.z >= 0, DVC_ESTATE, "dvc_ldpca_decode_batch: missing accepted record");
+      DVC_HIP(hipMemcpyAsync(items[i].decoded, dSlots.p + (size_t)r.z * N, N, hipMemcpyDeviceToHost, s));
+    }
+    DVC_HIP(hipStreamSynchronize(s));
+  }
+  // Stats as the sequential search reports them.
+  for (int i = 0; i < n; i++) {
+    dvc_bp_stats &st = items[i].stats;
+    memset(&st, 0, sizeof(st));
+    st.start_level = state[i].start;
+    st.entropy = state[i].entropy;
+    int lvl = -1;
+    for (int l = st.start_level; l < C; l++) {
+      const int4 r = rec[(size_t)i * C + l];
+      if (r.y == REC_ACCEPTED) { lvl = l; break; }
+      st.iterations += r.x;
+    }
+    DVC_CHECK(lvl >= 0, DVC_ESTATE, "dvc_ldpca: no accepted code");
+    st.level = lvl;
+    st.iterations += rec[(size_t)i * C + lvl].x;
+    st.rate = L->syndLength[lvl] + 8;
+    st.requests = lvl - st.start_level + 1;
+    st.used_inverse = rec[(size_t)i * C + lvl].w;
+  }
+  return DVC_OK;
+}
+
+int dvc_ldpca_decode(dvc_ctx *c, int N, const float *LR, const uint8_t *acc, uint8_t crc, int start_level,
+                     uint8_t *decoded, dvc_bp_stats *stats) {
+  DVC_CHECK(c, DVC_EINVAL, "dvc_ldpca_decode: NULL context");
+  DVC_CHECK(c->ladder(N), DVC_EINVAL, "dvc_ldpca_decode: N must be 1584 or 6336");
+  DVC_HIP(hipSetDevice(c->device));
+  dvc_ldpca_item it{N, LR, acc, crc, start_level, decoded, {}};
+  int rc = decode_group(c, N, 1, &it, 1, c->stream);
+  if (rc == DVC_OK && stats) *stats = it.stats;
+  return rc;
+}
+
+int dvc_ldpca_decode_batch(dvc_ctx *c, int n, dvc_ldpca_item *items) {
+  DVC_CHECK(c, DVC_EINVAL, "dvc_ldpca_decode_batch: NULL context");
+  DVC_CHECK(n >= 0 && (n == 0 || items), DVC_EINVAL, "dvc_ldpca_decode_batch: bad item list");
+  DVC_HIP(hipSetDevice(c->device));
+  std::vector<dvc_ldpca_item> g[2];
+  std::vector<int> gi[2];
+  for (int i = 0; i < n; i++) {
+    DVC_CHECK(items[i].N == 1584 || items[i].N == 6336, DVC_EINVAL, "dvc_ldpca_decode_batch: N must be 1584 or 6336");
+    int k = items[i].N == 6336;
+    g[k].push_back(items[i]);
+    gi[k].push_back(i);
+  }
+  for (int k = 0; k < 2; k++) {
+    if (g[k].empty()) continue;
+    int rc = decode_group(c, k ? 6336 : 1584, (int)g[k].size(), g[k].data(), 0, c->stream);
+    if (rc) return rc;
+    for (size_t j = 0; j < g[k].size(); j++) items[gi[k][j]].stats = g[k][j].stats;
+  }
+  return DVC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,71 @@
+// dvc_internal.h -- shared host-side state of the gfx950 decoding library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include "../../include/dvc_accel.h"
+
+namespace dvc {
+
+void set_error(const std::string &msg);
+
+#define DVC_HIP(call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      dvc::set_error(std::string(#call) + ": " + hipGetErrorString(e_));               \
+      return DVC_EDEVICE;                                                              \
+    }                                                                                  \
+  } while (0)
+
+#define DVC_CHECK(cond, code, msg)                                                     \
+  do {                                                                                 \
+    if (!(cond)) { dvc::set_error(msg); return (code); }                               \
+  } while (0)
+
+// Host copy of one ladder (all codes of one code length) and its device image.
+struct Ladder {
+  int N = 0, E = 0, period = 0, numCodes = 0;
+  std::vector<int> jc;             // N+1 (every column has degree 3)
+  std::vector<int> encIr;          // E encoder rows
+  std::vector<int> bitsPerPeriod;  // numCodes
+  std::vector<int> idx;            // numCodes*period
+  std::vector<int> syndLength;     // numCodes
+  std::vector<float> codeRate;     // numCodes
+  std::vector<int> decIr;          // numCodes*E decoder rows per edge
+  std::vector<uint8_t> invH;       // N * invRowBytes, MSB-first
+  int invRowBytes = 0;             // 4*ceil(N/32)
+  std::vector<uint8_t> crcBasis;   // N: CRC-8 of the message with only bit k set
+
+  // Device image, per BP code c (0..numCodes-2):
+  //   d_ir16[c*E + e]          decoder row of edge e (edges in column order)
+  //   d_rowEdge[c*E + j]       edges grouped by row, ascending edge within a row
+  //   d_rowPtr[c*(N+1) + r]    start of row r in d_rowEdge (r <= m)
+  //   d_tx[c*N + k]            accumulated-syndrome position of syndrome bit k
+  uint16_t *d_ir16 = nullptr, *d_rowEdge = nullptr, *d_rowPtr = nullptr, *d_tx = nullptr;
+  int *d_syndLength = nullptr;
+  float *d_codeRate = nullptr;
+  uint8_t *d_invH = nullptr, *d_crcBasis = nullptr;
+};
+
+int load_ladder(const std::string &dir, int N, Ladder &L);
+int upload_ladder(Ladder &L);
+void free_ladder(Ladder &L);
+
+}  // namespace dvc
+
+struct dvc_ctx {
+  int device = 0;
+  int max_iter = 50;
+  int bp_threads_cif = 0;
+  dvc::Ladder lad1584, lad6336;
+  hipStream_t stream = nullptr;
+  // scratch for the single-bitplane path
+  float *d_LR = nullptr, *d_LLR = nullptr;
+  uint8_t *d_acc = nullptr, *d_decoded = nullptr;
+  int *d_result = nullptr;  // [0] converged, [1] crc, [2] iterations, [3] start level, [4] entropy bits
+  void *h_pinned = nullptr;
+  size_t h_pinned_bytes = 0;
+  dvc::Ladder *ladder(int N) { return N == 1584 ? &lad1584 : (N == 6336 ? &lad6336 : nullptr); }
+};

@@ -1,0 +1,359 @@
+// ldpca.hip -- LDPCA sum-product decoding on gfx950.
+//
+// Replaces beliefPropagation_CUDA / beliefPropagation_CUDA_earlyJump
+// (DvcDec/src/LDPCA_cuda.cu:872-1031) and the rate search of LdpcaDecoder
+// (DvcDec/src/LdpcaDecoder.cpp:646-771).  Arithmetic follows the reference's
+// ParallelMode 0 decoder (beliefPropagation, LdpcaDecoder.cpp:462-580) step
+// for step, so every message, decision and iteration count equals the CPU
+// path's:
+//   * phi(x) = -logf(tanhf((x + 1e-9f)/2)) with the glibc-identical routines
+//     of dvc_math.h;
+//   * check-node sums accumulate in ascending edge order (= ascending column),
+//     the order of `rowTotal[ir[k]] += ...` (:514-515);
+//   * variable-node sums accumulate LLR + c2v0 + c2v1 + c2v2 (:527-539);
+//   * decisions are the sign bit of the overall LLR (:548); the syndrome test
+//     is the row parity (:551-568).
+//
+// Layout.  One workgroup owns one (bitplane, code) pair.  Messages live in LDS:
+// phi[E] holds |phi| of every variable-to-check message with the message's
+// sign in the sign bit; rowTot[m] holds each check's sum of |phi| with the
+// check's sign product (including the syndrome bit) in the sign bit; the
+// code's row->edge lists are staged into LDS once per item.  Thread t owns
+// columns t, t+T, ... (their three edges are contiguous, edges are stored in
+// column order), so the variable-node update, the next check-node input and
+// the decision are computed in registers without further exchange.  Two
+// barriers per iteration: after the check (row) pass and after the variable
+// (column) pass.
+//
+// Batched form.  A persistent grid walks a queue of (bitplane, code offset)
+// items in offset-major order.  An item whose code lies above the lowest code
+// that already converged with a matching CRC for its bitplane is skipped, and
+// a running item notices that at its next iteration and stops.  Every code
+// below the final choice always runs to completion, so the kept code is the
+// one the sequential search returns.
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+#include "dvc_internal.h"
+#include "dvc_math.h"
+#include "ldpca_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace dvc {
+
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+
+// Accumulated-syndrome bit i (MSB-first bytes, ByteToBit LdpcaDecoder.cpp:81-107).
+__device__ __forceinline__ uint32_t acc_bit(const uint8_t *acc, int i) {
+  return (acc[i >> 3] >> (7 - (i & 7))) & 1u;
+}
+
+// ------------------------------------------------------------------ CE + LLR
+// ComputeCE (LdpcaDecoder.cpp:127-143) and the LR -> LLR conversion (:663-667)
+// for one bitplane per workgroup.  The entropy is summed sequentially in
+// column order by lane 0, as the reference does.
+#define DVC_LOG2 (0.693147181f)
+__global__ __launch_bounds__(256) void ce_llr_kernel(const PlaneDesc *planes, int nplanes, int fixed_start,
+                                                     const float *codeRate1584, const float *codeRate6336,
+                                                     int numCodes) {
+  __shared__ float terms[6336];
+  const int p = blockIdx.x;
+  if (p >= nplanes) return;
+  PlaneDesc d = planes[p];
+  const int N = d.N;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) {
+    float lr = d.LR[k];
+    float Pco = lr / (1.0f + lr);
+    float tmp = dvc_logf(1 - Pco);
+    terms[k] = (Pco * (-dvc_logf(Pco) + tmp) - tmp) / DVC_LOG2;
+    d.LLR[k] = dvc_logf(lr);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float rlb = 0.0f;
+    for (int k = 0; k < N; k++) rlb += terms[k];
+    rlb /= (float)N;
+    const float *rate = N == 1584 ? codeRate1584 : codeRate6336;
+    int s = 0;
+    while (s < numCodes - 1 && rate[s] < rlb) s++;
+    PlaneState *st = d.state;
+    st->entropy = rlb;
+    if (st->start < 0 || !fixed_start) st->start = s;
+  }
+}
+// ------------------------------------------------------------------ BP
+template <int N, int T>
+struct BpShape {
+  static constexpr int E = 3 * N;
+  static constexpr int CPT = (N + T - 1) / T;      // columns per thread
+  static constexpr int DEC_WORDS = (CPT * T) / 32;  // ballot words of every column slot
+  static constexpr int SYN_WORDS = (N + 31) / 32;
+};
+
+template <int N, int T>
+__global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
+  using S = BpShape<N, T>;
+  constexpr int E = S::E;
+  __shared__ float phi[E];
+  __shared__ float rowTot[N];
+  __shared__ uint16_t rowEdge[E];
+  __shared__ uint16_t rowPtr[N + 2];
+  __shared__ uint32_t decBits[S::DEC_WORDS];
+  __shared__ uint32_t synBits[S::SYN_WORDS];
+  __shared__ int s_item, s_cancel;
+  __shared__ uint32_t s_crc[T / 64];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const LadderDev L = a.lad;
+
+  for (;;) {
+    // ---- grab the next item (lane 0 of the workgroup), broadcast through LDS
+    if (tid == 0) {
+      int item = -1;
+      for (;;) {
+        int w = atomicAdd(a.next, 1);
+        if (w >= a.total) break;
+        int b = w % a.nplanes, off = w / a.nplanes;
+        PlaneState *st = a.planes[b].state;
+        int level = st->start + off;
+        if (off >= a.max_offset || level >= L.numCodes - 1) continue;
+        if (__hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < level) continue;
+        item = w;
+        break;
+      }
+      s_item = item;
+    }
+    __syncthreads();
+    const int w = s_item;
+    if (w < 0) return;
+    const int b = w % a.nplanes;
+    const PlaneDesc pd = a.planes[b];
+    PlaneState *st = pd.state;
+    const int level = st->start + w / a.nplanes;
+    const int m = L.syndLength[level];
+    const uint16_t *gir = L.ir16 + (size_t)level * E;
+
+    // ---- stage the code's row lists and the syndrome (LdpcaDecoder.cpp:684-693)
+    for (int i = tid; i < E; i += T) rowEdge[i] = L.rowEdge[(size_t)level * E + i];
+    for (int i = tid; i <= m; i += T) rowPtr[i] = L.rowPtr[(size_t)level * (N + 1) + i];
+    for (int wd = tid; wd < S::SYN_WORDS; wd += T) {
+      uint32_t bits = 0;
+      for (int q = 0; q < 32; q++) {
+        int k = wd * 32 + q;
+        if (k < m) {
+          int cur = L.tx[(size_t)level * N + k];
+          uint32_t s = acc_bit(pd.acc, cur);
+          if (k > 0) s ^= acc_bit(pd.acc, L.tx[(size_t)level * N + k - 1]);
+          bits |= s << q;
+        }
+      }
+      synBits[wd] = bits;
+    }
+    // ---- variable-to-check messages start at the channel LLR (:478-481)
+    for (int j = 0; j < S::CPT; j++) {
+      int k = tid + j * T;
+      if (k < N) {
+        float v = pd.LLR[k];
+        uint32_t s = f2u(v) >> 31;
+        float mag = (s ? -1.0f : 1.0f) * v;
+        uint32_t f = (f2u(dvc_bp_phi(mag)) & 0x7fffffffu) | (s << 31);
+        phi[3 * k] = u2f(f); phi[3 * k + 1] = u2f(f); phi[3 * k + 2] = u2f(f);
+      }
+    }
+    __syncthreads();
+
+    int status = 0, iters = 0;
+    for (int t = 0;; t++) {
+      // ---- check-node pass: |phi| sums and sign products; parity of the
+      // previous decisions (:495-515, :551-568)
+      int fail = 0;
+      for (int r = tid; r < m; r += T) {
+        const int e0 = rowPtr[r], e1 = rowPtr[r + 1];
+        uint32_t syn = (synBits[r >> 5] >> (r & 31)) & 1u;
+        uint32_t sgn = syn, par = syn;
+        float sum = 0.0f;
+        for (int j = e0; j < e1; j++) {
+          const int e = rowEdge[j];
+          const uint32_t f = f2u(phi[e]);
+          sum += u2f(f & 0x7fffffffu);
+          sgn ^= f >> 31;
+          const int c = e / 3;
+          par ^= (decBits[c >> 5] >> (c & 31)) & 1u;
+        }
+        rowTot[r] = u2f(f2u(sum) | (sgn << 31));
+        if (t > 0 && par) fail = 1;
+      }
+      if (tid == 0)
+        s_cancel = __hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < level;
+      const int anyFail = __syncthreads_or(fail);
+      if (t > 0 && !anyFail) { status = 1; iters = t; break; }
+      if (s_cancel) { status = 3; iters = t; break; }
+      if (t == a.max_iter) { status = 0; iters = t; break; }
+
+      // ---- variable-node pass: c2v, overall LLR, decision, next v2c (:512-548)
+      for (int j = 0; j < S::CPT; j++) {
+        const int k = tid + j * T;
+        uint32_t dec = 0;
+        if (k < N) {
+          float c2v[3];
+          uint32_t ph[3];
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            const int e = 3 * k + q;
+            const uint32_t R = f2u(rowTot[gir[e]]);
+            ph[q] = f2u(phi[e]);
+            const float x = u2f(R & 0x7fffffffu) - u2f(ph[q] & 0x7fffffffu);
+            const float v = dvc_bp_phi(x);
+            c2v[q] = ((R ^ ph[q]) >> 31) ? v * -1.0f : v * 1.0f;
+          }
+          float tmp = pd.LLR[k];
+          tmp += c2v[0];
+          tmp += c2v[1];
+          tmp += c2v[2];
+          dec = f2u(tmp) >> 31;
+#pragma unroll
+          for (int q = 0; q < 3; q++) {
+            const float v2c = tmp - c2v[q];
+            const uint32_t s = f2u(v2c) >> 31;
+            const float mag = (s ? -1.0f : 1.0f) * v2c;
+            phi[3 * k + q] = u2f((f2u(dvc_bp_phi(mag)) & 0x7fffffffu) | (s << 31));
+          }
+        }
+        const uint64_t bal = __ballot(dec);
+        if (lane == 0) {
+          const int word = (j * T + wave * 64) >> 5;
+          decBits[word] = (uint32_t)bal;
+          decBits[word + 1] = (uint32_t)(bal >> 32);
+        }
+      }
+      __syncthreads();
+    }
+
+    // ---- converged: CRC-8 of the decisions (linear in the bits, crc8.h:5-6)
+    int ok = 0;
+    if (status == 1) {
+      uint32_t crc = 0;
+      for (int j = 0; j < S::CPT; j++) {
+        int k = tid + j * T;
+        if (k < N && ((decBits[k >> 5] >> (k & 31)) & 1u)) crc ^= L.crcBasis[k];
+      }
+      for (int o = 32; o > 0; o >>= 1) crc ^= __shfl_xor(crc, o);
+      if (lane == 0) s_crc[wave] = crc;
+      __syncthreads();
+      uint32_t tot = 0;
+      for (int i = 0; i < T / 64; i++) tot ^= s_crc[i];
+      ok = (tot == pd.crc);
+      if (ok) {
+        int slot = -1;
+        if (tid == 0) {
+          slot = atomicAdd(a.slot_count, 1);
+          s_item = slot;  // reuse as broadcast
+        }
+        __syncthreads();
+        slot = s_item;
+        if (slot < a.slot_capacity) {
+          uint8_t *dst = a.slots + (size_t)slot * N;
+          for (int j = 0; j < S::CPT; j++) {
+            int k = tid + j * T;
+            if (k < N) dst[k] = (uint8_t)((decBits[k >> 5] >> (k & 31)) & 1u);
+          }
+        }
+        if (tid == 0) {
+          a.records[(size_t)b * L.numCodes + level] = make_int4(iters, 2, slot, 0);
+          atomicMin(&st->best, level);
+        }
+      }
+    }
+    if (tid == 0 && !ok)
+      a.records[(size_t)b * L.numCodes + level] = make_int4(iters, status == 1 ? 1 : (status == 3 ? 3 : 0), -1, 0);
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ full rate
+// invH_mul (LdpcaDecoder.cpp:624-643) for every bitplane whose BP codes all
+// failed; one thread per output bit.
+__global__ void invh_kernel(const PlaneDesc *planes, int nplanes, LadderDev L, uint8_t *slots,
+                            int *slot_count, int slot_capacity, int4 *records) {
+  const int p = blockIdx.y;
+  if (p >= nplanes) return;
+  PlaneDesc d = planes[p];
+  PlaneState *st = d.state;
+  if (st->best < L.numCodes - 1) return;
+  const int N = L.N, nw = (N + 31) / 32;
+  __shared__ uint32_t sw[(6336 + 31) / 32];
+  __shared__ int s_slot;
+  // full-rate syndrome: s_k = a_k ^ a_{k-1} (identity transmission pattern)
+  for (int wd = threadIdx.x; wd < nw; wd += blockDim.x) {
+    uint32_t v = 0;
+    for (int q = 0; q < 32; q++) {
+      int k = wd * 32 + q;
+      uint32_t s = 0;
+      if (k < N) s = acc_bit(d.acc, k) ^ (k > 0 ? acc_bit(d.acc, k - 1) : 0u);
+      v |= s << (31 - (q & 31));  // MSB-first within the 32-bit chunk
+    }
+    sw[wd] = v;
+  }
+  if (threadIdx.x == 0) {
+    s_slot = atomicAdd(slot_count, 1);
+    records[(size_t)p * L.numCodes + (L.numCodes - 1)] = make_int4(0, 2, s_slot, 1);
+  }
+  __syncthreads();
+  const int slot = s_slot;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
+    const uint8_t *row = L.invH + (size_t)j * nw * 4;
+    uint32_t acc = 0;
+    for (int wd = 0; wd < nw; wd++) {
+      uint32_t r = ((uint32_t)row[4 * wd] << 24) | ((uint32_t)row[4 * wd + 1] << 16) |
+                   ((uint32_t)row[4 * wd + 2] << 8) | (uint32_t)row[4 * wd + 3];
+      acc ^= r & sw[wd];
+    }
+    if (slot >= 0 && slot < slot_capacity) slots[(size_t)slot * N + j] = (uint8_t)(__popc(acc) & 1);
+  }
+}
+
+// ------------------------------------------------------------------ host side
+template <int N, int T>
+static hipError_t launch_bp(const BpArgs &a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL((bp_persistent_kernel<N, T>), dim3(grid), dim3(T), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_bp_any(int N, int threads, const BpArgs &a, int grid, hipStream_t s) {
+  if (N == 6336) {
+    if (threads == 704) return launch_bp<6336, 704>(a, grid, s);
+    if (threads == 512) return launch_bp<6336, 512>(a, grid, s);
+    return launch_bp<6336, 1024>(a, grid, s);
+  }
+  if (threads == 512) return launch_bp<1584, 512>(a, grid, s);
+  return launch_bp<1584, 256>(a, grid, s);
+}
+
+hipError_t launch_ce_llr(const PlaneDesc *d_planes, int n, int fixed_start, const Ladder &l1584,
+                         const Ladder &l6336, hipStream_t s) {
+  hipLaunchKernelGGL(ce_llr_kernel, dim3(n), dim3(256), 0, s, d_planes, n, fixed_start, l1584.d_codeRate,
+                     l6336.d_codeRate, l6336.numCodes);
+  return hipGetLastError();
+}
+
+hipError_t launch_invh(const PlaneDesc *d_planes, int n, const LadderDev &L, uint8_t *slots, int *slot_count,
+                       int slot_capacity, int4 *records, hipStream_t s) {
+  // One workgroup column per bitplane; a single workgroup per bitplane keeps
+  // the slot assignment in blockIdx.x == 0 ordered before its use.
+  hipLaunchKernelGGL(invh_kernel, dim3(1, n), dim3(256), 0, s, d_planes, n, L, slots, slot_count,
+                     slot_capacity, records);
+  return hipGetLastError();
+}
+
+LadderDev ladder_dev(const Ladder &L) {
+  LadderDev d;
+  d.N = L.N; d.E = L.E; d.numCodes = L.numCodes;
+  d.ir16 = L.d_ir16; d.rowEdge = L.d_rowEdge; d.rowPtr = L.d_rowPtr; d.tx = L.d_tx;
+  d.syndLength = L.d_syndLength; d.invH = L.d_invH; d.crcBasis = L.d_crcBasis;
+  return d;
+}
+
+}  // namespace dvc

@@ -1,0 +1,59 @@
+// ldpca_kernels.h -- argument blocks shared by the LDPCA kernels and their
+// host-side launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dvc {
+
+struct Ladder;
+
+// Per-bitplane mutable state on the device.
+struct PlaneState {
+  int start;      // nCodeStart (-1: computed by the CE kernel)
+  int best;       // lowest code that converged with a matching CRC (numCodes-1 = none)
+  float entropy;  // H(X|Y) from ComputeCE
+  int pad;
+};
+
+struct PlaneDesc {
+  int N;
+  const float *LR;     // N likelihood ratios
+  float *LLR;          // N log-likelihood ratios (written by the CE kernel)
+  const uint8_t *acc;  // N/8 accumulated-syndrome bytes
+  uint32_t crc;        // encoder CRC-8
+  PlaneState *state;
+};
+
+struct LadderDev {
+  int N, E, numCodes;
+  const uint16_t *ir16, *rowEdge, *rowPtr, *tx;
+  const int *syndLength;
+  const uint8_t *invH, *crcBasis;
+};
+
+struct BpArgs {
+  LadderDev lad;
+  const PlaneDesc *planes;
+  int nplanes;
+  int *next;          // queue head (reset to 0 before a launch)
+  int total;          // nplanes * max_offset
+  int max_offset;     // codes above start tried per bitplane
+  int max_iter;
+  uint8_t *slots;     // accepted decisions, N bytes per slot
+  int *slot_count;
+  int slot_capacity;
+  int4 *records;      // [nplanes * numCodes]: iterations, status, slot, used_inverse
+};
+
+// record.y status values
+enum { REC_FAILED = 0, REC_CRC_MISMATCH = 1, REC_ACCEPTED = 2, REC_CANCELLED = 3, REC_NOT_RUN = -1 };
+
+LadderDev ladder_dev(const Ladder &L);
+hipError_t launch_bp_any(int N, int threads, const BpArgs &a, int grid, hipStream_t s);
+hipError_t launch_ce_llr(const PlaneDesc *d_planes, int n, int fixed_start, const Ladder &l1584,
+                         const Ladder &l6336, hipStream_t s);
+hipError_t launch_invh(const PlaneDesc *d_planes, int n, const LadderDev &L, uint8_t *slots, int *slot_count,
+                       int slot_capacity, int4 *records, hipStream_t s);
+
+}  // namespace dvc

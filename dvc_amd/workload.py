@@ -1,0 +1,293 @@
+"""Synthetic inputs for the decoder: seeded test video, the WZ-frame encoder
+that turns it into the decoder's bitstream fields, and seeded LDPCA bitplanes.
+
+Video (every workload): 8-bit luma, base 128, 40 Gaussian blobs (sigma
+U[4,20] px, amplitude U[20,120] with a random sign), a global integer
+translation per frame (dx, dy ~ U{-r..r}, accumulated), 4 independently moving
+16x16 patches, i.i.d. N(0, 2^2) noise, rounded and clipped.  Chroma would be
+constant 128 and is not generated (the decoder reads luma only,
+frame.cpp:91-111).
+
+Encoder: the WZ part of the reference encoder (DvcEnc: forwardTransform with
+the AC range, transform.cpp:229-276; quantize, quantization.cpp:95-140;
+extractBitplane, bitplane.cpp:4-38; LdpcaEncoder, LdpcaEncoder.cpp:64-95)
+with skip and intra blocks off, vectorised with numpy.  Integer arithmetic and
+the double-precision post-scaling round exactly as the C code does.
+"""
+import struct
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+DATA = Path(__file__).resolve().parent / "data"
+
+QTABLE = np.array([
+    [0] * 16,
+    [16, 8, 0, 0, 8, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [32, 8, 0, 0, 8, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [32, 8, 4, 0, 8, 4, 0, 0, 4, 0, 0, 0, 0, 0, 0, 0],
+    [32, 16, 8, 4, 16, 8, 4, 0, 8, 4, 0, 0, 4, 0, 0, 0],
+    [32, 16, 8, 4, 16, 8, 4, 4, 8, 4, 4, 0, 4, 4, 0, 0],
+    [64, 16, 8, 8, 16, 8, 8, 4, 8, 8, 4, 4, 8, 4, 4, 0],
+    [64, 32, 16, 8, 32, 16, 8, 4, 16, 8, 4, 4, 8, 4, 4, 0],
+    [128, 64, 32, 16, 64, 32, 16, 8, 32, 16, 8, 4, 16, 8, 4, 0]], np.int64)
+QBITS = np.array([
+    [0] * 16,
+    [4, 3, 0, 0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [5, 3, 0, 0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0],
+    [5, 3, 2, 0, 3, 2, 0, 0, 2, 0, 0, 0, 0, 0, 0, 0],
+    [5, 4, 3, 2, 4, 3, 2, 0, 3, 2, 0, 0, 2, 0, 0, 0],
+    [5, 4, 3, 2, 4, 3, 2, 2, 3, 2, 2, 0, 2, 2, 0, 0],
+    [6, 4, 3, 3, 4, 3, 3, 2, 3, 3, 2, 2, 3, 2, 2, 0],
+    [6, 5, 4, 3, 5, 4, 3, 2, 4, 3, 2, 2, 3, 2, 2, 0],
+    [7, 6, 5, 4, 6, 5, 4, 3, 5, 4, 3, 2, 4, 3, 2, 0]], np.int64)
+ZIGZAG = np.array([0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15])
+
+QCIF = (176, 144)
+CIF = (352, 288)
+
+
+def bitplanes_per_frame(q):
+    return int(QBITS[q].sum())
+
+
+# ----------------------------------------------------------------- video
+def synthetic_video(width, height, frames, shift_range, seed):
+    """uint8 array [frames, height, width] as described in the module doc."""
+    rng = np.random.default_rng(seed)
+    margin = shift_range * frames + 64
+    cw, ch = width + 2 * margin, height + 2 * margin
+    n_blobs = 40
+    cx = rng.uniform(0, cw, n_blobs)
+    cy = rng.uniform(0, ch, n_blobs)
+    sig = rng.uniform(4.0, 20.0, n_blobs)
+    amp = rng.uniform(20.0, 120.0, n_blobs) * rng.choice([-1.0, 1.0], n_blobs)
+    yy, xx = np.mgrid[0:ch, 0:cw].astype(np.float64)
+    canvas = np.full((ch, cw), 128.0)
+    for i in range(n_blobs):
+        canvas += amp[i] * np.exp(-((xx - cx[i]) ** 2 + (yy - cy[i]) ** 2) / (2.0 * sig[i] ** 2))
+    # four 16x16 patches with their own texture and velocity
+    patches = []
+    for _ in range(4):
+        tex = 128.0 + rng.uniform(-80, 80) + rng.normal(0.0, 25.0, (16, 16))
+        px, py = rng.uniform(0, width - 16), rng.uniform(0, height - 16)
+        vx, vy = rng.integers(-4, 5), rng.integers(-4, 5)
+        patches.append([tex, px, py, int(vx), int(vy)])
+    out = np.empty((frames, height, width), np.uint8)
+    ox = oy = 0
+    for f in range(frames):
+        if f > 0:
+            ox += int(rng.integers(-shift_range, shift_range + 1))
+            oy += int(rng.integers(-shift_range, shift_range + 1))
+        img = canvas[margin + oy:margin + oy + height, margin + ox:margin + ox + width].copy()
+        for p in patches:
+            tex, px, py, vx, vy = p
+            x0, y0 = int(round(px)), int(round(py))
+            img[y0:y0 + 16, x0:x0 + 16] = tex
+            # bounce inside the frame
+            nx, ny = px + vx, py + vy
+            if nx < 0 or nx > width - 16:
+                p[3] = -vx
+                nx = px - vx
+            if ny < 0 or ny > height - 16:
+                p[4] = -vy
+                ny = py - vy
+            p[1], p[2] = nx, ny
+        img += rng.normal(0.0, 2.0, img.shape)
+        out[f] = np.clip(np.rint(img), 0, 255).astype(np.uint8)
+    return out
+
+
+# ----------------------------------------------------------------- ladder
+@dataclass
+class EncLadder:
+    N: int
+    E: int
+    period: int
+    num_codes: int
+    jc: np.ndarray
+    ir: np.ndarray
+    bits_per_period: np.ndarray
+    idx: np.ndarray
+    crc_basis: np.ndarray = field(default=None)
+
+
+_CRC_TABLE = None
+
+
+def crc8(data):
+    """CRC-8, poly x^8+x^2+x+1, init 0, no reflection, xorout 0 (crc8.h)."""
+    global _CRC_TABLE
+    if _CRC_TABLE is None:
+        t = []
+        for i in range(256):
+            c = i
+            for _ in range(8):
+                c = ((c << 1) ^ 0x07) & 0xFF if c & 0x80 else (c << 1) & 0xFF
+            t.append(c)
+        _CRC_TABLE = t
+    crc = 0
+    for b in bytes(data):
+        crc = _CRC_TABLE[crc ^ b]
+    return crc
+
+
+def load_ladder(n):
+    blob = (DATA / f"reg{n}.ladder").read_bytes()
+    assert blob[:4] == b"DVCL"
+    ver, N, E, period, codes = struct.unpack_from("<5I", blob, 4)
+    off = 24
+    jc = np.frombuffer(blob, "<u4", N + 1, off).astype(np.int64); off += 4 * (N + 1)
+    ir = np.frombuffer(blob, "<u2", E, off).astype(np.int64); off += 2 * E
+    bpp = np.frombuffer(blob, "<u2", codes, off).astype(np.int64); off += 2 * codes
+    idx = np.frombuffer(blob, "<u2", codes * period, off).astype(np.int64).reshape(codes, period)
+    L = EncLadder(N, E, period, codes, jc, ir, bpp, idx)
+    # CRC is linear (zero init / xorout): CRC of a bitplane = XOR of the CRCs of its set bits
+    basis = np.zeros(N, np.uint8)
+    msg = bytearray(N // 8)
+    for k in range(N):
+        msg[k >> 3] = 0x80 >> (k & 7)
+        basis[k] = crc8(msg)
+        msg[k >> 3] = 0
+    L.crc_basis = basis
+    return L
+
+
+_LADDERS = {}
+
+
+def ladder(n):
+    if n not in _LADDERS:
+        _LADDERS[n] = load_ladder(n)
+    return _LADDERS[n]
+
+
+def ldpca_encode(L, bits):
+    """Accumulated syndrome bytes (N/8, MSB first) and CRC-8 of a bitplane."""
+    bits = np.asarray(bits, np.uint8)
+    col = np.repeat(np.arange(L.N), np.diff(L.jc))
+    synd = (np.bincount(L.ir, weights=bits[col].astype(np.float64), minlength=L.N).astype(np.int64) & 1)
+    acc = (np.cumsum(synd) & 1).astype(np.uint8)
+    crc = int(np.bitwise_xor.reduce(L.crc_basis[bits.astype(bool)])) if bits.any() else 0
+    return np.packbits(acc), crc
+
+
+def form_syndrome(L, code, acc_bits):
+    """Decoder syndrome of code `code` from N accumulated-syndrome bits
+    (LdpcaDecoder.cpp:684-693)."""
+    b = int(L.bits_per_period[code])
+    m = (L.N // L.period) * b
+    k = np.arange(m)
+    tx = L.idx[code][k % b] + (k // b) * L.period
+    a = acc_bits[tx]
+    s = a.copy()
+    s[1:] ^= a[:-1]
+    return s
+
+
+# ----------------------------------------------------------------- transform
+_A = 0.5
+_B = np.sqrt(0.4)
+POST_SCALE = np.array([_A * _A, _A * _B / 2, _A * _A, _A * _B / 2, _A * _B / 2, 0.4 / 4, _A * _B / 2, 0.4 / 4,
+                       _A * _A, _A * _B / 2, _A * _A, _A * _B / 2, _A * _B / 2, 0.4 / 4, _A * _B / 2, 0.4 / 4])
+
+
+def _round(x):
+    """ROUND (defines.h:23) on float64 arrays."""
+    return np.where(x >= 0.0, np.trunc(x + 0.5), -np.trunc(-x + 0.5)).astype(np.int64)
+
+
+def forward44_blocks(blk):
+    """forward44 (transform.cpp:63-104) on [..., 16] int arrays."""
+    b = blk.reshape(blk.shape[:-1] + (4, 4)).astype(np.int64)
+    x0, x1, x2, x3 = b[..., 0, :], b[..., 1, :], b[..., 2, :], b[..., 3, :]
+    t = np.stack([x0 + x1 + x2 + x3, 2 * x0 + x1 - x2 - 2 * x3, x0 - x1 - x2 + x3, x0 - 2 * x1 + 2 * x2 - x3], -2)
+    y0, y1, y2, y3 = t[..., 0], t[..., 1], t[..., 2], t[..., 3]
+    o = np.stack([y0 + y1 + y2 + y3, 2 * y0 + y1 - y2 - 2 * y3, y0 - y1 - y2 + y3, y0 - 2 * y1 + 2 * y2 - y3], -1)
+    o = o.reshape(blk.shape[:-1] + (16,))
+    return _round(o.astype(np.float64) * POST_SCALE)
+
+
+def frame_to_blocks(frame):
+    h, w = frame.shape
+    return frame.reshape(h // 4, 4, w // 4, 4).transpose(0, 2, 1, 3).reshape(h // 4, w // 4, 16)
+
+
+@dataclass
+class WZFrame:
+    """The WZ-bitstream fields of one frame (wzbitstream.cpp:13-126)."""
+    qindex: int
+    ac_range: np.ndarray     # int32[16]
+    crc: np.ndarray          # uint8[nbp]
+    acc: np.ndarray          # uint8[nbp, N/8]
+    bitplanes: np.ndarray    # uint8[nbp, N] (encoder side, for checks)
+
+
+def encode_wz_frame(frame, qindex):
+    h, w = frame.shape
+    n = (h // 4) * (w // 4)
+    L = ladder(n)
+    coef = forward44_blocks(frame_to_blocks(frame))          # [bh, bw, 16]
+    ac_range = np.abs(coef).reshape(-1, 16).max(0)
+    ac_range[0] = 0
+    qt, qb = QTABLE[qindex], QBITS[qindex]
+    step = np.zeros(16, np.int64)
+    step[0] = 1024 // qt[0]
+    shift = np.zeros(16, np.int64)
+    for i in range(1, 16):
+        if qt[i] > 0:
+            step[i] = int(np.ceil(float((ac_range[i] + 1) << 1) / float(qt[i])))
+            shift[i] = (qt[i] - 2) // 2
+    q = np.zeros_like(coef)
+    q[..., 0] = coef[..., 0] // step[0]  # DC is non-negative
+    for i in range(1, 16):
+        if qt[i] > 0:
+            c = coef[..., i]
+            q[..., i] = np.sign(c) * (np.abs(c) // step[i]) + shift[i]  # C division truncates
+    planes = []
+    qf = q.reshape(n, 16)
+    for band in range(16):
+        idx = ZIGZAG[band]
+        if qb[idx] <= 0:
+            break
+        for bit in range(qb[idx] - 1, -1, -1):
+            planes.append(((qf[:, idx] >> bit) & 1).astype(np.uint8))
+    planes = np.array(planes, np.uint8)
+    accs, crcs = [], []
+    for p in planes:
+        a, c = ldpca_encode(L, p)
+        accs.append(a)
+        crcs.append(c)
+    return WZFrame(qindex, ac_range.astype(np.int32), np.array(crcs, np.uint8), np.array(accs, np.uint8), planes)
+
+
+# ----------------------------------------------------------------- LDPCA set
+def ldpca_bitplane_set(n_cif=384, n_qcif=128, seed=2, delta=4.0):
+    """Seeded bitplanes with Laplacian soft inputs.
+
+    Per bitplane: p ~ U[0.02, 0.5], bits ~ Bernoulli(p); alpha ~ U[0.05, 2.0];
+    side information y = b*delta + Laplace(0, 1/alpha); the decoder input is
+    the likelihood ratio LR = P(b=0|y)/P(b=1|y) =
+    exp(alpha*(|y - delta| - |y|)) * (1-p)/p, computed in double, clipped to
+    the range condBitProb produces (noiseModel.cpp:301-306) and stored as
+    float32.  Returns a list of dicts (N, bits, LR, acc, crc, p, alpha) with
+    CIF and QCIF bitplanes interleaved.
+    """
+    rng = np.random.default_rng(seed)
+    sizes = [6336] * n_cif + [1584] * n_qcif
+    order = rng.permutation(len(sizes))
+    out = []
+    lo, hi = np.float32(0.0001 / 0.9999), np.float32(0.9999 / 0.0001)
+    for j in order:
+        n = sizes[j]
+        p = rng.uniform(0.02, 0.5)
+        alpha = rng.uniform(0.05, 2.0)
+        bits = (rng.random(n) < p).astype(np.uint8)
+        y = bits * delta + rng.laplace(0.0, 1.0 / alpha, n)
+        llr = alpha * (np.abs(y - delta) - np.abs(y)) + np.log((1.0 - p) / p)
+        lr = np.exp(np.clip(llr, -30, 30)).astype(np.float32)
+        lr = np.clip(lr, lo, hi).astype(np.float32)
+        acc, crc = ldpca_encode(ladder(n), bits)
+        out.append(dict(N=n, bits=bits, LR=lr, acc=acc, crc=crc, p=p, alpha=alpha))
+    return out

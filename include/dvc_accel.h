@@ -1,0 +1,106 @@
+/* dvc_accel.h -- C ABI of the MI355X (gfx950) Wyner-Ziv decoding hot path.
+ *
+ * The reference decoder (jcluox/dvc, DvcDec) reaches its accelerator through a
+ * set of extern "C" functions over file-scope global state, selected by
+ * codingOption.ParallelMode 2/3:
+ *   LDPCA   DvcDec/inc/LDPCA_cuda.h:16-21      InitCUDA, LDPCA_CUDA_Init_Variable,
+ *                                              beliefPropagation_CUDA(_earlyJump)
+ *   CNM     DvcDec/inc/noiseModel_kernel.h:13-19 CNM_CUDA_Init_Buffer, noiseDistribution_CUDA,
+ *                                              condBitProb_CUDA
+ *   SI      DvcDec/inc/updateSI_kernel.h:13-17, forwardME_kernel.h:11-13
+ *                                              updateSI_CUDA, forwardME_CUDA
+ *   SMF     DvcDec/inc/motionLearning_kernel.h:11-13 updateSMF_CUDA
+ * This header replaces that set with the same operations around an opaque
+ * context, so several frames and streams can be decoded at once.  All
+ * functions return 0 on success and a negative DVC_E* code on failure (never
+ * exit()); dvc_last_error() describes the last failure of the calling thread.
+ * Host pointers are caller-owned and only read or written during the call.
+ * A context is used by one host thread at a time.
+ */
+#ifndef DVC_ACCEL_H
+#define DVC_ACCEL_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DVC_OK 0
+#define DVC_EINVAL (-1)   /* bad argument (size, code length, pointer) */
+#define DVC_EDEVICE (-2)  /* HIP runtime error or no gfx950 device */
+#define DVC_EIO (-3)      /* ladder pack or cache file missing/corrupt */
+#define DVC_ENOMEM (-4)   /* device or host allocation failed */
+#define DVC_ESTATE (-5)   /* call out of order (e.g. frame state not set up) */
+
+typedef struct dvc_ctx dvc_ctx;
+
+typedef struct {
+  int device;            /* HIP device ordinal (reference: -GPUid, configfile.cpp:143-145) */
+  int max_iter;          /* BP iteration cap; the reference's MAX_ITERATIONS is 50
+                            (LdpcaDecoder.h:4); <= 0 selects 50 */
+  const char *data_dir;  /* directory with reg1584.ladder / reg6336.ladder; NULL = the
+                            library's own data directory */
+  int bp_threads;        /* threads per BP workgroup for N=6336 (0 = autotuned choice) */
+} dvc_params;
+
+/* InitCUDA + LDPCA_CUDA_Init_Variable (LDPCA_cuda.cu:36-89, 546-763): selects the
+ * device and makes both ladders (N=1584 and N=6336, 65 codes each) and their
+ * full-rate inverses resident. */
+int dvc_ctx_create(const dvc_params *params, dvc_ctx **out);
+/* LDPCA_CUDA_Free_Variable and the *_Free_Buffer calls. */
+int dvc_ctx_destroy(dvc_ctx *ctx);
+const char *dvc_last_error(void);
+/* Host-only: checks the ladder packs in data_dir (NULL = the library's own
+ * data directory) and writes the cached full-rate inverses (reg<N>.invh) if
+ * they are missing.  Needs no GPU. */
+int dvc_prepare_data(const char *data_dir);
+
+/* Ladder facts for code length N: number of codes and m = syndLength[c]
+ * (ReadLdpcaLadderFile_LdpcaDecoder.cpp:182). synd_length may be NULL. */
+int dvc_ladder_info(dvc_ctx *ctx, int N, int *num_codes, int *num_edges, int *synd_length);
+/* Copies the full-rate inverse (N rows of 4*ceil(N/32) bytes, MSB-first, the
+ * Inverse_Matrix_H_Reg<N>.bin layout) and decoder rows of code c (E ints). */
+int dvc_ladder_export(dvc_ctx *ctx, int N, int code, int *ir_out, uint8_t *inv_out);
+
+typedef struct {
+  int start_level;   /* nCodeStart: first code with rate >= H(X|Y) (LdpcaDecoder.cpp:657-659) */
+  int level;         /* accepted code index */
+  int rate;          /* m + 8 bits (syndrome + CRC, LdpcaDecoder.cpp:681) */
+  int requests;      /* codes tried = level - start_level + 1 */
+  int iterations;    /* BP iterations summed over the codes tried (sequential search) */
+  int used_inverse;  /* 1 when the full-rate GF(2) inverse produced the bitplane */
+  float entropy;     /* conditional entropy H(X|Y) used for start_level */
+} dvc_bp_stats;
+
+/* One bitplane, sequential rate search: LdpcaDecoder (LdpcaDecoder.cpp:646-771)
+ * with beliefPropagation_CUDA (LDPCA_cuda.cu:972-1031) per code.
+ *   LR          N likelihood ratios p0/p1 (host memory), as condBitProb writes them
+ *   acc         N/8 accumulated-syndrome bytes, MSB first (WZ bitstream layout)
+ *   crc         the encoder's CRC-8 of the bitplane
+ *   start_level -1: computed on the device from ComputeCE (LdpcaDecoder.cpp:127-143);
+ *               >= 0: used as given
+ *   decoded     N output bytes (0/1)
+ * BP follows ParallelMode 0 semantics (beliefPropagation, LdpcaDecoder.cpp:462-580). */
+int dvc_ldpca_decode(dvc_ctx *ctx, int N, const float *LR, const uint8_t *acc, uint8_t crc,
+                     int start_level, uint8_t *decoded, dvc_bp_stats *stats);
+
+typedef struct {
+  int N;                  /* 1584 or 6336 */
+  const float *LR;        /* N floats */
+  const uint8_t *acc;     /* N/8 bytes */
+  uint8_t crc;
+  int start_level;        /* -1: from ComputeCE on the device */
+  uint8_t *decoded;       /* N bytes out */
+  dvc_bp_stats stats;     /* out (iterations = those of the accepted code and the
+                             codes below it, as the sequential search would run) */
+} dvc_ldpca_item;
+
+/* Many bitplanes of mixed code length in one persistent launch; every code
+ * from start_level upward is decoded speculatively and the smallest code
+ * whose BP converges and passes the CRC is kept, which is the code the
+ * sequential search selects.  Host pointers. */
+int dvc_ldpca_decode_batch(dvc_ctx *ctx, int n, dvc_ldpca_item *items);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,258 @@
+// ref_harness.cpp -- a thin driver around the reference decoder's own CPU
+// sources (DvcDec/src/*.cpp, compiled from /root/reference by oracle/Makefile
+// into oracle/_ref/libdvcref.so).  It sets the reference's global state for
+// one frame geometry and exposes its functions with plain-pointer C entry
+// points, so tests can check the oracle's restatement against the reference
+// on seeded inputs.  TEST INFRASTRUCTURE ONLY; built only where the reference
+// tree exists.
+//
+// The reference's LDPCA decoder is reached through -Wl,--wrap=LdpcaDecoder and
+// -Wl,--wrap=ComputeCE(_OpenMP) so that decodeGOP's per-bitplane results (rate,
+// entropy) can be recorded without editing the reference.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+#include "global.h"
+#include "LdpcaDecoder.h"
+#include "ReadLdpcaLadderFile_LdpcaDecoder.h"
+#include "sideInfoCreation.h"
+#include "filter.h"
+#include "noiseModel.h"
+#include "motionLearning.h"
+#include "transform.h"
+#include "reconstruction.h"
+#include "bitplane.h"
+#include "frame.h"
+#include "DVC_Decoder.h"
+
+// Iteration counter that the reference's beliefPropagation_OpenMP increments
+// (declared extern at LdpcaDecoder.cpp:30 and frame.cpp:10); the harness owns
+// it and reports it through ref_bp_iterations().
+int LDPC_iterations = 0;
+
+static int g_nframes = 0;
+static unsigned char *g_flags = NULL;  // all-WZ block-mode map
+static int g_log_rate[4096], g_log_level[4096], g_log_n = 0;
+static float g_log_entropy[4096];
+static int g_ent_n = 0;
+
+extern "C" {
+void __real_LdpcaDecoder(float *, unsigned char *, unsigned char *, int *, unsigned char, float);
+float __real_ComputeCE(float *);
+float __real_ComputeCE_OpenMP(float *);
+
+void __wrap_LdpcaDecoder(float *LR, unsigned char *acc, unsigned char *decoded, int *rate,
+                         unsigned char crc, float entropy) {
+  __real_LdpcaDecoder(LR, acc, decoded, rate, crc, entropy);
+  if (g_log_n < 4096) {
+    g_log_rate[g_log_n] = *rate;
+    int lvl = -1;
+    for (int c = 0; c < numCodes; c++)
+      if (syndLength[c] + 8 == *rate) { lvl = c; break; }
+    g_log_level[g_log_n] = lvl;
+    g_log_n++;
+  }
+}
+float __wrap_ComputeCE(float *p) {
+  float e = __real_ComputeCE(p);
+  if (g_ent_n < 4096) g_log_entropy[g_ent_n++] = e;
+  return e;
+}
+float __wrap_ComputeCE_OpenMP(float *p) {
+  float e = __real_ComputeCE_OpenMP(p);
+  if (g_ent_n < 4096) g_log_entropy[g_ent_n++] = e;
+  return e;
+}
+
+int ref_setup(int W, int H, int qindex, int mode, int nthreads, int nframes) {
+  memset(&codingOption, 0, sizeof(codingOption));
+  codingOption.FrameWidth = W;
+  codingOption.FrameHeight = H;
+  codingOption.FramesToBeDecoded = nframes;
+  codingOption.ParallelMode = mode;
+  codingOption.UseCoreNum = nthreads;
+  codingOption.Qindex = (unsigned char)qindex;
+  codingOption.GOP = 2;
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  FrameSize = W * H;
+  Block44Num = FrameSize / 16;
+  Block88Num = Block44Num / 4;
+  Block1616Num = Block88Num / 4;
+  Block44Height = H / 4; Block44Width = W / 4;
+  Block88Height = H / 8; Block88Width = W / 8;
+  Block1616Height = H / 16; Block1616Width = W / 16;
+  Qtable = QuantizationTable[qindex];
+  QtableBits = QuantizationTableBits[qindex];
+  QbitsNumPerBlk = 0;
+  for (int j = 0; j < 16; j++) QbitsNumPerBlk += QtableBits[j];
+  for (int j = 0; j < 16; j++) ACLevelShift[j] = (j > 0 && Qtable[j] > 0) ? (Qtable[j] - 2) / 2 : 0;
+  g_nframes = nframes;
+  initFrameBuffer();
+  init_1616ME(SearchInfo1616);
+  initTransform();
+  encodedFrames = (EncodedFrame *)calloc(nframes, sizeof(EncodedFrame));
+  for (int i = 0; i < nframes; i++) {
+    decodedFrames[i].hasReferenced = 0;
+    decodedFrames[i].isWZFrame = 0;
+    decodedFrames[i].bitRate = 0;
+  }
+  free(g_flags);
+  g_flags = (unsigned char *)calloc(Block88Num, 1);
+  return 0;
+}
+
+int ref_load_ladder(const char *lad, const char *inv) {
+  ReadLadder_decoder((char *)lad, (char *)inv);
+  initLdpcaBuffer();
+  return 0;
+}
+
+int ref_num_codes(void) { return numCodes; }
+int ref_synd_length(int c) { return syndLength[c]; }
+int ref_ir(int c, int e) { return ir[c][e]; }
+int ref_bp_iterations(void) { return LDPC_iterations; }
+
+void ref_set_frame(int idx, const unsigned char *pix, const unsigned char *orig) {
+  memcpy(decodedFrames[idx].reconsFrame, pix, FrameSize);
+  if (orig) memcpy(oriFrames[idx], orig, FrameSize);
+  decodedFrames[idx].hasReferenced = 0;
+}
+
+void ref_set_encoded(int idx, const int *acRange, const unsigned char *crc, const unsigned char *acc) {
+  EncodedFrame *e = &encodedFrames[idx];
+  memcpy(e->ACRange, acRange, sizeof(int) * 16);
+  e->BlockModeFlag = (unsigned char *)calloc(Block88Num, 1);
+  e->skipMode = 0; e->intraMode = 0; e->skipBlockBits = 1; e->intraBlockBits = 1;
+  e->CRC = (unsigned char *)malloc(QbitsNumPerBlk);
+  memcpy(e->CRC, crc, QbitsNumPerBlk);
+  e->AccumulatedSyndrome = (unsigned char **)malloc(sizeof(unsigned char *) * QbitsNumPerBlk);
+  int nb = Block44Num / 8;
+  for (int j = 0; j < QbitsNumPerBlk; j++) {
+    e->AccumulatedSyndrome[j] = (unsigned char *)malloc(nb);
+    memcpy(e->AccumulatedSyndrome[j], acc + (size_t)j * nb, nb);
+  }
+}
+
+// decodeGOP (DVC_Decoder.cpp:26-474) for one (begin, end) pair.
+int ref_decode_gop(int begin, int end) {
+  g_log_n = 0; g_ent_n = 0;
+  decodeGOP(begin, end);
+  return g_log_n;
+}
+void ref_get_frame(int idx, unsigned char *out) { memcpy(out, decodedFrames[idx].reconsFrame, FrameSize); }
+double ref_get_psnr(int idx) { return decodedFrames[idx].PSNR; }
+int ref_get_bitrate(int idx) { return decodedFrames[idx].bitRate; }
+int ref_get_log(int *rates, int *levels, float *entropy) {
+  memcpy(rates, g_log_rate, sizeof(int) * g_log_n);
+  memcpy(levels, g_log_level, sizeof(int) * g_log_n);
+  memcpy(entropy, g_log_entropy, sizeof(float) * g_ent_n);
+  return g_log_n;
+}
+
+// ---- function-level entry points
+int ref_bp(int mode, int code, float *LLR, unsigned char *synd, unsigned char *decoded) {
+  int m = syndLength[code];
+  return mode == 0 ? beliefPropagation(LLR, synd, decoded, m, code)
+                   : beliefPropagation_OpenMP(LLR, synd, decoded, m, code);
+}
+void ref_ldpca(float *LR, unsigned char *acc_bits, unsigned char crc, float entropy,
+               unsigned char *decoded, int *rate) {
+  __real_LdpcaDecoder(LR, acc_bits, decoded, rate, crc, entropy);
+}
+float ref_compute_ce(float *LR) { return __real_ComputeCE(LR); }
+float ref_compute_ce_omp(float *LR) { return __real_ComputeCE_OpenMP(LR); }
+
+void ref_lowpass(const unsigned char *in, unsigned char *out) {
+  lowPassFilter((unsigned char *)in, codingOption.FrameWidth, codingOption.FrameHeight, out);
+}
+void ref_fir(const unsigned char *in, unsigned char *out) {
+  FIR_filter((unsigned char *)in, codingOption.FrameWidth, codingOption.FrameHeight, out);
+}
+
+// createSideInfo (sideInfoCreation.cpp:438-551): key frames must be set with
+// ref_set_frame.  mbmv / bmv receive 6 doubles per MV (past, future, vector).
+void ref_create_side_info(int past, int fut, int cur, unsigned char *si, int *residue, double *mbmv,
+                          double *bmv) {
+  createSideInfo(past, fut, cur, &sideInfo);
+  memcpy(si, sideInfo.sideInfoFrame, FrameSize);
+  memcpy(residue, sideInfo.residueFrame, sizeof(int) * FrameSize);
+  if (mbmv) memcpy(mbmv, sideInfo.mcroBlockMV, sizeof(MV) * Block1616Num);
+  if (bmv) memcpy(bmv, sideInfo.BlockMV, sizeof(MV) * Block88Num);
+}
+
+static SIR make_sir(void) {
+  SIR sir;
+  sir.searchInfo = (SearchInfo *)malloc(sizeof(SearchInfo) * Block44Num);
+  initSIR(&sir, g_flags);
+  return sir;
+}
+
+// updateSI (sideInfoCreation.cpp:642-736) after ref_create_side_info set the
+// interpolated key frames.  si is read and written.
+void ref_update_si(int cur, int past, int fut, const unsigned char *recons, unsigned char *si, int omp) {
+  SIR sir = make_sir();
+  memcpy(sideInfo.sideInfoFrame, si, FrameSize);
+  if (omp) updateSI_OpenMP(&sideInfo, (unsigned char *)recons, cur, past, fut, &sir);
+  else updateSI(&sideInfo, (unsigned char *)recons, cur, past, fut, &sir);
+  memcpy(si, sideInfo.sideInfoFrame, FrameSize);
+  free(sir.searchInfo);
+}
+
+void ref_forward_transform(const int *pix, int *out) { forwardTransform((int *)pix, out); }
+void ref_inverse_transform(const int *trans, unsigned char *recons) {
+  SIR sir = make_sir();
+  inverseTransform((int *)trans, recons, &sir);
+  free(sir.searchInfo);
+}
+void ref_noise_parameter(int *trans, float *alpha) { noiseParameter(trans, alpha, g_flags); }
+void ref_overcomplete(const unsigned char *pix, int *oct, int omp) {
+  if (omp) overCompleteTransform_OpenMP((unsigned char *)pix, (Trans *)oct, codingOption.FrameHeight - 4, codingOption.FrameWidth - 4);
+  else overCompleteTransform((unsigned char *)pix, (Trans *)oct, codingOption.FrameHeight - 4, codingOption.FrameWidth - 4);
+}
+int ref_init_smf(void *smf) {
+  SIR sir = make_sir();
+  initSMF((SMF *)smf, &sir);
+  int n = sir.searchCandidate;
+  free(sir.searchInfo);
+  return n;
+}
+void ref_init_si_ml(int *si_ml, const int *oct) { init_si_ML(si_ml, (Trans *)oct); }
+void ref_update_smf(void *smf, const unsigned char *recons, const unsigned char *si, int omp) {
+  SIR sir = make_sir();
+  if (omp) updateSMF_OpenMP((SMF *)smf, (unsigned char *)recons, (unsigned char *)si, &sir);
+  else updateSMF((SMF *)smf, (unsigned char *)recons, (unsigned char *)si, &sir);
+  free(sir.searchInfo);
+}
+void ref_update_si_ml(int *sideTrans, const int *oct, int band, void *smf, int omp) {
+  SIR sir = make_sir();
+  if (omp) update_si_ML_OpenMP(sideTrans, (Trans *)oct, band, (SMF *)smf, &sir);
+  else update_si_ML(sideTrans, (Trans *)oct, band, (SMF *)smf, &sir);
+  free(sir.searchInfo);
+}
+int ref_noise_distribution(int range, int index, float *alpha, void *smf, const int *oct, float *out, int omp) {
+  SIR sir = make_sir();
+  float *d = omp ? noiseDistribution_OpenMP(range, index, alpha, (SMF *)smf, (Trans *)oct, &sir)
+                 : noiseDistribution(range, index, alpha, (SMF *)smf, (Trans *)oct, &sir);
+  int per = index == 0 ? range : 2 * range + 1;
+  memcpy(out, d, sizeof(float) * (size_t)per * sir.searchCandidate);
+  free(d);
+  free(sir.searchInfo);
+  return per;
+}
+void ref_cond_bit_prob(float *LR, int index, int bit, int step, int *reconsTrans, int range, float *dist, int omp) {
+  SIR sir = make_sir();
+  if (omp) condBitProb_OpenMP(LR, index, bit, step, reconsTrans, range, dist, &sir);
+  else condBitProb(LR, index, bit, step, reconsTrans, range, dist, &sir);
+  free(sir.searchInfo);
+}
+void ref_put_bitplane(const unsigned char *bp, int *reconsTrans, int index) {
+  putBitplaneToCoeff((unsigned char *)bp, reconsTrans, index, g_flags);
+}
+void ref_reconstruction(int *reconsTrans, int *sideTrans, int index, float *alpha, int *stepSize) {
+  unsigned char *db = (unsigned char *)malloc(FrameSize);
+  reconstruction(reconsTrans, sideTrans, index, alpha, stepSize, g_flags, db);
+  free(db);
+}
+double ref_psnr(const unsigned char *a, const unsigned char *b) { return psnr((unsigned char *)a, (unsigned char *)b); }
+}  // extern "C"

@@ -1,0 +1,196 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and, where it was
+built, to the reference's own sources (oracle/_ref/*.so).  Test
+infrastructure: the checker the HIP path is compared against."""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+REF_DEC_SO = ROOT / "oracle" / "_ref" / "libdvcref.so"
+REF_ENC_SO = ROOT / "oracle" / "_ref" / "libdvcencref.so"
+REFERENCE = Path("/root/reference")
+
+c_int, c_float, c_double, c_void_p, c_uint8 = ctypes.c_int, ctypes.c_float, ctypes.c_double, ctypes.c_void_p, ctypes.c_uint8
+
+
+def P(a):
+    return a.ctypes.data_as(c_void_p)
+
+
+class OrcLadder(ctypes.Structure):
+    _fields_ = [("numCodes", c_int), ("N", c_int), ("E", c_int), ("period", c_int),
+                ("jc", c_void_p), ("bitsPerPeriod", c_void_p), ("idx", c_void_p), ("ir", c_void_p),
+                ("syndLength", c_void_p), ("codeRate", c_void_p), ("invH", c_void_p), ("invWords", c_int)]
+
+
+class OrcEncLadder(ctypes.Structure):
+    _fields_ = [("N", c_int), ("E", c_int), ("jc", c_void_p), ("ir", c_void_p)]
+
+
+class OrcStats(ctypes.Structure):
+    _fields_ = [("start_level", c_int), ("level", c_int), ("rate", c_int), ("requests", c_int),
+                ("total_iters", c_int), ("used_inverse", c_int)]
+
+
+class OrcGeom(ctypes.Structure):
+    _fields_ = [(n, c_int) for n in ("W", "H", "frameSize", "b44n", "b44w", "b44h", "b88n", "b88w", "b88h",
+                                     "b1616n", "b1616w", "b1616h")]
+
+
+class OrcOpts(ctypes.Structure):
+    _fields_ = [("mode", c_int), ("nthreads", c_int), ("max_iter", c_int)]
+
+
+class OrcFrameResult(ctypes.Structure):
+    _fields_ = [("nbitplanes", c_int), ("bitrate", c_int), ("levels", c_int * 128), ("start_levels", c_int * 128),
+                ("iters", c_int * 128), ("entropy", c_float * 128), ("psnr", c_double),
+                ("t_si", c_double), ("t_oct", c_double), ("t_cnm", c_double), ("t_condbit", c_double),
+                ("t_ldpca", c_double), ("t_updsi", c_double), ("t_ml", c_double), ("t_total", c_double)]
+
+
+SMF_DTYPE = np.dtype([("prob", "<f4", 81), ("searchTop", "<i4"), ("searchLeft", "<i4"), ("searchRight", "<i4"),
+                      ("searchBottom", "<i4"), ("searchWidth", "<i4"), ("blkIdx", "<i4"), ("probLen", "<i4")])
+SEARCH_INFO_DTYPE = np.dtype([(n, "<i4") for n in ("searchTop", "searchBottom", "searchLeft", "searchRight", "top",
+                                                     "left", "searchWidth", "searchRange", "blkIdx")])
+MV_DTYPE = np.dtype([("past", "<f8", 2), ("future", "<f8", 2), ("vector", "<f8", 2)])
+
+_orc = None
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        if not ORACLE_SO.exists():
+            raise FileNotFoundError(f"{ORACLE_SO} missing: run `make -C oracle` (or __graft_entry__.build())")
+        _orc = ctypes.CDLL(str(ORACLE_SO))
+        _orc.orc_compute_ce.restype = c_float
+        _orc.orc_compute_ce_omp.restype = c_float
+        _orc.orc_crc8.restype = c_uint8
+        _orc.orc_psnr.restype = c_double
+        _orc.orc_bp_mode0.restype = c_int
+        _orc.orc_bp_mode1.restype = c_int
+    return _orc
+
+
+def geom(w, h):
+    g = OrcGeom()
+    orc().orc_geom_init(ctypes.byref(g), w, h)
+    return g
+
+
+class Ladder:
+    """Decoder ladder held by the oracle (derived from the committed pack or
+    read from the reference's text files)."""
+
+    def __init__(self, n=None, lad_path=None, inv_path=None):
+        self.L = OrcLadder()
+        lib = orc()
+        if lad_path is not None:
+            rc = lib.orc_read_decoder_ladder(ctypes.byref(self.L), str(lad_path).encode(),
+                                             str(inv_path).encode() if inv_path else None)
+            assert rc == 0, f"cannot read {lad_path}"
+            self.enc = None
+        else:
+            from dvc_amd import workload
+            pk = workload.ladder(n)
+            self.enc = OrcEncLadder()
+            self._jc = np.ascontiguousarray(pk.jc, np.int32)
+            self._ir = np.ascontiguousarray(pk.ir, np.int32)
+            self.enc.N, self.enc.E = pk.N, pk.E
+            self.enc.jc, self.enc.ir = self._jc.ctypes.data, self._ir.ctypes.data
+            bpp = np.ascontiguousarray(pk.bits_per_period, np.int32)
+            idx = np.ascontiguousarray(pk.idx, np.int32)
+            rc = lib.orc_derive_decoder_ladder(ctypes.byref(self.L), ctypes.byref(self.enc), pk.num_codes, pk.period,
+                                               P(bpp), P(idx))
+            assert rc == 0
+            words = (pk.N + 31) // 32
+            inv = np.ctypeslib.as_array(ctypes.cast(self.L.invH, ctypes.POINTER(ctypes.c_uint8)),
+                                        (pk.N * 4 * words,))
+            rc = lib.orc_invert_h(ctypes.byref(self.enc), P(inv), words)
+            assert rc == 0
+        self.N, self.E, self.num_codes = self.L.N, self.L.E, self.L.numCodes
+
+    def ref(self):
+        return ctypes.byref(self.L)
+
+    def ir(self, code):
+        a = np.ctypeslib.as_array(ctypes.cast(self.L.ir, ctypes.POINTER(c_int)), (self.num_codes * self.E,))
+        return a[code * self.E:(code + 1) * self.E].copy()
+
+    def synd_length(self):
+        return np.ctypeslib.as_array(ctypes.cast(self.L.syndLength, ctypes.POINTER(c_int)), (self.num_codes,)).copy()
+
+    def inv(self):
+        words = (self.N + 31) // 32
+        return np.ctypeslib.as_array(ctypes.cast(self.L.invH, ctypes.POINTER(ctypes.c_uint8)),
+                                     (self.N * 4 * words,)).copy()
+
+
+_LADDERS = {}
+
+
+def ladder(n):
+    if n not in _LADDERS:
+        _LADDERS[n] = Ladder(n)
+    return _LADDERS[n]
+
+
+def compute_ce(lr):
+    lr = np.ascontiguousarray(lr, np.float32)
+    return float(orc().orc_compute_ce(P(lr), lr.size))
+
+
+def ldpca_decode(lr, acc_bytes, crc, mode=0, max_iter=50, nthreads=1, decoded_init=None, entropy=None):
+    """Oracle LdpcaDecoder: returns (decoded, stats dict, entropy)."""
+    lr = np.array(lr, np.float32)  # converted to LLR in place, keep the caller's copy
+    n = lr.size
+    L = ladder(n)
+    ent = compute_ce(lr) if entropy is None else entropy
+    acc_bits = np.unpackbits(np.asarray(acc_bytes, np.uint8))[:n].copy()
+    dec = np.full(n, 0xAA, np.uint8) if decoded_init is None else np.array(decoded_init, np.uint8)
+    st = OrcStats()
+    orc().orc_ldpca_decode(L.ref(), P(lr), P(acc_bits), c_uint8(int(crc)), c_float(ent), mode, max_iter, nthreads,
+                           P(dec), ctypes.byref(st))
+    return dec, {f: getattr(st, f) for f, _ in st._fields_}, ent
+
+
+def bp(lr_llr, synd, code, mode=0, max_iter=50, decoded_init=None, nthreads=1):
+    llr = np.ascontiguousarray(lr_llr, np.float32)
+    n = llr.size
+    L = ladder(n)
+    synd = np.ascontiguousarray(synd, np.uint8)
+    dec = np.full(n, 0xAA, np.uint8) if decoded_init is None else np.array(decoded_init, np.uint8)
+    it = c_int()
+    if mode == 0:
+        ok = orc().orc_bp_mode0(L.ref(), code, P(llr), P(synd), P(dec), max_iter, ctypes.byref(it))
+    else:
+        ok = orc().orc_bp_mode1(L.ref(), code, P(llr), P(synd), P(dec), max_iter, nthreads, ctypes.byref(it))
+    return ok, dec, it.value
+
+
+# ------------------------------------------------------------ reference libs
+_ref_dec = _ref_enc = None
+
+
+def have_reference_build():
+    return REF_DEC_SO.exists() and REF_ENC_SO.exists() and REFERENCE.exists()
+
+
+def ref_dec():
+    global _ref_dec
+    if _ref_dec is None:
+        _ref_dec = ctypes.CDLL(str(REF_DEC_SO))
+        _ref_dec.ref_compute_ce.restype = c_float
+        _ref_dec.ref_compute_ce_omp.restype = c_float
+        _ref_dec.ref_get_psnr.restype = c_double
+        _ref_dec.ref_psnr.restype = c_double
+    return _ref_dec
+
+
+def ref_enc():
+    global _ref_enc
+    if _ref_enc is None:
+        _ref_enc = ctypes.CDLL(str(REF_ENC_SO))
+    return _ref_enc
