@@ -1,0 +1,147 @@
+"""Benchmark of the Wyner-Ziv decoding hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Workloads
+  ldpca_batch   512 seeded bitplanes (384 x N=6336, 128 x N=1584) decoded in
+                one speculative persistent launch per step; one step = the
+                whole set.  value = BP edge-updates/s, counting the
+                iterations the sequential rate search runs (an edge-update is
+                one edge's check-to-variable plus variable-to-check update in
+                one iteration; E = 3N per code).
+
+Prints one JSON line (see the end of main()).  The CPU baseline leg runs the
+oracle's sequential decoder (ParallelMode 0 semantics, one core) on a bounded
+sample of the same bitplanes.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "WZ frames/sec and BP edge-updates/s (batched vs plain, 1 GPU); p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def bench_ldpca(args):
+    from dvc_amd import _lib, workload as W
+    t0 = time.time()
+    planes = W.ldpca_bitplane_set(n_cif=384, n_qcif=128, seed=2)
+    log(f"[bench] generated {len(planes)} bitplanes in {time.time() - t0:.1f}s")
+    lrs = [p["LR"] for p in planes]
+    accs = [p["acc"] for p in planes]
+    crcs = [p["crc"] for p in planes]
+    ctx = _lib.Context(device=0, max_iter=args.max_iter)
+    for i in range(args.warmup):
+        decs, sts = ctx.ldpca_decode_batch(lrs, accs, crcs)
+        log(f"[bench] warmup {i} done")
+    bad = sum(int(not np.array_equal(d, p["bits"])) for d, p in zip(decs, planes)) if args.warmup else 0
+    ctx.profile(True)
+    times = []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        decs, sts = ctx.ldpca_decode_batch(lrs, accs, crcs)
+        times.append(time.perf_counter() - t)
+        log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms")
+    bp_ms, bp_n = ctx.profile_get("bp_6336")
+    bpq_ms, bpq_n = ctx.profile_get("bp_1584")
+    ce_ms, ce_n = ctx.profile_get("ce_llr")
+    ctx.profile(False)
+    bad += sum(int(not np.array_equal(d, p["bits"])) for d, p in zip(decs, planes))
+    edge_updates = sum(st["iterations"] * 3 * p["N"] for st, p in zip(sts, planes))
+    t_step = float(np.mean(times))
+    value = edge_updates / t_step
+
+    # Roofline of the dominant kernel (bp_6336): algorithmic HBM bytes per
+    # launch = per accepted (bitplane, code) item 4N (LLR) + ceil(m/8)
+    # (syndrome) + ceil(N/8) (decisions) + 2E (row indices of the code),
+    # summed over every code the sequential search runs.
+    ladders = {n: W.ladder(n) for n in (1584, 6336)}
+    m_of = {n: (n // 66) * ladders[n].bits_per_period for n in ladders}
+    hbm_bytes = 0
+    lds_bytes = 0
+    for st, p in zip(sts, planes):
+        if p["N"] != 6336:
+            continue
+        n = p["N"]
+        for lvl in range(st["start_level"], st["level"] + 1):
+            if lvl == 64:
+                continue
+            hbm_bytes += 4 * n + (int(m_of[n][lvl]) + 7) // 8 + (n + 7) // 8 + 2 * 3 * n
+        lds_bytes += 16 * st["iterations"] * 3 * n
+    bp_avg_ms = bp_ms / max(bp_n, 1)
+    achieved = hbm_bytes / (bp_avg_ms * 1e-3) / 1e9 if bp_n else None
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "BP edge-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t_step * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded Bernoulli bitplanes, Laplacian soft inputs)",
+        "config": {"workload": "ldpca_batch: 512 bitplanes (384 N=6336 + 128 N=1584), all codes speculative, "
+                               f"max_iter={args.max_iter}", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "kernel": "bp_persistent_kernel<6336>", "kernel_ms_avg": bp_avg_ms,
+                     "lds_GBps": lds_bytes / (bp_avg_ms * 1e-3) / 1e9 if bp_n else None},
+        "kernels_ms": {"bp_6336": bp_ms / max(bp_n, 1), "bp_1584": bpq_ms / max(bpq_n, 1),
+                       "ce_llr": ce_ms / max(ce_n, 1)},
+        "wrong_bitplanes": bad,
+        "edge_updates_per_step": edge_updates,
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_ldpca(planes, args)
+    return res
+
+
+def cpu_baseline_ldpca(planes, args):
+    """Oracle (CPU restatement, ParallelMode 0, one core) on a bounded sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    sample = [p for p in planes if p["N"] == 6336][:args.cpu_planes]
+    t = time.perf_counter()
+    eu = 0
+    for p in sample:
+        _, st, _ = O.ldpca_decode(p["LR"], p["acc"], p["crc"], max_iter=args.max_iter)
+        eu += st["total_iters"] * 3 * p["N"]
+        if time.perf_counter() - t > args.cpu_seconds:
+            break
+    dt = time.perf_counter() - t
+    return {"value": eu / dt, "unit": "BP edge-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{len(sample)} N=6336 bitplanes of the same set (or {args.cpu_seconds}s cap), "
+                      f"sequential rate search, {dt:.1f}s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="ldpca_batch")
+    ap.add_argument("--max-iter", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-planes", type=int, default=12)
+    ap.add_argument("--cpu-seconds", type=float, default=25.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(streams shard across GPUs without any exchange; see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+    res = bench_ldpca(args)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -57,6 +57,8 @@ SIGNATURES = {
     "dvc_ctx_destroy": (_c.c_int, [_c.c_void_p]),
     "dvc_last_error": (_c.c_char_p, []),
     "dvc_prepare_data": (_c.c_int, [_c.c_char_p]),
+    "dvc_profile": (_c.c_int, [_c.c_void_p, _c.c_int]),
+    "dvc_profile_get": (_c.c_int, [_c.c_void_p, _c.c_char_p, _c.POINTER(_c.c_double), _c.POINTER(_c.c_long)]),
     "dvc_ladder_info": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_void_p]),
     "dvc_ladder_export": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_void_p, _c.c_void_p]),
     "dvc_ldpca_decode": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_uint8,
@@ -108,6 +110,14 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def profile(self, enable=True):
+        _check(lib.dvc_profile(self.h, 1 if enable else 0), "dvc_profile")
+
+    def profile_get(self, name):
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        _check(lib.dvc_profile_get(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)), "dvc_profile_get")
+        return ms.value, n.value
 
     def ladder_info(self, n):
         nc, ne = ctypes.c_int(), ctypes.c_int()

@@ -56,6 +56,52 @@ static int persistent_grid(int N, int threads) {
 
 using namespace dvc;
 
+hipEvent_t dvc_ctx::timer_begin(const char *name, hipStream_t s, int *slot) {
+  *slot = -1;
+  if (!profiling) return nullptr;
+  int k = -1;
+  for (size_t i = 0; i < timers.size(); i++)
+    if (timers[i].name == name) { k = (int)i; break; }
+  if (k < 0) { timers.push_back(KernelTimer{name}); k = (int)timers.size() - 1; }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  (void)hipEventRecord(e, s);
+  *slot = k;
+  return e;
+}
+
+void dvc_ctx::timer_end(int slot, hipEvent_t start, hipStream_t s) {
+  if (slot < 0 || !start) return;
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return;
+  (void)hipEventRecord(e, s);
+  timers[slot].pending.push_back({start, e});
+}
+
+void dvc_ctx::timer_collect() {
+  for (auto &t : timers) {
+    for (auto &pe : t.pending) {
+      float ms = 0.0f;
+      if (hipEventSynchronize(pe.second) == hipSuccess && hipEventElapsedTime(&ms, pe.first, pe.second) == hipSuccess) {
+        t.total_ms += ms;
+        t.launches++;
+      }
+      (void)hipEventDestroy(pe.first);
+      (void)hipEventDestroy(pe.second);
+    }
+    t.pending.clear();
+  }
+}
+
+// Timed launch helper: TIMED(ctx, "name", stream, expr-returning-hipError_t)
+#define TIMED(c, name, s, expr)                                  \
+  do {                                                           \
+    int slot_;                                                   \
+    hipEvent_t ev_ = (c)->timer_begin(name, s, &slot_);          \
+    DVC_HIP(expr);                                               \
+    (c)->timer_end(slot_, ev_, s);                               \
+  } while (0)
+
 extern "C" {
 
 const char *dvc_last_error(void) { return g_err.c_str(); }
@@ -107,6 +153,24 @@ int dvc_prepare_data(const char *data_dir) {
     int rc = load_ladder(dir, N, L);
     if (rc) return rc;
   }
+  return DVC_OK;
+}
+
+int dvc_profile(dvc_ctx *c, int enable) {
+  DVC_CHECK(c, DVC_EINVAL, "dvc_profile: NULL context");
+  c->timer_collect();
+  c->timers.clear();
+  c->profiling = enable != 0;
+  return DVC_OK;
+}
+
+int dvc_profile_get(dvc_ctx *c, const char *name, double *total_ms, long *launches) {
+  DVC_CHECK(c && name, DVC_EINVAL, "dvc_profile_get: NULL argument");
+  c->timer_collect();
+  *total_ms = 0.0;
+  *launches = 0;
+  for (auto &t : c->timers)
+    if (t.name == name) { *total_ms = t.total_ms; *launches = t.launches; }
   return DVC_OK;
 }
 
@@ -166,7 +230,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
   DVC_HIP(hipMemcpyAsync(dPlanes.p, planes.data(), sizeof(PlaneDesc) * n, hipMemcpyHostToDevice, s));
   DVC_HIP(hipMemcpyAsync(dState.p, state.data(), sizeof(PlaneState) * n, hipMemcpyHostToDevice, s));
   DVC_HIP(hipMemcpyAsync(dRec.p, recInit.data(), sizeof(int4) * recInit.size(), hipMemcpyHostToDevice, s));
-  DVC_HIP(launch_ce_llr(dPlanes.p, n, 1, c->lad1584, c->lad6336, s));
+  TIMED(c, "ce_llr", s, launch_ce_llr(dPlanes.p, n, 1, c->lad1584, c->lad6336, s));
   DVC_HIP(hipMemcpyAsync(state.data(), dState.p, sizeof(PlaneState) * n, hipMemcpyDeviceToHost, s));
   DVC_HIP(hipStreamSynchronize(s));
 
@@ -184,7 +248,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
         DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
         BpArgs a{ld, dPlanes.p + i, 1, dCounters.p, 1, 1, c->max_iter, dSlots.p, dCounters.p + 1, slotCap,
                  dRec.p + (size_t)i * C};
-        DVC_HIP(launch_bp_any(N, threads, a, 1, s));
+        TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, 1, s));
         int4 r;
         DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + level, sizeof(r), hipMemcpyDeviceToHost, s));
         DVC_HIP(hipStreamSynchronize(s));
@@ -198,7 +262,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
         int zero[2] = {0, 0};
         DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
         DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
-        DVC_HIP(launch_invh(dPlanes.p + i, 1, ld, dSlots.p, dCounters.p + 1, slotCap, dRec.p + (size_t)i * C, s));
+        TIMED(c, "invh", s, launch_invh(dPlanes.p + i, 1, ld, dSlots.p, dCounters.p + 1, slotCap, dRec.p + (size_t)i * C, s));
         int4 r;
         DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + (C - 1), sizeof(r), hipMemcpyDeviceToHost, s));
         DVC_HIP(hipStreamSynchronize(s));
@@ -212,8 +276,8 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
     const int maxOff = C - 1;
     BpArgs a{ld, dPlanes.p, n, dCounters.p, n * maxOff, maxOff, c->max_iter, dSlots.p, dCounters.p + 1, slotCap, dRec.p};
     int grid = std::min(persistent_grid(N, threads), std::max(1, n * maxOff));
-    DVC_HIP(launch_bp_any(N, threads, a, grid, s));
-    DVC_HIP(launch_invh(dPlanes.p, n, ld, dSlots.p, dCounters.p + 1, slotCap, dRec.p, s));
+    TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
+    TIMED(c, "invh", s, launch_invh(dPlanes.p, n, ld, dSlots.p, dCounters.p + 1, slotCap, dRec.p, s));
     int counters[2];
     DVC_HIP(hipMemcpyAsync(rec.data(), dRec.p, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost, s));
     DVC_HIP(hipMemcpyAsync(state.data(), dState.p, sizeof(PlaneState) * n, hipMemcpyDeviceToHost, s));

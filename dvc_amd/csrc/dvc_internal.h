@@ -55,7 +55,22 @@ void free_ladder(Ladder &L);
 
 }  // namespace dvc
 
+// Per-kernel timing with HIP events recorded on the launching stream.
+struct KernelTimer {
+  std::string name;
+  double total_ms = 0.0;
+  long launches = 0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
 struct dvc_ctx {
+  bool profiling = false;
+  std::vector<KernelTimer> timers;
+  // Records start/stop events around a launch when profiling is on.
+  hipEvent_t timer_begin(const char *name, hipStream_t s, int *slot);
+  void timer_end(int slot, hipEvent_t start, hipStream_t s);
+  void timer_collect();
+
   int device = 0;
   int max_iter = 50;
   int bp_threads_cif = 0;

@@ -54,6 +54,15 @@ const char *dvc_last_error(void);
  * they are missing.  Needs no GPU. */
 int dvc_prepare_data(const char *data_dir);
 
+/* Kernel timing (no counterpart in the reference, which times whole
+ * components with gettimeofday, global.cpp:60-76): with profiling enabled the
+ * library records HIP events around each kernel launch on the stream it runs
+ * on.  dvc_profile(ctx, 1) resets and enables, dvc_profile(ctx, 0) disables;
+ * dvc_profile_get sums the elapsed time of every launch of kernel `name`
+ * ("bp_6336", "bp_1584", "ce_llr", "invh", ...) since the reset. */
+int dvc_profile(dvc_ctx *ctx, int enable);
+int dvc_profile_get(dvc_ctx *ctx, const char *name, double *total_ms, long *launches);
+
 /* Ladder facts for code length N: number of codes and m = syndLength[c]
  * (ReadLdpcaLadderFile_LdpcaDecoder.cpp:182). synd_length may be NULL. */
 int dvc_ladder_info(dvc_ctx *ctx, int N, int *num_codes, int *num_edges, int *synd_length);

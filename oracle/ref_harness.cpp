@@ -37,14 +37,19 @@ static int g_log_rate[4096], g_log_level[4096], g_log_n = 0;
 static float g_log_entropy[4096];
 static int g_ent_n = 0;
 
+// The wrapped functions are C++ (mangled) symbols; the linker's --wrap works on
+// those names, so the wrapper and the real entry carry the mangled suffixes.
+#define REAL_LDPCA __real__Z12LdpcaDecoderPfPhS0_Pihf
+#define REAL_CE __real__Z9ComputeCEPf
+#define REAL_CE_OMP __real__Z16ComputeCE_OpenMPPf
 extern "C" {
-void __real_LdpcaDecoder(float *, unsigned char *, unsigned char *, int *, unsigned char, float);
-float __real_ComputeCE(float *);
-float __real_ComputeCE_OpenMP(float *);
+void REAL_LDPCA(float *, unsigned char *, unsigned char *, int *, unsigned char, float);
+float REAL_CE(float *);
+float REAL_CE_OMP(float *);
 
-void __wrap_LdpcaDecoder(float *LR, unsigned char *acc, unsigned char *decoded, int *rate,
+void __wrap__Z12LdpcaDecoderPfPhS0_Pihf(float *LR, unsigned char *acc, unsigned char *decoded, int *rate,
                          unsigned char crc, float entropy) {
-  __real_LdpcaDecoder(LR, acc, decoded, rate, crc, entropy);
+  REAL_LDPCA(LR, acc, decoded, rate, crc, entropy);
   if (g_log_n < 4096) {
     g_log_rate[g_log_n] = *rate;
     int lvl = -1;
@@ -54,13 +59,13 @@ void __wrap_LdpcaDecoder(float *LR, unsigned char *acc, unsigned char *decoded, 
     g_log_n++;
   }
 }
-float __wrap_ComputeCE(float *p) {
-  float e = __real_ComputeCE(p);
+float __wrap__Z9ComputeCEPf(float *p) {
+  float e = REAL_CE(p);
   if (g_ent_n < 4096) g_log_entropy[g_ent_n++] = e;
   return e;
 }
-float __wrap_ComputeCE_OpenMP(float *p) {
-  float e = __real_ComputeCE_OpenMP(p);
+float __wrap__Z16ComputeCE_OpenMPPf(float *p) {
+  float e = REAL_CE_OMP(p);
   if (g_ent_n < 4096) g_log_entropy[g_ent_n++] = e;
   return e;
 }
@@ -158,10 +163,10 @@ int ref_bp(int mode, int code, float *LLR, unsigned char *synd, unsigned char *d
 }
 void ref_ldpca(float *LR, unsigned char *acc_bits, unsigned char crc, float entropy,
                unsigned char *decoded, int *rate) {
-  __real_LdpcaDecoder(LR, acc_bits, decoded, rate, crc, entropy);
+  REAL_LDPCA(LR, acc_bits, decoded, rate, crc, entropy);
 }
-float ref_compute_ce(float *LR) { return __real_ComputeCE(LR); }
-float ref_compute_ce_omp(float *LR) { return __real_ComputeCE_OpenMP(LR); }
+float ref_compute_ce(float *LR) { return REAL_CE(LR); }
+float ref_compute_ce_omp(float *LR) { return REAL_CE_OMP(LR); }
 
 void ref_lowpass(const unsigned char *in, unsigned char *out) {
   lowPassFilter((unsigned char *)in, codingOption.FrameWidth, codingOption.FrameHeight, out);

@@ -194,3 +194,82 @@ def ref_enc():
     if _ref_enc is None:
         _ref_enc = ctypes.CDLL(str(REF_ENC_SO))
     return _ref_enc
+
+
+# ------------------------------------------------------------ whole frames
+def decode_wz_frame(past, fut, orig, wz, qindex, mode=0, nthreads=1, max_iter=50, idx=(0, 1, 2)):
+    """Oracle decodeGOP body for one WZ frame.  Returns (recons, result dict)."""
+    h, w = past.shape
+    g = geom(w, h)
+    n = (h // 4) * (w // 4)
+    L = ladder(n)
+    past = np.ascontiguousarray(past, np.uint8)
+    fut = np.ascontiguousarray(fut, np.uint8)
+    orig = np.ascontiguousarray(orig, np.uint8)
+    acr = np.ascontiguousarray(wz.ac_range, np.int32)
+    crc = np.ascontiguousarray(wz.crc, np.uint8)
+    acc = np.ascontiguousarray(wz.acc, np.uint8)
+    rec = np.zeros((h, w), np.uint8)
+    opt = OrcOpts(mode, nthreads, max_iter)
+    res = OrcFrameResult()
+    rc = orc().orc_decode_wz_frame(ctypes.byref(g), qindex, L.ref(), P(past), P(fut), idx[0], idx[1], idx[2],
+                                   P(acr), P(crc), P(acc), P(orig), ctypes.byref(opt), P(rec), ctypes.byref(res))
+    assert rc == 0
+    nb = res.nbitplanes
+    out = dict(nbitplanes=nb, bitrate=res.bitrate, levels=list(res.levels[:nb]),
+               start_levels=list(res.start_levels[:nb]), iters=list(res.iters[:nb]),
+               entropy=list(res.entropy[:nb]), psnr=res.psnr,
+               times_ms=dict(si=res.t_si, oct=res.t_oct, cnm=res.t_cnm, condbit=res.t_condbit,
+                             ldpca=res.t_ldpca, updsi=res.t_updsi, ml=res.t_ml, total=res.t_total))
+    return rec, out
+
+
+def write_text_ladder(L, lad_path, inv_path):
+    """Writes an oracle ladder in the reference's RegDecoder text format and
+    its inverse in the packed binary layout (ReadLdpcaLadderFile_LdpcaDecoder.cpp:131-201)."""
+    jc = np.ctypeslib.as_array(ctypes.cast(L.L.jc, ctypes.POINTER(c_int)), (L.N + 1,))
+    bpp = np.ctypeslib.as_array(ctypes.cast(L.L.bitsPerPeriod, ctypes.POINTER(c_int)), (L.num_codes,))
+    idx = np.ctypeslib.as_array(ctypes.cast(L.L.idx, ctypes.POINTER(c_int)), (L.num_codes * L.L.period,))
+    with open(lad_path, "w") as f:
+        f.write(f"{L.num_codes} {L.N} {L.E} {L.L.period}\n")
+        f.write(" ".join(map(str, jc)) + "\n")
+        for c in range(L.num_codes):
+            b = int(bpp[c])
+            f.write(f"{b}\n" + " ".join(map(str, idx[c * L.L.period:c * L.L.period + b])) + "\n")
+            f.write(" ".join(map(str, L.ir(c))) + "\n")
+    Path(inv_path).write_bytes(L.inv().tobytes())
+
+
+class RefDecoder:
+    """The reference decoder's own sources (oracle/_ref/libdvcref.so) set up
+    for one frame geometry.  The reference keeps global state, so one geometry
+    per process: call setup() again to switch."""
+
+    def __init__(self, w, h, qindex, mode=0, nthreads=1, nframes=3, tmpdir="/tmp"):
+        self.lib = ref_dec()
+        self.w, self.h, self.q = w, h, qindex
+        self.lib.ref_setup(w, h, qindex, mode, nthreads, nframes)
+        n = (w // 4) * (h // 4)
+        if n == 1584:
+            lad = REFERENCE / "release/DvcDec/LDPCA_LadderFile/RegDecoder1584.lad"
+            inv = REFERENCE / "release/DvcDec/LDPCA_LadderFile/Inverse_Matrix_H_Reg1584.bin"
+        else:
+            lad, inv = Path(tmpdir) / "RegDecoder6336.lad", Path(tmpdir) / "Inverse_Matrix_H_Reg6336.bin"
+            if not (lad.exists() and inv.exists()):
+                write_text_ladder(ladder(6336), lad, inv)
+        self.lib.ref_load_ladder(str(lad).encode(), str(inv).encode())
+
+    def decode_wz_frame(self, past, fut, orig, wz):
+        h, w = past.shape
+        self.lib.ref_set_frame(0, P(np.ascontiguousarray(past)), P(np.ascontiguousarray(orig)))
+        self.lib.ref_set_frame(2, P(np.ascontiguousarray(fut)), P(np.ascontiguousarray(orig)))
+        self.lib.ref_set_frame(1, P(np.ascontiguousarray(orig)), P(np.ascontiguousarray(orig)))
+        self.lib.ref_set_encoded(1, P(np.ascontiguousarray(wz.ac_range, np.int32)),
+                                 P(np.ascontiguousarray(wz.crc, np.uint8)), P(np.ascontiguousarray(wz.acc, np.uint8)))
+        n = self.lib.ref_decode_gop(0, 2)
+        rates = np.zeros(4096, np.int32); levels = np.zeros(4096, np.int32); ent = np.zeros(4096, np.float32)
+        self.lib.ref_get_log(P(rates), P(levels), P(ent))
+        rec = np.zeros((h, w), np.uint8)
+        self.lib.ref_get_frame(1, P(rec))
+        return rec, dict(levels=levels[:n].tolist(), rates=rates[:n].tolist(), entropy=ent[:n].tolist(),
+                         psnr=self.lib.ref_get_psnr(1), bitrate=self.lib.ref_get_bitrate(1))

@@ -1,0 +1,70 @@
+"""LDPCA on the GPU against the oracle: decoded bits, accepted code, entropy,
+start code and iteration counts must be identical (ParallelMode 0 semantics),
+for the plain sequential path and the speculative batched path."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from dvc_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(dec, st, pl, odec, ost, oent):
+    assert np.array_equal(dec, odec), "decoded bitplane differs from the oracle"
+    assert np.array_equal(dec, pl["bits"]), "decoded bitplane differs from the source"
+    assert st["entropy"] == np.float32(oent)
+    assert st["start_level"] == ost["start_level"]
+    assert st["level"] == ost["level"]
+    assert st["rate"] == ost["rate"]
+    assert st["iterations"] == ost["total_iters"]
+    assert st["used_inverse"] == ost["used_inverse"]
+
+
+@pytest.mark.parametrize("n", [1584, 6336])
+def test_plain_matches_oracle(ctx, n):
+    planes = W.ldpca_bitplane_set(n_cif=6 if n == 6336 else 0, n_qcif=10 if n == 1584 else 0, seed=11)
+    for pl in planes:
+        dec, st = ctx.ldpca_decode(pl["LR"], pl["acc"], pl["crc"])
+        odec, ost, oent = O.ldpca_decode(pl["LR"], pl["acc"], pl["crc"])
+        print(n, st, ost, flush=True)
+        _check(dec, st, pl, odec, ost, oent)
+
+
+def test_batch_matches_oracle_mixed(ctx):
+    planes = W.ldpca_bitplane_set(n_cif=12, n_qcif=12, seed=12)
+    decs, sts = ctx.ldpca_decode_batch([p["LR"] for p in planes], [p["acc"] for p in planes],
+                                       [p["crc"] for p in planes])
+    for pl, dec, st in zip(planes, decs, sts):
+        odec, ost, oent = O.ldpca_decode(pl["LR"], pl["acc"], pl["crc"])
+        _check(dec, st, pl, odec, ost, oent)
+
+
+def test_batch_equals_plain(ctx):
+    planes = W.ldpca_bitplane_set(n_cif=8, n_qcif=8, seed=13)
+    decs, sts = ctx.ldpca_decode_batch([p["LR"] for p in planes], [p["acc"] for p in planes],
+                                       [p["crc"] for p in planes])
+    for pl, dec, st in zip(planes, decs, sts):
+        pdec, pst = ctx.ldpca_decode(pl["LR"], pl["acc"], pl["crc"])
+        assert np.array_equal(dec, pdec)
+        assert st == pst
+
+
+def test_forced_inverse_on_bad_crc(ctx):
+    """A wrong CRC makes every BP code fail the CRC test; the search ends at
+    the full-rate inverse (LdpcaDecoder.cpp:695-716), which is not
+    CRC-checked and still returns the source."""
+    pl = W.ldpca_bitplane_set(n_cif=0, n_qcif=1, seed=14)[0]
+    dec, st = ctx.ldpca_decode(pl["LR"], pl["acc"], (pl["crc"] + 1) & 0xFF)
+    odec, ost, _ = O.ldpca_decode(pl["LR"], pl["acc"], (pl["crc"] + 1) & 0xFF)
+    assert st["used_inverse"] == 1 and ost["used_inverse"] == 1
+    assert st["level"] == 64 and st["rate"] == 1584 + 8
+    assert np.array_equal(dec, pl["bits"]) and np.array_equal(dec, odec)
+    assert st["iterations"] == ost["total_iters"]
+
+
+def test_fixed_start_level(ctx):
+    pl = W.ldpca_bitplane_set(n_cif=0, n_qcif=1, seed=15)[0]
+    dec, st = ctx.ldpca_decode(pl["LR"], pl["acc"], pl["crc"], start_level=0)
+    assert st["start_level"] == 0
+    assert np.array_equal(dec, pl["bits"])
