@@ -50,6 +50,28 @@ class LdpcaItem(ctypes.Structure):
                 ("decoded", ctypes.c_void_p), ("stats", BpStats)]
 
 
+class FrameStats(ctypes.Structure):
+    _fields_ = [("nbitplanes", ctypes.c_int), ("bitrate", ctypes.c_int), ("levels", ctypes.c_int * 128),
+                ("start_levels", ctypes.c_int * 128), ("iterations", ctypes.c_int * 128),
+                ("used_inverse", ctypes.c_int * 128), ("entropy", ctypes.c_float * 128),
+                ("psnr", ctypes.c_double), ("recon_host_fixups", ctypes.c_int)]
+
+    def as_dict(self):
+        n = self.nbitplanes
+        return dict(nbitplanes=n, bitrate=self.bitrate, levels=list(self.levels[:n]),
+                    start_levels=list(self.start_levels[:n]), iterations=list(self.iterations[:n]),
+                    used_inverse=list(self.used_inverse[:n]), entropy=list(self.entropy[:n]), psnr=self.psnr,
+                    recon_host_fixups=self.recon_host_fixups)
+
+
+class WZFrameSpec(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("qindex", ctypes.c_int),
+                ("past_idx", ctypes.c_int), ("cur_idx", ctypes.c_int), ("fut_idx", ctypes.c_int),
+                ("past", ctypes.c_void_p), ("fut", ctypes.c_void_p), ("orig", ctypes.c_void_p),
+                ("ac_range", ctypes.c_void_p), ("crc", ctypes.c_void_p), ("acc", ctypes.c_void_p),
+                ("recons", ctypes.c_void_p), ("stats", ctypes.POINTER(FrameStats))]
+
+
 # Every symbol include/dvc_accel.h declares, with its ctypes signature.
 _c = ctypes
 SIGNATURES = {
@@ -64,6 +86,7 @@ SIGNATURES = {
     "dvc_ldpca_decode": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_uint8,
                                     _c.c_int, _c.c_void_p, _c.POINTER(BpStats)]),
     "dvc_ldpca_decode_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(LdpcaItem)]),
+    "dvc_decode_wz_frames": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WZFrameSpec), _c.c_int]),
 }
 for _name, (_res, _args) in SIGNATURES.items():
     _f = getattr(lib, _name)
@@ -143,6 +166,33 @@ class Context:
         _check(lib.dvc_ldpca_decode(self.h, n, _ptr(lr), _ptr(acc), int(crc), int(start_level), _ptr(out),
                                     ctypes.byref(st)), "dvc_ldpca_decode")
         return out, st.as_dict()
+
+    def decode_wz_frames(self, frames, batched=True):
+        """frames: list of dicts with past, fut (uint8 HxW), orig (optional),
+        wz (workload.WZFrame), and optional idx=(past, cur, fut).  Returns a
+        list of (recons uint8 HxW, stats dict)."""
+        k = len(frames)
+        specs = (WZFrameSpec * max(k, 1))()
+        stats = [FrameStats() for _ in range(k)]
+        keep, outs = [], []
+        for i, fr in enumerate(frames):
+            past = np.ascontiguousarray(fr["past"], np.uint8)
+            fut = np.ascontiguousarray(fr["fut"], np.uint8)
+            orig = None if fr.get("orig") is None else np.ascontiguousarray(fr["orig"], np.uint8)
+            wz = fr["wz"]
+            acr = np.ascontiguousarray(wz.ac_range, np.int32)
+            crc = np.ascontiguousarray(wz.crc, np.uint8)
+            acc = np.ascontiguousarray(wz.acc, np.uint8)
+            rec = np.zeros_like(past)
+            keep += [past, fut, orig, acr, crc, acc]
+            outs.append(rec)
+            pi, ci, fi = fr.get("idx", (0, 1, 2))
+            h, w = past.shape
+            specs[i] = WZFrameSpec(w, h, wz.qindex, pi, ci, fi, past.ctypes.data, fut.ctypes.data,
+                                   None if orig is None else orig.ctypes.data, acr.ctypes.data, crc.ctypes.data,
+                                   acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats[i]))
+        _check(lib.dvc_decode_wz_frames(self.h, k, specs, 1 if batched else 0), "dvc_decode_wz_frames")
+        return [(outs[i], stats[i].as_dict()) for i in range(k)]
 
     def ldpca_decode_batch(self, lrs, accs, crcs, start_levels=None):
         """Speculative batched decode of many bitplanes (mixed N allowed)."""

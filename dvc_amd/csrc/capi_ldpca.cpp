@@ -123,6 +123,7 @@ int dvc_ctx_create(const dvc_params *params, dvc_ctx **out) {
   c->device = p.device;
   c->max_iter = p.max_iter > 0 ? p.max_iter : 50;
   c->bp_threads_cif = p.bp_threads;
+  c->sm_count = prop.multiProcessorCount;
   std::string dir = p.data_dir ? std::string(p.data_dir) : default_data_dir();
   int rc;
   if ((rc = load_ladder(dir, 1584, c->lad1584)) || (rc = load_ladder(dir, 6336, c->lad6336)) ||
@@ -141,6 +142,8 @@ int dvc_ctx_destroy(dvc_ctx *c) {
   (void)hipSetDevice(c->device);
   free_ladder(c->lad1584);
   free_ladder(c->lad6336);
+  if (c->d_costFme) (void)hipFree(c->d_costFme);
+  if (c->d_costUsi) (void)hipFree(c->d_costUsi);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DVC_OK;
@@ -205,7 +208,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
   DevBuf<PlaneState> dState;
   DevBuf<int4> dRec;
   DevBuf<int> dCounters;  // [0] queue head, [1] slot count
-  const int slotCap = sequential ? 2 : std::max(4 * n, n + 64);
+  const int slotCap = sequential ? 2 : 65 * n + 1;  // each (bitplane, code) accepts at most once
   DVC_HIP(dLR.alloc((size_t)n * N));
   DVC_HIP(dLLR.alloc((size_t)n * N));
   DVC_HIP(dAcc.alloc((size_t)n * nb));
@@ -221,7 +224,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
     DVC_HIP(hipMemcpyAsync(dLR.p + (size_t)i * N, items[i].LR, sizeof(float) * N, hipMemcpyHostToDevice, s));
     DVC_HIP(hipMemcpyAsync(dAcc.p + (size_t)i * nb, items[i].acc, nb, hipMemcpyHostToDevice, s));
     planes[i] = PlaneDesc{N, dLR.p + (size_t)i * N, dLLR.p + (size_t)i * N, dAcc.p + (size_t)i * nb,
-                          (uint32_t)items[i].crc, dState.p + i};
+                          (uint32_t)items[i].crc, dState.p + i, dRec.p + (size_t)i * C};
     int st0 = items[i].start_level;
     DVC_CHECK(st0 < C, DVC_EINVAL, "dvc_ldpca: start_level beyond the ladder");
     state[i] = PlaneState{st0 >= 0 ? st0 : -1, C - 1, 0.0f, 0};
@@ -246,8 +249,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
         int zero[2] = {0, 0};
         DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
         DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
-        BpArgs a{ld, dPlanes.p + i, 1, dCounters.p, 1, 1, c->max_iter, dSlots.p, dCounters.p + 1, slotCap,
-                 dRec.p + (size_t)i * C};
+        BpArgs a{ld, dPlanes.p + i, 1, dCounters.p, 1, 1, c->max_iter, dSlots.p, dCounters.p + 1, slotCap};
         TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, 1, s));
         int4 r;
         DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + level, sizeof(r), hipMemcpyDeviceToHost, s));
@@ -262,7 +264,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
         int zero[2] = {0, 0};
         DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
         DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
-        TIMED(c, "invh", s, launch_invh(dPlanes.p + i, 1, ld, dSlots.p, dCounters.p + 1, slotCap, dRec.p + (size_t)i * C, s));
+        TIMED(c, "invh", s, launch_invh(dPlanes.p + i, 1, ld, dSlots.p, dCounters.p + 1, slotCap, s));
         int4 r;
         DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + (C - 1), sizeof(r), hipMemcpyDeviceToHost, s));
         DVC_HIP(hipStreamSynchronize(s));
@@ -274,10 +276,10 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
     int zero[2] = {0, 0};
     DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
     const int maxOff = C - 1;
-    BpArgs a{ld, dPlanes.p, n, dCounters.p, n * maxOff, maxOff, c->max_iter, dSlots.p, dCounters.p + 1, slotCap, dRec.p};
+    BpArgs a{ld, dPlanes.p, n, dCounters.p, n * maxOff, maxOff, c->max_iter, dSlots.p, dCounters.p + 1, slotCap};
     int grid = std::min(persistent_grid(N, threads), std::max(1, n * maxOff));
     TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
-    TIMED(c, "invh", s, launch_invh(dPlanes.p, n, ld, dSlots.p, dCounters.p + 1, slotCap, dRec.p, s));
+    TIMED(c, "invh", s, launch_invh(dPlanes.p, n, ld, dSlots.p, dCounters.p + 1, slotCap, s));
     int counters[2];
     DVC_HIP(hipMemcpyAsync(rec.data(), dRec.p, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost, s));
     DVC_HIP(hipMemcpyAsync(state.data(), dState.p, sizeof(PlaneState) * n, hipMemcpyDeviceToHost, s));

@@ -72,6 +72,8 @@ struct dvc_ctx {
   void timer_collect();
 
   int device = 0;
+  int sm_count = 256;
+  double *d_costFme = nullptr, *d_costUsi = nullptr;  // motion-cost factor tables
   int max_iter = 50;
   int bp_threads_cif = 0;
   dvc::Ladder lad1584, lad6336;

@@ -1,0 +1,858 @@
+// frame_kernels.hip -- side information, transform, correlation-noise-model,
+// motion-learning and reconstruction kernels of the WZ frame decode on gfx950.
+//
+// Each kernel restates one reference function (file:line of DvcDec/src/) with
+// the reference's operand order and precision: integer SADs, FP64 motion
+// costs and interpolation weights, FP32 noise-model sums in the reference's
+// summation order, and the glibc-identical expf of dvc_math.h.  The results
+// are bit-identical to the CPU path.  Work is laid out per frame of a batch:
+// blockIdx.y (or .z) selects the frame, the other grid dimensions its blocks.
+#include <float.h>
+#include "dvc_math.h"
+#include "frame_kernels.h"
+#include "ldpca_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace dvc {
+
+#define DVC_K 0.05f  // defines.h:10, promoted to double where it is used
+
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+
+// computeEndPoint (sideInfoCreation.cpp:14-22)
+__device__ __forceinline__ int end_point(double st, double vec, double scale, int lower, int upper) {
+  int e = (int)((st + vec * scale) + 0.5);
+  return e < lower ? lower : (e > upper ? upper : e);
+}
+// costFunc (sideInfoCreation.cpp:10-12)
+__device__ __forceinline__ double cost_func(double mad, double x, double y) {
+  return mad * (1.0 + (double)DVC_K * sqrt(x * x + y * y));
+}
+
+// Lexicographic (cost, index) minimum across a wave: the first candidate in
+// scan order among those of least cost, which is what a strict-< raster scan
+// returns.
+__device__ __forceinline__ void wave_argmin(double &cost, int &idx) {
+  for (int o = 32; o > 0; o >>= 1) {
+    double c2 = __shfl_xor(cost, o);
+    int i2 = __shfl_xor(idx, o);
+    if (c2 < cost || (c2 == cost && i2 < idx)) { cost = c2; idx = i2; }
+  }
+}
+
+// ------------------------------------------------------------ key frames
+// lowPassFilter (filter.cpp:214-233): sum/9 in double equals integer division.
+// FIR_filter (filter.cpp:55-117): the 6-tap (1,-5,20,20,-5,1)/32 half-pel
+// filter; its dyadic coefficients make (int)(sum + 0.5) then the [0,255]
+// clip equal to clip((S + 16) >> 5) (1-D) and clip((S + 512) >> 10) (2-D).
+__global__ void k_keyframe_filters(const uint8_t *in, uint8_t *lp, uint8_t *half, int W, int H) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+  const int W2 = 2 * W;
+  if (j >= W2) return;
+  if (i < H && j < W) {
+    int sum = 0;
+    for (int m = i - 1; m <= i + 1; m++) {
+      if (m < 0 || m >= H) continue;
+      for (int n = j - 1; n <= j + 1; n++) {
+        if (n < 0 || n >= W) continue;
+        sum += in[m * W + n];
+      }
+    }
+    lp[i * W + j] = (uint8_t)(sum / 9);
+  }
+  for (int ii = i; ii < 2 * H; ii += gridDim.y) {
+    const int sub = (j & 1) + ((ii & 1) << 1), y = ii >> 1, x = j >> 1;
+    const int tap[6] = {1, -5, 20, 20, -5, 1};
+    int v;
+    if (sub == 0) {
+      v = in[imin(H - 1, y) * W + imin(W - 1, x)];
+    } else if (sub == 1) {
+      int s = 0;
+      for (int q = 0; q < 6; q++) s += tap[q] * in[y * W + imax(0, imin(W - 1, x - 2 + q))];
+      v = (s + 16) >> 5;
+    } else if (sub == 2) {
+      int s = 0;
+      for (int q = 0; q < 6; q++) s += tap[q] * in[imax(0, imin(H - 1, y - 2 + q)) * W + x];
+      v = (s + 16) >> 5;
+    } else {
+      int s = 0;
+      for (int a = 0; a < 6; a++) {
+        const int row = imax(0, imin(H - 1, y - 2 + a)) * W;
+        int r = 0;
+        for (int q = 0; q < 6; q++) r += tap[q] * in[row + imax(0, imin(W - 1, x - 2 + q))];
+        s += tap[a] * r;
+      }
+      v = (s + 512) >> 10;
+    }
+    half[ii * W2 + j] = (uint8_t)imax(0, imin(255, v));
+  }
+}
+
+// ------------------------------------------------------------ forward ME
+// forwardME (sideInfoCreation.cpp:61-118): per macroblock of the future
+// low-pass frame, exhaustive search of the past low-pass frame over the
+// clipped +-32 window; cost SAD*(1 + K*|v|) in double, first minimum in raster
+// order.  One workgroup per macroblock; the window is staged in LDS.
+__global__ __launch_bounds__(256) void k_forward_me(const FrameDev *F, const int *list, CostTables ct) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int bw = f.W / 16, nmb = bw * (f.H / 16), mb = blockIdx.x;
+  if (mb >= nmb) return;
+  const int W = f.W, top = (mb / bw) * 16, left = (mb % bw) * 16;
+  const int sT = imax(0, top - 32), sB = imin(f.H - 16, top + 32);
+  const int sL = imax(0, left - 32), sR = imin(W - 16, left + 32);
+  const int ww = sR - sL + 1, wh = sB - sT + 1, winW = ww + 15, winH = wh + 15;
+  __shared__ uint8_t fb[256];
+  __shared__ uint8_t win[80 * 80];
+  __shared__ double s_cost[4];
+  __shared__ int s_idx[4];
+  const int tid = threadIdx.x;
+  fb[tid] = f.fut_lp[(top + (tid >> 4)) * W + left + (tid & 15)];
+  for (int i = tid; i < winW * winH; i += 256) win[i] = f.past_lp[(sT + i / winW) * W + sL + i % winW];
+  __syncthreads();
+  double best = DBL_MAX;
+  int bidx = 0x7fffffff;
+  for (int c = tid; c < ww * wh; c += 256) {
+    const int oy = c / ww, ox = c % ww;
+    int sad = 0;
+    for (int m = 0; m < 16; m++) {
+      const uint8_t *wr = win + (oy + m) * winW + ox;
+      const uint8_t *fr = fb + m * 16;
+#pragma unroll
+      for (int q = 0; q < 16; q++) sad += abs((int)fr[q] - (int)wr[q]);
+    }
+    const int dx = sL + ox - left, dy = sT + oy - top;
+    const double cost = (double)sad * ct.fme[(dy + 32) * 65 + dx + 32];
+    if (cost < best) { best = cost; bidx = c; }
+  }
+  wave_argmin(best, bidx);
+  if ((tid & 63) == 0) { s_cost[tid >> 6] = best; s_idx[tid >> 6] = bidx; }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < 4; w++)
+      if (s_cost[w] < best || (s_cost[w] == best && s_idx[w] < bidx)) { best = s_cost[w]; bidx = s_idx[w]; }
+    const int px = sL + bidx % ww, py = sT + bidx / ww;
+    MVd r;
+    r.px = (double)px; r.py = (double)py; r.fx = 0.0; r.fy = 0.0;
+    r.vx = (double)(px - left); r.vy = (double)(py - top);
+    f.fwd[mb] = r;
+    f.holes[2 * mb] = (double)left * f.iPC + (double)px * f.iFC;
+    f.holes[2 * mb + 1] = (double)top * f.iPC + (double)py * f.iFC;
+  }
+}
+
+// ------------------------------------------------------------ bidirectional ME chain
+// One bidirectional search step of bidirectME (sideInfoCreation.cpp:250-306)
+// for block blk, run by one wave; the block's range is bounded by its
+// neighbours' current MVs (up/left already refined, down/right not yet).
+__device__ void bidir_block(const FrameDev &f, MVd *mv, int blk, int bw, int bh, int blockLen, int lane) {
+  const int i = blk / bw, j = blk % bw, W2 = 2 * f.W, H2 = 2 * f.H, bl2 = blockLen * 2;
+  const int bot = H2 - bl2, right = W2 - bl2, y2 = i * bl2, x2 = j * bl2;
+  const int sT = (i - 1 >= 0) ? (int)mv[blk - bw].fy + bl2 : (int)mv[blk].fy;
+  const int sB = (i + 1 < bh) ? (int)mv[blk + bw].fy - bl2 : (int)mv[blk].fy;
+  const int sL = (j - 1 >= 0) ? (int)mv[blk - 1].fx + bl2 : (int)mv[blk].fx;
+  const int sR = (j + 1 < bw) ? (int)mv[blk + 1].fx - bl2 : (int)mv[blk].fx;
+  const int nx = sR - sL + 1, ny = sB - sT + 1;
+  const double scale = f.iPC / f.iFC;
+  double best = DBL_MAX;
+  int bidx = 0x7fffffff;
+  if (nx > 0 && ny > 0) {
+    for (int c = lane; c < nx * ny; c += 64) {
+      const int fy = sT + c / nx, fx = sL + c % nx;
+      const int py = end_point(y2, (double)(y2 - fy), scale, 0, bot);
+      const int px = end_point(x2, (double)(x2 - fx), scale, 0, right);
+      int mad = 0;
+      for (int m = 0; m < bl2; m += 2) {
+        const uint8_t *pr = f.past_half + (py + m) * W2 + px;
+        const uint8_t *fr = f.fut_half + (fy + m) * W2 + fx;
+        for (int q = 0; q < bl2; q += 2) mad += abs((int)fr[q] - (int)pr[q]);
+      }
+      const double cost = cost_func((double)mad, (double)(px - fx), (double)(py - fy));
+      if (cost < 10000000.0 && (cost < best)) { best = cost; bidx = c; }
+    }
+  }
+  wave_argmin(best, bidx);
+  if (lane == 0 && bidx != 0x7fffffff) {
+    const int fy = sT + bidx / nx, fx = sL + bidx % nx;
+    const int py = end_point(y2, (double)(y2 - fy), scale, 0, bot);
+    const int px = end_point(x2, (double)(x2 - fx), scale, 0, right);
+    MVd r;
+    r.fx = (double)fx; r.fy = (double)fy; r.px = (double)px; r.py = (double)py;
+    r.vx = (double)(px - fx); r.vy = (double)(py - fy);
+    mv[blk] = r;
+  }
+}
+
+// Anti-diagonal sweep: blocks on diagonal d = i + j depend only on diagonal
+// d-1 (refined) and d+1 (not yet refined), so this order equals the raster
+// order of the in-place reference loop.
+__device__ void bidir_sweep(const FrameDev &f, MVd *mv, int blockLen) {
+  const int bw = f.W / blockLen, bh = f.H / blockLen;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, nwaves = blockDim.x >> 6;
+  for (int d = 0; d <= bw + bh - 2; d++) {
+    const int i0 = imax(0, d - (bw - 1)), i1 = imin(bh - 1, d);
+    for (int i = i0 + wave; i <= i1; i += nwaves) bidir_block(f, mv, i * bw + (d - i), bw, bh, blockLen, lane);
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// The rest of createSideInfo (sideInfoCreation.cpp:119-143, 510-550) and
+// motionFilterAndComp (:312-436) for one frame per workgroup.
+__global__ __launch_bounds__(1024) void k_si_chain(const FrameDev *F, const int *list) {
+  const FrameDev &f = F[list[blockIdx.x]];
+  const int tid = threadIdx.x, T = blockDim.x, W = f.W, H = f.H;
+  const int bw16 = W / 16, nmb = bw16 * (H / 16), bw8 = W / 8, bh8 = H / 8, nb8 = bw8 * bh8;
+  const int W2 = 2 * W, H2 = 2 * H;
+  // nearest hole per macroblock (:120-142) and the x2 half-pel scaling (:511-521)
+  const double maxDst = W * W + H * H;
+  for (int i = tid; i < nmb; i += T) {
+    const int top = (i / bw16) * 16, left = (i % bw16) * 16;
+    double minDst = maxDst;
+    int best = 0;
+    for (int jj = 0; jj < nmb; jj++) {
+      const double hx = f.holes[2 * jj], hy = f.holes[2 * jj + 1];
+      const double dst = (left - hx) * (left - hx) + (top - hy) * (top - hy);
+      if (dst < minDst) { minDst = dst; best = jj; }
+    }
+    const MVd b = f.fwd[best];
+    MVd o;
+    o.fx = (double)end_point(left, -b.vx, f.iFC, 0, W - 16);
+    o.fy = (double)end_point(top, -b.vy, f.iFC, 0, H - 16);
+    o.px = (double)end_point(left, b.vx, f.iPC, 0, W - 16);
+    o.py = (double)end_point(top, b.vy, f.iPC, 0, H - 16);
+    o.fx *= 2; o.fy *= 2; o.px *= 2; o.py *= 2;
+    o.vx = o.px - o.fx; o.vy = o.py - o.fy;
+    f.mbmv[i] = o;
+  }
+  __threadfence_block();
+  __syncthreads();
+  bidir_sweep(f, f.mbmv, 16);
+  // 8x8 initial MVs from the 16x16 vectors (:524-545)
+  for (int b = tid; b < nb8; b += T) {
+    const int i = b / bw8, j = b % bw8, y2 = i << 4, x2 = j << 4;
+    const MVd m = f.mbmv[(i >> 1) * bw16 + (j >> 1)];
+    MVd r;
+    r.fx = (double)end_point(x2, -m.vx, f.iFC, 0, (W - 8) << 1);
+    r.fy = (double)end_point(y2, -m.vy, f.iFC, 0, (H - 8) << 1);
+    r.px = (double)end_point(x2, m.vx, f.iPC, 0, (W - 8) << 1);
+    r.py = (double)end_point(y2, m.vy, f.iPC, 0, (H - 8) << 1);
+    r.vx = r.px - r.fx; r.vy = r.py - r.fy;
+    f.bmv[b] = r;
+  }
+  __threadfence_block();
+  __syncthreads();
+  bidir_sweep(f, f.bmv, 8);
+  // MSE per 8x8 block (:324-342); integer-valued sums are exact in double
+  for (int b = tid; b < nb8; b += T) {
+    const MVd m = f.bmv[b];
+    const int fy = (int)m.fy, fx = (int)m.fx, py = (int)m.py, px = (int)m.px;
+    double mse = 0.0;
+    for (int a = 0; a < 16; a += 2)
+      for (int q = 0; q < 16; q += 2) {
+        const double d = (double)f.fut_half[(fy + a) * W2 + fx + q] - (double)f.past_half[(py + a) * W2 + px + q];
+        mse += d * d;
+      }
+    f.mse[b] = mse / 64;
+  }
+  __threadfence_block();
+  __syncthreads();
+  // weighted vector-median filter over the 3x3 neighbourhood (:345-405)
+  for (int b = tid; b < nb8; b += T) {
+    const int i = b / bw8, j = b % bw8, y2 = i << 4, x2 = j << 4;
+    int nbr[9], cnt = 0;
+    for (int m = i - 1; m <= i + 1; m++) {
+      if (m < 0 || m >= bh8) continue;
+      for (int q = j - 1; q <= j + 1; q++) {
+        if (q < 0 || q >= bw8) continue;
+        nbr[cnt++] = m * bw8 + q;
+      }
+    }
+    double minW = 0.0;
+    int best = nbr[0];
+    for (int k = 1; k < cnt; k++) {
+      const double dx = f.bmv[best].vx - f.bmv[nbr[k]].vx, dy = f.bmv[best].vy - f.bmv[nbr[k]].vy;
+      minW += f.mse[best] / f.mse[nbr[k]] * sqrt(dx * dx + dy * dy);
+    }
+    for (int c = 1; c < cnt; c++) {
+      const int cand = nbr[c];
+      double w = 0.0;
+      for (int k = 0; k < cnt; k++) {
+        if (c == k) continue;
+        const double dx = f.bmv[cand].vx - f.bmv[nbr[k]].vx, dy = f.bmv[cand].vy - f.bmv[nbr[k]].vy;
+        w += f.mse[cand] / f.mse[nbr[k]] * sqrt(dx * dx + dy * dy);
+      }
+      if (w < minW) { minW = w; best = cand; }
+    }
+    const double vx = f.bmv[best].vx, vy = f.bmv[best].vy;
+    MVd r;
+    r.fx = (double)end_point(x2, -vx, f.iFC, 0, W2 - 16);
+    r.fy = (double)end_point(y2, -vy, f.iFC, 0, H2 - 16);
+    r.px = (double)end_point(x2, vx, f.iPC, 0, W2 - 16);
+    r.py = (double)end_point(y2, vy, f.iPC, 0, H2 - 16);
+    r.vx = r.px - r.fx; r.vy = r.py - r.fy;
+    f.bmv_tmp[b] = r;
+  }
+  __threadfence_block();
+  __syncthreads();
+  // motion compensation (:407-434)
+  for (int p = tid; p < W * H; p += T) {
+    const int y = p / W, x = p % W, b = (y >> 3) * bw8 + (x >> 3);
+    const MVd m = f.bmv_tmp[b];
+    const int a = (y & 7) << 1, q = (x & 7) << 1;
+    const int fv = f.fut_half[((int)m.fy + a) * W2 + (int)m.fx + q];
+    const int pv = f.past_half[((int)m.py + a) * W2 + (int)m.px + q];
+    f.si[p] = (uint8_t)(int)((double)fv * f.iFC + (double)pv * f.iPC);
+    f.residue[p] = (fv - pv) >> 1;
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (int b = tid; b < nb8; b += T) f.bmv[b] = f.bmv_tmp[b];
+}
+
+// ------------------------------------------------------------ transform helpers
+__device__ __forceinline__ int round_d(double x) { return x >= 0.0 ? (int)(x + 0.5) : -(int)(-x + 0.5); }
+
+// forward44 (transform.cpp:63-104) with the double post-scale of initTransform (:12-33).
+__device__ void forward44(int *blk) {
+  // b = sqrt(0.4) as the host's correctly rounded double (transform.cpp:14)
+  const double a2 = 0.25, ab2 = 0.5 * 0x1.43d136248490fp-1 / 2, b24 = 0.4 / 4;
+  int t[16];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int x0 = blk[c], x1 = blk[4 + c], x2 = blk[8 + c], x3 = blk[12 + c];
+    t[c] = x0 + x1 + x2 + x3;
+    t[4 + c] = (x0 << 1) + x1 - x2 - (x3 << 1);
+    t[8 + c] = x0 - x1 - x2 + x3;
+    t[12 + c] = x0 - (x1 << 1) + (x2 << 1) - x3;
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int x0 = t[4 * r], x1 = t[4 * r + 1], x2 = t[4 * r + 2], x3 = t[4 * r + 3];
+    blk[4 * r] = x0 + x1 + x2 + x3;
+    blk[4 * r + 1] = (x0 << 1) + x1 - x2 - (x3 << 1);
+    blk[4 * r + 2] = x0 - x1 - x2 + x3;
+    blk[4 * r + 3] = x0 - (x1 << 1) + (x2 << 1) - x3;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int r = i >> 2, c = i & 3;
+    const double ps = ((r & 1) == 0 && (c & 1) == 0) ? a2 : (((r & 1) && (c & 1)) ? b24 : ab2);
+    blk[i] = round_d((double)blk[i] * ps);
+  }
+}
+
+// inverse44 (transform.cpp:106-151) with preScale*64.
+__device__ void inverse44(int *blk) {
+  const double a2 = 0.25 * 64.0, ab = 0.5 * 0x1.43d136248490fp-1 * 64.0, b2 = 0.4 * 64.0;
+  int t[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const int r = i >> 2, c = i & 3;
+    const double ps = ((r & 1) == 0 && (c & 1) == 0) ? a2 : (((r & 1) && (c & 1)) ? b2 : ab);
+    blk[i] = round_d((double)blk[i] * ps);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int x0 = blk[c], x1 = blk[4 + c], x2 = blk[8 + c], x3 = blk[12 + c];
+    t[c] = x0 + x1 + x2 + (x3 >> 1);
+    t[4 + c] = x0 + (x1 >> 1) - x2 - x3;
+    t[8 + c] = x0 - (x1 >> 1) - x2 + x3;
+    t[12 + c] = x0 - x1 + x2 - (x3 >> 1);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int x0 = t[4 * r], x1 = t[4 * r + 1], x2 = t[4 * r + 2], x3 = t[4 * r + 3];
+    blk[4 * r] = x0 + x1 + x2 + (x3 >> 1);
+    blk[4 * r + 1] = x0 + (x1 >> 1) - x2 - x3;
+    blk[4 * r + 2] = x0 - (x1 >> 1) - x2 + x3;
+    blk[4 * r + 3] = x0 - x1 + x2 - (x3 >> 1);
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) blk[i] >>= 6;
+}
+
+__constant__ int c_zigzag[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
+__constant__ int c_qbits[9][16] = {
+    {0},
+    {4, 3, 0, 0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {5, 3, 0, 0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+    {5, 3, 2, 0, 3, 2, 0, 0, 2, 0, 0, 0, 0, 0, 0, 0},
+    {5, 4, 3, 2, 4, 3, 2, 0, 3, 2, 0, 0, 2, 0, 0, 0},
+    {5, 4, 3, 2, 4, 3, 2, 2, 3, 2, 2, 0, 2, 2, 0, 0},
+    {6, 4, 3, 3, 4, 3, 3, 2, 3, 3, 2, 2, 3, 2, 2, 0},
+    {6, 5, 4, 3, 5, 4, 3, 2, 4, 3, 2, 2, 3, 2, 2, 0},
+    {7, 6, 5, 4, 6, 5, 4, 3, 5, 4, 3, 2, 4, 3, 2, 0}};
+
+// ------------------------------------------------------------ noise parameter
+// forwardTransform of the residue (transform.cpp:153-190) and noiseParameter
+// (noiseModel.cpp:9-75) for one frame per workgroup.  The band means and
+// second moments are float sums over all blocks in raster order, one band per
+// thread, as the reference's loop accumulates them.
+__global__ __launch_bounds__(1024) void k_noise_param(const FrameDev *F, const int *list) {
+  const FrameDev &f = F[list[blockIdx.x]];
+  const int tid = threadIdx.x, T = blockDim.x, W = f.W, N = f.N, bw = W / 4;
+  __shared__ float s_mean[16], s_var[16];
+  __shared__ int s_nbands;
+  for (int b = tid; b < N; b += T) {
+    const int top = (b / bw) * 4, left = (b % bw) * 4;
+    int blk[16];
+    for (int m = 0; m < 4; m++)
+      for (int n = 0; n < 4; n++) blk[4 * m + n] = f.residue[(top + m) * W + left + n];
+    forward44(blk);
+    for (int m = 0; m < 4; m++)
+      for (int n = 0; n < 4; n++) f.sideTrans[(top + m) * W + left + n] = blk[4 * m + n];
+  }
+  if (tid == 0) {
+    int nb = 0;
+    while (nb < 16 && c_qbits[f.qindex][c_zigzag[nb]] > 0) nb++;
+    s_nbands = nb;
+  }
+  __threadfence_block();
+  __syncthreads();
+  const int nb = s_nbands;
+  if (tid < nb) {
+    const int index = c_zigzag[tid];
+    float mean = 0.0f, var = 0.0f;
+    for (int b = 0; b < N; b++) {
+      const int top = (b / bw) * 4, left = (b % bw) * 4;
+      const float c = fabsf((float)f.sideTrans[(top + (index >> 2)) * W + left + (index & 3)]);
+      mean += c;
+      var += c * c;
+    }
+    mean /= (float)N;
+    var = var / (float)N - mean * mean;
+    s_mean[tid] = mean;
+    s_var[tid] = var;
+  }
+  __syncthreads();
+  for (int e = tid; e < N * nb; e += T) {
+    const int b = e / nb, band = e % nb, index = c_zigzag[band];
+    const int top = (b / bw) * 4, left = (b % bw) * 4;
+    const int ci = (top + (index >> 2)) * W + left + (index & 3);
+    const float c = fabsf((float)f.sideTrans[ci]);
+    const float dst = (float)(int)c - s_mean[band];
+    const float dst2 = dst * dst;
+    f.alpha[ci] = dst2 > s_var[band] ? sqrtf(2 / dst2) : sqrtf(2 / s_var[band]);
+  }
+}
+
+// ------------------------------------------------------------ over-complete transform
+// overCompleteTransform (transform.cpp:275-298); with initSideTrans, also
+// init_si_ML (motionLearning.cpp:49-68) from the block-aligned positions.
+__global__ __launch_bounds__(256) void k_oct(const FrameDev *F, const int *list, int initSideTrans) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int ow = f.W - 3, npos = ow * (f.H - 3);
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npos) return;
+  const int top = p / ow, left = p % ow, W = f.W;
+  int blk[16];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int n = 0; n < 4; n++) blk[4 * m + n] = f.si[(top + m) * W + left + n];
+  forward44(blk);
+  int4 *o = (int4 *)(f.oct + (size_t)16 * p);
+  o[0] = make_int4(blk[0], blk[1], blk[2], blk[3]);
+  o[1] = make_int4(blk[4], blk[5], blk[6], blk[7]);
+  o[2] = make_int4(blk[8], blk[9], blk[10], blk[11]);
+  o[3] = make_int4(blk[12], blk[13], blk[14], blk[15]);
+  if (initSideTrans && (top & 3) == 0 && (left & 3) == 0 && top <= f.H - 4 && left <= f.W - 4)
+    for (int m = 0; m < 4; m++)
+      for (int n = 0; n < 4; n++) f.sideTrans[(top + m) * W + left + n] = blk[4 * m + n];
+}
+
+// SMF window of WZ block b (initSMF, motionLearning.cpp:6-47).
+struct SmfWin { int top, left, sT, sB, sL, sR, w, len, self; };
+__device__ __forceinline__ SmfWin smf_window(const FrameDev &f, int b) {
+  SmfWin s;
+  const int bw = f.W / 4;
+  s.top = (b / bw) * 4; s.left = (b % bw) * 4;
+  s.sT = imax(0, s.top - 4); s.sB = imin(f.H - 4, s.top + 4);
+  s.sL = imax(0, s.left - 4); s.sR = imin(f.W - 4, s.left + 4);
+  s.w = s.sR - s.sL + 1;
+  s.len = s.w * (s.sB - s.sT + 1);
+  s.self = (s.top - s.sT) * s.w + (s.left - s.sL);
+  return s;
+}
+
+__global__ void k_smf_init(const FrameDev *F, const int *list) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= f.N) return;
+  const SmfWin s = smf_window(f, b);
+  for (int p = 0; p < 81; p++) f.smf[(size_t)81 * b + p] = (p == s.self) ? 1.0f : 0.0f;
+}
+
+// ------------------------------------------------------------ noise distribution
+// noiseDistribution (noiseModel.cpp:77-132): DC prob[k] = expf(-a*|k - SI_DC|);
+// AC prob[k] = sum_m p_m * expf(-a*|k - range - SI_m|), m in window raster
+// order.  A window term whose probability is exactly +0 adds +0 to a
+// non-negative sum and is skipped when a is finite.  Grid: (k chunks, block, frame).
+__global__ __launch_bounds__(256) void k_noise_dist(const FrameDev *F, const int *list) {
+  const FrameDev &f = F[list[blockIdx.z]];
+  const int b = blockIdx.y;
+  if (b >= f.N) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ow = f.W - 3, index = f.index, range = f.range;
+  const SmfWin s = smf_window(f, b);
+  const float a = f.alpha[(s.top + (index >> 2)) * f.W + s.left + (index & 3)];
+  if (index == 0) {
+    if (k >= range) return;
+    const int side = f.oct[(size_t)16 * (s.top * ow + s.left)];
+    f.dist[(size_t)b * f.distStride + k] = dvc_expf(-a * (float)abs(k - side));
+    return;
+  }
+  __shared__ float s_p[81];
+  __shared__ int s_side[81];
+  __shared__ int s_n;
+  if (threadIdx.x == 0) {
+    const bool fin = isfinite(a);
+    int n = 0;
+    for (int p = 0; p < s.len; p++) {
+      const float pr = f.smf[(size_t)81 * b + p];
+      if (fin && pr == 0.0f) continue;
+      const int m = s.sT + p / s.w, q = s.sL + p % s.w;
+      s_p[n] = pr;
+      s_side[n] = f.oct[(size_t)16 * (m * ow + q) + index];
+      n++;
+    }
+    s_n = n;
+  }
+  __syncthreads();
+  const int rmax = 2 * range + 1;
+  if (k >= rmax) return;
+  const int pc = k - range, n = s_n;
+  float sum = 0.0f;
+  for (int m = 0; m < n; m++) sum += s_p[m] * dvc_expf(-a * (float)abs(pc - s_side[m]));
+  f.dist[(size_t)b * f.distStride + k] = sum;
+}
+
+// ------------------------------------------------------------ conditional bit probability
+// condBitProb (noiseModel.cpp:191-310): interval sums in ascending k, p0/(p0+p1),
+// clipped likelihood ratio.  One thread per block.
+__global__ void k_condbit(const FrameDev *F, const int *list) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= f.N) return;
+  const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4;
+  const int index = f.index, bit = f.bit, step = f.step, range = f.range;
+  const int iq = (1 << (bit - 1)) - 1;
+  const int rc = f.reconsTrans[(top + (index >> 2)) * f.W + left + (index & 3)];
+  const float *prob = f.dist + (size_t)b * f.distStride;
+  int v0l, v0u, v1l, v1u, lim;
+  if (index == 0) {
+    const int q0l = rc << bit, q1l = ((rc << 1) + 1) << (bit - 1);
+    const int q0u = q0l + iq, q1u = q1l + iq;
+    v0l = q0l * step; v1l = q1l * step; v0u = (q0u + 1) * step; v1u = (q1u + 1) * step;
+    lim = range;
+  } else {
+    const int ls = f.levelShift;
+    const int q0l = (rc << bit) - ls, q1l = (((rc << 1) + 1) << (bit - 1)) - ls;
+    const int q0u = q0l + iq;
+    int q1u = q1l + iq;
+    if (q1u > ls) q1u = ls;
+    v0l = (q0l <= 0) ? (q0l - 1) * step + 1 : q0l * step;
+    v1l = (q1l <= 0) ? (q1l - 1) * step + 1 : q1l * step;
+    v0u = (q0u < 0) ? q0u * step + 1 : (q0u + 1) * step;
+    v1u = (q1u < 0) ? q1u * step + 1 : (q1u + 1) * step;
+    v0l += range; v0u += range; v1l += range; v1u += range;
+    lim = 2 * range + 1;
+  }
+  v0l = imax(v0l, 0); v1l = imax(v1l, 0); v0u = imin(v0u, lim); v1u = imin(v1u, lim);
+  float p0 = 0.0f, p1 = 0.0f;
+  for (int k = v0l; k < v0u; k++) p0 += prob[k];
+  for (int k = v1l; k < v1u; k++) p1 += prob[k];
+  p0 = p0 / (p0 + p1);
+  p1 = 1 - p0;
+  float lr;
+  if ((double)p0 <= 0.0001) lr = (float)(0.0001 / 0.9999);
+  else if ((double)p0 >= 0.9999) lr = (float)(0.9999 / 0.0001);
+  else lr = p0 / p1;
+  f.LR[b] = lr;
+}
+
+// ------------------------------------------------------------ bitplane -> coefficients
+// putBitplaneToCoeff (bitplane.cpp:4-22) with the bits of the code the rate
+// search accepted (its record names the decision slot).
+__global__ void k_put_bitplane(const FrameDev *F, const int *list, const BpFrameLink *links) {
+  const int fi = list[blockIdx.y];
+  const FrameDev &f = F[fi];
+  const BpFrameLink &lk = links[blockIdx.y];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= f.N) return;
+  const int best = lk.state->best;
+  const int slot = (best >= 0 && best < 65) ? lk.records[best].z : -1;
+  if (slot < 0 || slot >= lk.slot_capacity) return;  // the host reports the missing record
+  const uint8_t bitv = lk.slots[(size_t)slot * f.N + b];
+  f.decoded[b] = bitv;
+  const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4, index = f.index;
+  int *p = &f.reconsTrans[(top + (index >> 2)) * f.W + left + (index & 3)];
+  *p = (*p << 1) + (int)bitv;
+}
+
+// ------------------------------------------------------------ reconstruction
+// reconstruction (reconstruction.cpp:184-289).  exp() here is the device's
+// double exponential; an entry whose pre-rounding value lies within a margin
+// of a rounding boundary (where a last-bit difference from the host's exp
+// could change the integer) is left to the host, which recomputes it with the
+// C library (dvc_frame_* host code).
+__global__ void k_reconstruct(const FrameDev *F, const int *list) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= f.N) return;
+  const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4, index = f.index, step = f.step;
+  const int ci = (top + (index >> 2)) * f.W + left + (index & 3);
+  const int side = f.sideTrans[ci];
+  const double a = f.alpha[ci];
+  int lo, hi;
+  if (index == 0) {
+    const int q = f.reconsTrans[ci];
+    lo = q * step; hi = (q + 1) * step;
+  } else {
+    const int q = f.reconsTrans[ci] - f.levelShift;
+    if (q == 0) { lo = -step + 1; hi = step; }
+    else if (q > 0) { lo = q * step; hi = (q + 1) * step; }
+    else { lo = (q - 1) * step + 1; hi = q * step + 1; }
+  }
+  if (a <= 0.0001) { f.sideTrans[ci] = (lo + hi) / 2; return; }
+  double v;
+  int base, sign;
+  if (side < lo) { v = 1.0 / a + (double)step / (1.0 - exp(a * (double)step)); base = lo; sign = 1; }
+  else if (side >= hi) { v = 1.0 / a + (double)step / (1.0 - exp(a * (double)step)); base = hi; sign = -1; }
+  else {
+    const double r = side - lo, d = hi - side;
+    const double er = exp(-a * r), ed = exp(-a * d);
+    v = ((r + 1.0 / a) * er - (d + 1.0 / a) * ed) / (2.0 - (er + ed));
+    base = side; sign = 1;
+  }
+  const double av = fabs(v) + 0.5;
+  const double frac = av - floor(av);
+  const double margin = 1e-7 * (1.0 + fabs(v));
+  if (!isfinite(v) || frac < margin || frac > 1.0 - margin) {
+    const int slot = atomicAdd(f.tieCount, 1);
+    if (slot < f.N) f.tieList[slot] = ci;
+    return;
+  }
+  f.sideTrans[ci] = base + sign * round_d(v);
+}
+
+// inverseTransform (transform.cpp:234-273).
+__global__ void k_inverse_transform(const FrameDev *F, const int *list) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= f.N) return;
+  const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4, W = f.W;
+  int blk[16];
+  for (int m = 0; m < 4; m++)
+    for (int n = 0; n < 4; n++) blk[4 * m + n] = f.sideTrans[(top + m) * W + left + n];
+  inverse44(blk);
+  for (int m = 0; m < 4; m++)
+    for (int n = 0; n < 4; n++) {
+      const int c = blk[4 * m + n];
+      f.recons[(top + m) * W + left + n] = (uint8_t)(c < 0 ? 0 : (c > 255 ? 255 : c));
+    }
+}
+
+// ------------------------------------------------------------ SI refresh
+// updateSI + subPelRefine (sideInfoCreation.cpp:598-736): per WZ 4x4 block of
+// the partially reconstructed frame, separate full-pel argmins on the past
+// and future key frames (cost SAD*(1+K*|2v|) in double), 9-position half-pel
+// refinement on each interpolated frame, then SI = (int)(P*iPC + F*iFC).
+// One wave per block, 4 blocks per workgroup.
+__global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int *list, CostTables ct) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + wave;
+  if (b >= f.N) return;
+  const int W = f.W, H = f.H, W2 = 2 * W, H2 = 2 * H, bw = W / 4;
+  const int top = (b / bw) * 4, left = (b % bw) * 4;
+  const int sT = imax(0, top - 32), sB = imin(H - 4, top + 32), sL = imax(0, left - 32), sR = imin(W - 4, left + 32);
+  const int ww = sR - sL + 1, nc = ww * (sB - sT + 1);
+  int blk[16];
+#pragma unroll
+  for (int m = 0; m < 4; m++)
+#pragma unroll
+    for (int n = 0; n < 4; n++) blk[4 * m + n] = f.recons[(top + m) * W + left + n];
+  double pBest = DBL_MAX, fBest = DBL_MAX;
+  int pIdx = 0x7fffffff, fIdx = 0x7fffffff;
+  for (int c = lane; c < nc; c += 64) {
+    const int y = sT + c / ww, x = sL + c % ww;
+    int ps = 0, fs = 0;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const uint8_t *pr = f.past + (y + m) * W + x;
+      const uint8_t *fr = f.fut + (y + m) * W + x;
+#pragma unroll
+      for (int n = 0; n < 4; n++) {
+        ps += abs(blk[4 * m + n] - (int)pr[n]);
+        fs += abs(blk[4 * m + n] - (int)fr[n]);
+      }
+    }
+    const double fac = ct.usi[(y - top + 32) * 65 + (x - left + 32)];
+    const double pc = (double)ps * fac, fc = (double)fs * fac;
+    if (pc < pBest) { pBest = pc; pIdx = c; }
+    if (fc < fBest) { fBest = fc; fIdx = c; }
+  }
+  wave_argmin(pBest, pIdx);
+  wave_argmin(fBest, fIdx);
+  int pbx = sL + pIdx % ww, pby = sT + pIdx / ww, fbx = sL + fIdx % ww, fby = sT + fIdx / ww;
+  // subPelRefine: lanes 0-8 evaluate the 9 positions of the past frame, 16-24 the future
+  const int which = lane >> 4, pos = lane & 15;
+  double cost = DBL_MAX;
+  int cidx = 0x7fffffff;
+  if (which < 2 && pos < 9) {
+    const int bx = which ? fbx : pbx, by = which ? fby : pby;
+    const uint8_t *ref = which ? f.fut_half : f.past_half;
+    const int t = 2 * by + pos / 3 - 1, l = 2 * bx + pos % 3 - 1;
+    if (t >= 0 && t <= H2 - 8 && l >= 0 && l <= W2 - 8) {
+      int s = 0;
+      for (int m = 0; m < 4; m++)
+        for (int n = 0; n < 4; n++) s += abs(blk[4 * m + n] - (int)ref[(t + 2 * m) * W2 + l + 2 * n]);
+      cost = (double)s;
+      cidx = pos;
+    }
+  }
+  // argmin within each 16-lane group (first minimum in (i, j) order)
+  for (int o = 8; o > 0; o >>= 1) {
+    const double c2 = __shfl_xor(cost, o);
+    const int i2 = __shfl_xor(cidx, o);
+    if (c2 < cost || (c2 == cost && i2 < cidx)) { cost = c2; cidx = i2; }
+  }
+  const int pSub = __shfl(cidx, 0), fSub = __shfl(cidx, 16);
+  const int phx = 2 * pbx + pSub % 3 - 1, phy = 2 * pby + pSub / 3 - 1;
+  const int fhx = 2 * fbx + fSub % 3 - 1, fhy = 2 * fby + fSub / 3 - 1;
+  if (lane < 16) {
+    const int m = lane >> 2, n = lane & 3;
+    const int pv = f.past_half[(phy + 2 * m) * W2 + phx + 2 * n];
+    const int fv = f.fut_half[(fhy + 2 * m) * W2 + fhx + 2 * n];
+    f.si[(top + m) * W + left + n] = (uint8_t)(int)((double)pv * f.iPC + (double)fv * f.iFC);
+  }
+}
+
+// ------------------------------------------------------------ motion learning
+// updateSMF (motionLearning.cpp:70-127): p = expf(-MU*SSE) over the +-4
+// window, normalised by the float sum in raster order.
+__global__ void k_update_smf(const FrameDev *F, const int *list) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= f.N) return;
+  const int W = f.W;
+  const SmfWin s = smf_window(f, b);
+  float blk[16];
+  for (int m = 0; m < 4; m++)
+    for (int n = 0; n < 4; n++) blk[4 * m + n] = (float)f.recons[(s.top + m) * W + s.left + n];
+  float *pr = f.smf + (size_t)81 * b;
+  float sum = 0.0f;
+  int p = 0;
+  for (int m = s.sT; m <= s.sB; m++)
+    for (int q = s.sL; q <= s.sR; q++) {
+      float sse = 0.0f;
+      for (int ii = 0; ii < 4; ii++)
+        for (int jj = 0; jj < 4; jj++) {
+          const float d = blk[4 * ii + jj] - (float)f.si[(m + ii) * W + q + jj];
+          sse += d * d;
+        }
+      pr[p] = dvc_expf(-0.01f * sse);
+      sum += pr[p];
+      p++;
+    }
+  if ((double)sum > 0.0) {
+    for (int q = 0; q < p; q++) pr[q] = pr[q] / sum;
+  } else {
+    pr[s.self] = 1.0f;
+  }
+}
+
+// update_si_ML (motionLearning.cpp:129-160): SMF-weighted over-complete
+// coefficients for the bands after the frame's current band, accumulated in
+// window raster order and truncated.
+__global__ void k_update_si_ml(const FrameDev *F, const int *list, int) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int bandStart = f.band + 1;  // update_si_ML(..., band+1, ...) (DVC_Decoder.cpp:369)
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= f.N) return;
+  const int W = f.W, ow = W - 3;
+  const SmfWin s = smf_window(f, b);
+  float acc[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) acc[i] = 0.0f;
+  const float *pr = f.smf + (size_t)81 * b;
+  int p = 0;
+  for (int m = s.sT; m <= s.sB; m++)
+    for (int q = s.sL; q <= s.sR; q++) {
+      const float prob = pr[p++];
+      const int *o = f.oct + (size_t)16 * (m * ow + q);
+#pragma unroll
+      for (int k = 0; k < 16; k++)
+        if (k >= bandStart) {
+          const int index = c_zigzag[k];
+          acc[index] += prob * (float)o[index];
+        }
+    }
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    if (k >= bandStart) {
+      const int index = c_zigzag[k];
+      f.sideTrans[(s.top + (index >> 2)) * W + s.left + (index & 3)] = (int)acc[index];
+    }
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_keyframe_filters(const uint8_t *in, uint8_t *lp, uint8_t *half, int W, int H, hipStream_t s) {
+  hipLaunchKernelGGL(k_keyframe_filters, dim3((2 * W + 127) / 128, H), dim3(128), 0, s, in, lp, half, W, H);
+  return hipGetLastError();
+}
+hipError_t launch_create_si(const FrameDev *F, const int *list, int n, int maxMB, CostTables ct, hipStream_t s) {
+  hipLaunchKernelGGL(k_forward_me, dim3(maxMB, n), dim3(256), 0, s, F, list, ct);
+  hipLaunchKernelGGL(k_si_chain, dim3(n), dim3(1024), 0, s, F, list);
+  return hipGetLastError();
+}
+hipError_t launch_noise_param(const FrameDev *F, const int *list, int n, int, hipStream_t s) {
+  hipLaunchKernelGGL(k_noise_param, dim3(n), dim3(1024), 0, s, F, list);
+  return hipGetLastError();
+}
+hipError_t launch_oct(const FrameDev *F, const int *list, int n, int maxPos, int initSideTrans, hipStream_t s) {
+  hipLaunchKernelGGL(k_oct, dim3((maxPos + 255) / 256, n), dim3(256), 0, s, F, list, initSideTrans);
+  return hipGetLastError();
+}
+hipError_t launch_noise_dist(const FrameDev *F, const int *list, int n, int maxN, int maxStride, hipStream_t s) {
+  hipLaunchKernelGGL(k_noise_dist, dim3((maxStride + 255) / 256, maxN, n), dim3(256), 0, s, F, list);
+  return hipGetLastError();
+}
+hipError_t launch_condbit(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s) {
+  hipLaunchKernelGGL(k_condbit, dim3((maxN + 127) / 128, n), dim3(128), 0, s, F, list);
+  return hipGetLastError();
+}
+hipError_t launch_put_bitplane(const FrameDev *F, const int *list, const BpFrameLink *links, int n, int maxN,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_put_bitplane, dim3((maxN + 255) / 256, n), dim3(256), 0, s, F, list, links);
+  return hipGetLastError();
+}
+hipError_t launch_reconstruct(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s) {
+  hipLaunchKernelGGL(k_reconstruct, dim3((maxN + 255) / 256, n), dim3(256), 0, s, F, list);
+  return hipGetLastError();
+}
+hipError_t launch_inverse_transform(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s) {
+  hipLaunchKernelGGL(k_inverse_transform, dim3((maxN + 255) / 256, n), dim3(256), 0, s, F, list);
+  return hipGetLastError();
+}
+hipError_t launch_update_si(const FrameDev *F, const int *list, int n, int maxN, CostTables ct, hipStream_t s) {
+  hipLaunchKernelGGL(k_update_si, dim3((maxN + 3) / 4, n), dim3(256), 0, s, F, list, ct);
+  return hipGetLastError();
+}
+hipError_t launch_smf_init(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s) {
+  hipLaunchKernelGGL(k_smf_init, dim3((maxN + 255) / 256, n), dim3(256), 0, s, F, list);
+  return hipGetLastError();
+}
+hipError_t launch_update_smf(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s) {
+  hipLaunchKernelGGL(k_update_smf, dim3((maxN + 127) / 128, n), dim3(128), 0, s, F, list);
+  return hipGetLastError();
+}
+hipError_t launch_update_si_ml(const FrameDev *F, const int *list, int n, int maxN, int bandStart, hipStream_t s) {
+  hipLaunchKernelGGL(k_update_si_ml, dim3((maxN + 127) / 128, n), dim3(128), 0, s, F, list, bandStart);
+  return hipGetLastError();
+}
+
+}  // namespace dvc

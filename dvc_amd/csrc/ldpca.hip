@@ -262,13 +262,13 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
           }
         }
         if (tid == 0) {
-          a.records[(size_t)b * L.numCodes + level] = make_int4(iters, 2, slot, 0);
+          pd.rec[level] = make_int4(iters, 2, slot, 0);
           atomicMin(&st->best, level);
         }
       }
     }
     if (tid == 0 && !ok)
-      a.records[(size_t)b * L.numCodes + level] = make_int4(iters, status == 1 ? 1 : (status == 3 ? 3 : 0), -1, 0);
+      pd.rec[level] = make_int4(iters, status == 1 ? 1 : (status == 3 ? 3 : 0), -1, 0);
     __syncthreads();
   }
 }
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
 // invH_mul (LdpcaDecoder.cpp:624-643) for every bitplane whose BP codes all
 // failed; one thread per output bit.
 __global__ void invh_kernel(const PlaneDesc *planes, int nplanes, LadderDev L, uint8_t *slots,
-                            int *slot_count, int slot_capacity, int4 *records) {
+                            int *slot_count, int slot_capacity) {
   const int p = blockIdx.y;
   if (p >= nplanes) return;
   PlaneDesc d = planes[p];
@@ -299,7 +299,7 @@ __global__ void invh_kernel(const PlaneDesc *planes, int nplanes, LadderDev L, u
   }
   if (threadIdx.x == 0) {
     s_slot = atomicAdd(slot_count, 1);
-    records[(size_t)p * L.numCodes + (L.numCodes - 1)] = make_int4(0, 2, s_slot, 1);
+    d.rec[L.numCodes - 1] = make_int4(0, 2, s_slot, 1);
   }
   __syncthreads();
   const int slot = s_slot;
@@ -340,11 +340,11 @@ hipError_t launch_ce_llr(const PlaneDesc *d_planes, int n, int fixed_start, cons
 }
 
 hipError_t launch_invh(const PlaneDesc *d_planes, int n, const LadderDev &L, uint8_t *slots, int *slot_count,
-                       int slot_capacity, int4 *records, hipStream_t s) {
+                       int slot_capacity, hipStream_t s) {
   // One workgroup column per bitplane; a single workgroup per bitplane keeps
   // the slot assignment in blockIdx.x == 0 ordered before its use.
   hipLaunchKernelGGL(invh_kernel, dim3(1, n), dim3(256), 0, s, d_planes, n, L, slots, slot_count,
-                     slot_capacity, records);
+                     slot_capacity);
   return hipGetLastError();
 }
 

@@ -23,6 +23,15 @@ struct PlaneDesc {
   const uint8_t *acc;  // N/8 accumulated-syndrome bytes
   uint32_t crc;        // encoder CRC-8
   PlaneState *state;
+  int4 *rec;           // [numCodes] per code: iterations, status, decision slot, used_inverse
+};
+
+// What k_put_bitplane needs to find the accepted decisions of a bitplane.
+struct BpFrameLink {
+  const PlaneState *state;
+  const int4 *records;
+  const uint8_t *slots;
+  int slot_capacity;
 };
 
 struct LadderDev {
@@ -43,7 +52,6 @@ struct BpArgs {
   uint8_t *slots;     // accepted decisions, N bytes per slot
   int *slot_count;
   int slot_capacity;
-  int4 *records;      // [nplanes * numCodes]: iterations, status, slot, used_inverse
 };
 
 // record.y status values
@@ -54,6 +62,6 @@ hipError_t launch_bp_any(int N, int threads, const BpArgs &a, int grid, hipStrea
 hipError_t launch_ce_llr(const PlaneDesc *d_planes, int n, int fixed_start, const Ladder &l1584,
                          const Ladder &l6336, hipStream_t s);
 hipError_t launch_invh(const PlaneDesc *d_planes, int n, const LadderDev &L, uint8_t *slots, int *slot_count,
-                       int slot_capacity, int4 *records, hipStream_t s);
+                       int slot_capacity, hipStream_t s);
 
 }  // namespace dvc

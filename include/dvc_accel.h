@@ -109,6 +109,40 @@ typedef struct {
  * sequential search selects.  Host pointers. */
 int dvc_ldpca_decode_batch(dvc_ctx *ctx, int n, dvc_ldpca_item *items);
 
+/* ---------------------------------------------------------------- WZ frames */
+typedef struct {
+  int nbitplanes;
+  int bitrate;                /* decodedFrames[].bitRate: mode flags + AC ranges + sum of rates */
+  int levels[128];            /* accepted code per bitplane (decode order) */
+  int start_levels[128];
+  int iterations[128];        /* BP iterations of the sequential search per bitplane */
+  int used_inverse[128];
+  float entropy[128];
+  double psnr;                /* against orig when given (frame.cpp:115-124) */
+  int recon_host_fixups;      /* reconstruction entries recomputed with the C library's exp */
+} dvc_frame_stats;
+
+typedef struct {
+  int width, height;          /* 176x144 (N = 1584) or 352x288 (N = 6336) */
+  int qindex;                 /* 1..8 */
+  int past_idx, cur_idx, fut_idx;  /* frame numbers: cur is decoded from past and fut */
+  const uint8_t *past, *fut;  /* decoded key frames (luma, W*H) */
+  const uint8_t *orig;        /* optional original frame for PSNR */
+  const int *ac_range;        /* [16] ACRange of the WZ bitstream (wzbitstream.cpp:19-30) */
+  const uint8_t *crc;         /* [nbitplanes] CRC-8 per bitplane */
+  const uint8_t *acc;         /* [nbitplanes * N/8] accumulated syndromes */
+  uint8_t *recons;            /* out: W*H decoded frame */
+  dvc_frame_stats *stats;     /* out, may be NULL */
+} dvc_wz_frame;
+
+/* decodeGOP's per-WZ-frame body (DVC_Decoder.cpp:26-474, skip/intra/deblock
+ * off) for n frames.  batched = 0: frames one after another with the
+ * sequential rate search per bitplane (the plain drop-in of ParallelMode 2);
+ * batched = 1: all frames advance together, one bitplane per step, with the
+ * speculative persistent LDPCA launch.  Outputs are identical in both modes
+ * and equal the CPU path's (ParallelMode 0). */
+int dvc_decode_wz_frames(dvc_ctx *ctx, int n, const dvc_wz_frame *frames, int batched);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,68 @@
+"""Whole WZ-frame decoding on the GPU (dvc_decode_wz_frames) against the
+reference's own decodes (golden fixtures from decodeGOP) and the oracle:
+reconstruction, accepted code per bitplane, entropy bits, bitrate and PSNR
+must be identical, for the plain and the batched mode, and a frame decoded in
+a mixed batch equals the same frame decoded alone."""
+import hashlib
+import json
+import struct
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from dvc_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def fbits(x):
+    return struct.unpack("<I", struct.pack("<f", float(x)))[0]
+
+
+def _case_inputs(c):
+    vid = W.synthetic_video(c["W"], c["H"], c["frames"], c["shift"], c["seed"])
+    wz = W.encode_wz_frame(vid[c["wz"]], c["qindex"])
+    return dict(past=vid[0], fut=vid[2], orig=vid[c["wz"]], wz=wz)
+
+
+def _check_against_golden(rec, st, c):
+    print(c["W"], c["qindex"], "levels", st["levels"][:12], "golden", c["levels"][:12], flush=True)
+    assert st["levels"] == c["levels"]
+    assert [fbits(e) for e in st["entropy"]] == c["entropy_bits"]
+    assert st["bitrate"] == c["bitrate"]
+    assert sha(rec) == c["recons_sha256"]
+    assert st["psnr"] == c["psnr"]
+
+
+@pytest.mark.parametrize("case", range(4))
+@pytest.mark.parametrize("batched", [False, True])
+def test_frame_matches_reference_golden(ctx, case, batched):
+    c = json.loads((GOLD / "frames.json").read_text())["cases"][case]
+    (rec, st), = ctx.decode_wz_frames([_case_inputs(c)], batched=batched)
+    _check_against_golden(rec, st, c)
+
+
+def test_mixed_batch_equals_solo(ctx):
+    cases = json.loads((GOLD / "frames.json").read_text())["cases"]
+    inputs = [_case_inputs(c) for c in cases]
+    outs = ctx.decode_wz_frames(inputs, batched=True)
+    for (rec, st), c in zip(outs, cases):
+        _check_against_golden(rec, st, c)
+
+
+def test_frames_against_oracle_fresh_seed(ctx):
+    vid = W.synthetic_video(176, 144, 3, 3, seed=777)
+    for q in (1, 6):
+        wz = W.encode_wz_frame(vid[1], q)
+        orec, oinfo = O.decode_wz_frame(vid[0], vid[2], vid[1], wz, q)
+        (rec, st), = ctx.decode_wz_frames([dict(past=vid[0], fut=vid[2], orig=vid[1], wz=wz)], batched=True)
+        assert np.array_equal(rec, orec)
+        assert st["levels"] == oinfo["levels"] and st["bitrate"] == oinfo["bitrate"]
+        assert st["iterations"] == oinfo["iters"]
