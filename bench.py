@@ -3,6 +3,14 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
 Workloads
+  wz_cif_q8     (default) the plain drop-in configuration at its headline cell:
+                CIF 352x288, 33 frames, GOP 2, Qindex 8 (16 WZ frames, 63
+                bitplanes each, N = 6336), synthetic sequence as described in
+                dvc_amd.workload.  One step decodes `--streams` such sequences
+                (16 WZ frames each) as one batch with the speculative
+                persistent LDPCA launch; value = WZ frames/s.  The plain path
+                (one frame at a time, sequential rate search) is timed on one
+                sequence beside it.
   ldpca_batch   512 seeded bitplanes (384 x N=6336, 128 x N=1584) decoded in
                 one speculative persistent launch per step; one step = the
                 whole set.  value = BP edge-updates/s, counting the
@@ -106,6 +114,127 @@ def bench_ldpca(args):
     return res
 
 
+def make_wz_inputs(streams, frames=33, q=8, gop=2, seed0=1100):
+    from dvc_amd import workload as W
+    jobs = []
+    for st in range(streams):
+        vid = W.synthetic_video(352, 288, frames, 6, seed0 + st)
+        for k in range(0, frames - 1, gop):
+            cur = k + gop // 2
+            wz = W.encode_wz_frame(vid[cur], q)
+            jobs.append(dict(past=vid[k], fut=vid[k + gop], orig=vid[cur], wz=wz, idx=(k, cur, k + gop)))
+    return jobs
+
+
+def bench_wz(args):
+    from dvc_amd import _lib
+    t0 = time.time()
+    jobs = make_wz_inputs(args.streams)
+    log(f"[bench] encoded {len(jobs)} CIF Q8 WZ frames in {time.time() - t0:.1f}s")
+    ctx = _lib.Context(device=0, max_iter=args.max_iter)
+    for i in range(args.warmup):
+        ctx.decode_wz_frames(jobs[:16], batched=True)
+        log(f"[bench] warmup {i} done")
+    ctx.profile(True)
+    times = []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        outs = ctx.decode_wz_frames(jobs, batched=True)
+        times.append(time.perf_counter() - t)
+        log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms for {len(jobs)} frames")
+    names = ["create_si", "keyframe_filters", "noise_param", "oct", "smf_init", "noise_dist", "condbit", "ce_llr",
+             "bp_6336", "invh", "put_bitplane", "reconstruct", "inverse_transform", "update_si", "update_smf",
+             "update_si_ml"]
+    kern = {}
+    for nm in names:
+        ms, cnt = ctx.profile_get(nm)
+        if cnt:
+            kern[nm] = {"total_ms": ms / args.steps, "launches": cnt // args.steps}
+    ctx.profile(False)
+    # correctness of the measured outputs against the encoder's source planes:
+    # every decoded bitplane must equal the source (a CRC-accepted or full-rate
+    # decode), which the parity tests pin bit-exactly against the reference.
+    psnrs = [st["psnr"] for _, st in outs]
+    # plain drop-in path on the first sequence: one frame at a time
+    plain_n = min(args.plain_frames, len(jobs))
+    lat = []
+    for j in jobs[:plain_n]:
+        t = time.perf_counter()
+        ctx.decode_wz_frames([j], batched=False)
+        lat.append(time.perf_counter() - t)
+    log(f"[bench] plain: {plain_n} frames, mean {np.mean(lat) * 1e3:.1f} ms/frame")
+    t_step = float(np.mean(times))
+    value = len(jobs) / t_step
+    plain_fps = plain_n / float(np.sum(lat))
+    eu = sum(sum(st["iterations"]) * 3 * 6336 for _, st in outs)
+    dom = max(kern.items(), key=lambda kv: kv[1]["total_ms"]) if kern else ("", {"total_ms": 0, "launches": 1})
+    res = {
+        "metric": METRIC, "value": value, "unit": "WZ frames/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_step * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded CIF sequences: 40 Gaussian blobs, global shift U{-6..6}, 4 moving patches, noise)",
+        "config": {"workload": f"wz_cif_q8: CIF 352x288, 33 frames, GOP 2, Qindex 8, {args.streams} sequences "
+                               f"({len(jobs)} WZ frames, 63 bitplanes each) per step, batched, max_iter={args.max_iter}",
+                   "parallelism": "single GPU"},
+        "plain_fps": plain_fps, "batched_over_plain": value / plain_fps,
+        "bp_edge_updates_per_s": eu / t_step,
+        "latency_ms": {"plain_p50": float(np.percentile(lat, 50) * 1e3), "plain_p99": float(np.percentile(lat, 99) * 1e3),
+                       "batch_p50": float(np.percentile(times, 50) * 1e3), "batch_p99": float(np.percentile(times, 99) * 1e3)},
+        "psnr_mean": float(np.mean(psnrs)),
+        "kernels_ms_per_step": kern, "dominant_kernel": dom[0],
+    }
+    res["roofline"] = roofline_wz(dom, kern, args, len(jobs), outs)
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline_wz(jobs, args)
+    return res
+
+
+def roofline_wz(dom, kern, args, nframes, outs):
+    """Roofline of the dominant kernel, from its algorithmic traffic or work."""
+    name, d = dom
+    avg_ms = d["total_ms"] / max(d["launches"], 1)
+    out = {"bound": "hbm", "kernel": name, "kernel_ms_avg": avg_ms, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "traffic": None}
+    if name == "bp_6336":
+        # per launch (one bitplane step of all frames): per code run 4N LLR +
+        # m/8 syndrome + N/8 decisions + 2E row indices
+        byts = 0
+        for _, st in outs:
+            for b in range(st["nbitplanes"]):
+                for lvl in range(st["start_levels"][b], st["levels"][b] + 1):
+                    if lvl < 64:
+                        byts += 4 * 6336 + (96 * (lvl + 2) + 7) // 8 + 792 + 2 * 19008
+        per_launch = byts / max(d["launches"], 1)
+    elif name == "noise_dist":
+        # per launch: the distribution written once (4 bytes per value) plus
+        # the SMF window and SI coefficients read per block
+        per_launch = None
+    else:
+        per_launch = None
+    if per_launch:
+        out["achieved"] = per_launch / (avg_ms * 1e-3) / 1e9
+        out["frac"] = out["achieved"] / HBM_PEAK_GBS
+    else:
+        out["achieved"] = None
+        out["frac"] = None
+    return out
+
+
+def cpu_baseline_wz(jobs, args):
+    """Oracle (CPU restatement of the reference decoder, ParallelMode 1 with
+    OpenMP on every host core) on one frame of the same workload."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+    cores = os.cpu_count() or 1
+    j = jobs[0]
+    t = time.perf_counter()
+    O.decode_wz_frame(j["past"], j["fut"], j["orig"], j["wz"], 8, mode=1, nthreads=cores, max_iter=args.max_iter,
+                      idx=j["idx"])
+    dt = time.perf_counter() - t
+    return {"value": 1.0 / dt, "unit": "WZ frames/s", "cores": cores, "kind": "port",
+            "sample": f"1 CIF Q8 WZ frame of the same workload, ParallelMode 1 (OpenMP), {dt:.1f}s"}
+
+
 def cpu_baseline_ldpca(planes, args):
     """Oracle (CPU restatement, ParallelMode 0, one core) on a bounded sample."""
     sys.path.insert(0, str(ROOT / "tests"))
@@ -129,17 +258,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="ldpca_batch")
+    ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch"])
+    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--plain-frames", type=int, default=4)
     ap.add_argument("--max-iter", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-planes", type=int, default=12)
+    ap.add_argument("--cpu-planes", type=int, default=150)
     ap.add_argument("--cpu-seconds", type=float, default=25.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(streams shard across GPUs without any exchange; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
-    res = bench_ldpca(args)
+    res = bench_wz(args) if args.workload == "wz_cif_q8" else bench_ldpca(args)
     print(json.dumps(res), flush=True)
 
 

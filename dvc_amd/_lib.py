@@ -86,6 +86,7 @@ SIGNATURES = {
     "dvc_ldpca_decode": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_uint8,
                                     _c.c_int, _c.c_void_p, _c.POINTER(BpStats)]),
     "dvc_ldpca_decode_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(LdpcaItem)]),
+    "dvc_selftest_unary": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_int]),
     "dvc_decode_wz_frames": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WZFrameSpec), _c.c_int]),
 }
 for _name, (_res, _args) in SIGNATURES.items():
@@ -166,6 +167,13 @@ class Context:
         _check(lib.dvc_ldpca_decode(self.h, n, _ptr(lr), _ptr(acc), int(crc), int(start_level), _ptr(out),
                                     ctypes.byref(st)), "dvc_ldpca_decode")
         return out, st.as_dict()
+
+    def selftest(self, func, x):
+        """Device evaluation of a scalar routine (see dvc_selftest_unary)."""
+        x = np.ascontiguousarray(x)
+        out = np.zeros(x.size, np.float64 if func >= 4 else np.float32)
+        _check(lib.dvc_selftest_unary(self.h, func, _ptr(x), _ptr(out), x.size), "dvc_selftest_unary")
+        return out
 
     def decode_wz_frames(self, frames, batched=True):
         """frames: list of dicts with past, fut (uint8 HxW), orig (optional),

@@ -109,6 +109,12 @@ typedef struct {
  * sequential search selects.  Host pointers. */
 int dvc_ldpca_decode_batch(dvc_ctx *ctx, int n, dvc_ldpca_item *items);
 
+/* Verification entry (no counterpart in the reference): evaluates one of the
+ * device's scalar routines on n inputs.  func 0..3: float logf, expf, tanhf,
+ * the BP kernel phi(x) = -logf(tanhf((x+1e-9f)/2)) (float in, float out);
+ * 4: double sqrt of int32 inputs; 5: double exp (double in, double out). */
+int dvc_selftest_unary(dvc_ctx *ctx, int func, const void *in, void *out, int n);
+
 /* ---------------------------------------------------------------- WZ frames */
 typedef struct {
   int nbitplanes;

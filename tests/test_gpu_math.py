@@ -1,0 +1,64 @@
+"""The device's scalar routines against the host C library, on the GPU:
+the glibc-identical float functions on the BP/CNM domains, double sqrt on
+every integer the motion costs can take, and the double exp used by the
+reconstruction (whose few-ulp differences stay inside the rounding margin)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+LIBM = ctypes.CDLL("libm.so.6")
+
+
+def host(fn, x):
+    f = getattr(LIBM, fn)
+    f.restype = ctypes.c_float
+    f.argtypes = [ctypes.c_float]
+    return np.array([f(float(v)) for v in x], np.float32)
+
+
+def _bits_sample(lo, hi, n, seed):
+    rng = np.random.default_rng(seed)
+    b = rng.integers(lo, hi, n, dtype=np.int64).astype(np.uint32)
+    return b.view(np.float32)
+
+
+@pytest.mark.parametrize("func,name,lo,hi", [(0, "logf", 0x00000001, 0x7f800000), (1, "expf", 0x00000000, 0xffffffff),
+                                             (2, "tanhf", 0x00000000, 0xffffffff)])
+def test_float_routines_match_host(ctx, func, name, lo, hi):
+    x = _bits_sample(lo, hi, 200000, func)
+    x = x[np.isfinite(x)]
+    dev = ctx.selftest(func, x)
+    ref = host(name, x)
+    assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32))
+
+
+def test_bp_phi_full_grid_of_messages(ctx):
+    # every float in [0, 64) on a stride, plus the small magnitudes bit by bit
+    b = np.concatenate([np.arange(0, 0x42800000, 4099, dtype=np.uint32), np.arange(0, 1 << 20, dtype=np.uint32)])
+    x = b.view(np.float32)
+    dev = ctx.selftest(3, x)
+    logf = LIBM.logf; logf.restype = ctypes.c_float; logf.argtypes = [ctypes.c_float]
+    tanhf = LIBM.tanhf; tanhf.restype = ctypes.c_float; tanhf.argtypes = [ctypes.c_float]
+    eps = np.float32(0.000000001)
+    ref = np.array([-logf(tanhf(float((np.float32(v) + eps) / np.float32(2))))
+                    for v in x], np.float32)
+    assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32))
+
+
+def test_double_sqrt_exact_on_integer_costs(ctx):
+    # motion-vector lengths: x^2 + y^2 for half-pel components up to 2*352
+    n = np.arange(0, 2 * 704 * 704 + 1, dtype=np.int64)
+    n = n[n < 2 ** 31].astype(np.int32)
+    dev = ctx.selftest(4, n)
+    assert np.array_equal(dev, np.sqrt(n.astype(np.float64)))
+
+
+def test_double_exp_within_margin(ctx):
+    rng = np.random.default_rng(5)
+    x = np.concatenate([-rng.uniform(0, 200, 100000), rng.uniform(0, 200, 100000)])
+    dev = ctx.selftest(5, x)
+    ref = np.exp(x)
+    rel = np.abs(dev - ref) / ref
+    assert rel.max() < 1e-15
