@@ -72,6 +72,10 @@ class WZFrameSpec(ctypes.Structure):
                 ("recons", ctypes.c_void_p), ("stats", ctypes.POINTER(FrameStats))]
 
 
+class EngineParams(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int), ("sequential_bp", ctypes.c_int)]
+
+
 # Every symbol include/dvc_accel.h declares, with its ctypes signature.
 _c = ctypes
 SIGNATURES = {
@@ -88,6 +92,14 @@ SIGNATURES = {
     "dvc_ldpca_decode_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(LdpcaItem)]),
     "dvc_selftest_unary": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_int]),
     "dvc_decode_wz_frames": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WZFrameSpec), _c.c_int]),
+    "dvc_engine_create": (_c.c_int, [_c.c_void_p, _c.POINTER(EngineParams), _c.POINTER(_c.c_void_p)]),
+    "dvc_engine_destroy": (_c.c_int, [_c.c_void_p]),
+    "dvc_engine_submit": (_c.c_int, [_c.c_void_p, _c.POINTER(WZFrameSpec), _c.c_uint64, _c.c_uint64,
+                                     _c.POINTER(_c.c_uint64)]),
+    "dvc_engine_step": (_c.c_int, [_c.c_void_p, _c.POINTER(_c.c_int)]),
+    "dvc_engine_status": (_c.c_int, [_c.c_void_p, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int), _c.POINTER(_c.c_uint64)]),
+    "dvc_engine_done": (_c.c_int, [_c.c_void_p, _c.c_uint64, _c.POINTER(_c.c_int)]),
+    "dvc_engine_release_key": (_c.c_int, [_c.c_void_p, _c.c_uint64]),
 }
 for _name, (_res, _args) in SIGNATURES.items():
     _f = getattr(lib, _name)
@@ -217,3 +229,75 @@ class Context:
             items[i] = LdpcaItem(lr.size, lr.ctypes.data, acc.ctypes.data, int(crcs[i]), sl, out.ctypes.data, BpStats())
         _check(lib.dvc_ldpca_decode_batch(self.h, k, items), "dvc_ldpca_decode_batch")
         return outs, [items[i].stats.as_dict() for i in range(k)]
+
+
+def _frame_spec(fr, stats):
+    """Builds a WZFrameSpec; returns (spec, arrays to keep alive, recons)."""
+    past = np.ascontiguousarray(fr["past"], np.uint8)
+    fut = np.ascontiguousarray(fr["fut"], np.uint8)
+    orig = None if fr.get("orig") is None else np.ascontiguousarray(fr["orig"], np.uint8)
+    wz = fr["wz"]
+    acr = np.ascontiguousarray(wz.ac_range, np.int32)
+    crc = np.ascontiguousarray(wz.crc, np.uint8)
+    acc = np.ascontiguousarray(wz.acc, np.uint8)
+    rec = np.zeros_like(past)
+    pi, ci, fi = fr.get("idx", (0, 1, 2))
+    h, w = past.shape
+    spec = WZFrameSpec(w, h, wz.qindex, pi, ci, fi, past.ctypes.data, fut.ctypes.data,
+                       None if orig is None else orig.ctypes.data, acr.ctypes.data, crc.ctypes.data,
+                       acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats))
+    return spec, [past, fut, orig, acr, crc, acc, rec], rec
+
+
+class Engine:
+    """Continuous-batching decoder (dvc_engine_*): submit frames at any time,
+    step, and collect results as their tickets complete."""
+
+    def __init__(self, ctx, max_inflight=64, hbm_budget_bytes=0, sequential_bp=False):
+        p = EngineParams(int(hbm_budget_bytes), int(max_inflight), 1 if sequential_bp else 0)
+        h = ctypes.c_void_p()
+        _check(lib.dvc_engine_create(ctx.h, ctypes.byref(p), ctypes.byref(h)), "dvc_engine_create")
+        self.h = h
+        self.ctx = ctx
+        self.live = {}
+
+    def submit(self, fr, past_key=0, fut_key=0):
+        stats = FrameStats()
+        spec, keep, rec = _frame_spec(fr, stats)
+        t = ctypes.c_uint64()
+        _check(lib.dvc_engine_submit(self.h, ctypes.byref(spec), past_key, fut_key, ctypes.byref(t)), "dvc_engine_submit")
+        self.live[t.value] = (spec, keep, rec, stats)
+        return t.value
+
+    def step(self):
+        n = ctypes.c_int()
+        _check(lib.dvc_engine_step(self.h, ctypes.byref(n)), "dvc_engine_step")
+        return n.value
+
+    def done(self, ticket):
+        d = ctypes.c_int()
+        _check(lib.dvc_engine_done(self.h, ticket, ctypes.byref(d)), "dvc_engine_done")
+        return bool(d.value)
+
+    def result(self, ticket):
+        _, _, rec, stats = self.live.pop(ticket)
+        return rec, stats.as_dict()
+
+    def status(self):
+        q, i, u = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
+        _check(lib.dvc_engine_status(self.h, ctypes.byref(q), ctypes.byref(i), ctypes.byref(u)), "dvc_engine_status")
+        return q.value, i.value, u.value
+
+    def release_key(self, key):
+        _check(lib.dvc_engine_release_key(self.h, key), "dvc_engine_release_key")
+
+    def close(self):
+        if self.h:
+            lib.dvc_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

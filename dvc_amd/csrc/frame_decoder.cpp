@@ -18,6 +18,7 @@
 #include <math.h>
 #include <string.h>
 #include <algorithm>
+#include <deque>
 #include <map>
 #include <vector>
 #include "dvc_internal.h"
@@ -90,7 +91,12 @@ struct KeyFrameDev {
 };
 
 struct Job {
-  const dvc_wz_frame *spec = nullptr;
+  dvc_wz_frame spec{};           // caller's request; pointers must stay valid until completion
+  std::vector<int> acRange;      // owned copies of the bitstream fields
+  std::vector<uint8_t> crc, acc;
+  uint64_t ticket = 0, pastKey = 0, futKey = 0;
+  bool setup = false;
+  size_t bytes = 0;
   int W = 0, H = 0, N = 0, nbp = 0;
   std::vector<BandPlan> bands;
   int bitrate0 = 0;
@@ -120,6 +126,22 @@ struct Arena {
 
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
+// Device bytes one frame needs (the layout alloc_job carves).
+static size_t job_bytes(const Job &j) {
+  const int W = j.W, H = j.H, N = j.N, FS = W * H;
+  int maxStride = 0;
+  for (auto &b : j.bands) maxStride = std::max(maxStride, b.stride);
+  size_t t = 0;
+  for (size_t b : {(size_t)FS, 4 * (size_t)FS, 4 * (size_t)FS, 4 * (size_t)FS, 4 * (size_t)FS,
+                   (size_t)64 * (W - 3) * (H - 3), (size_t)4 * 81 * N, (size_t)4 * N * std::max(maxStride, 1),
+                   4 * (size_t)N, (size_t)N, (size_t)FS, sizeof(MVd) * (size_t)(FS / 256), sizeof(MVd) * (size_t)(FS / 64),
+                   sizeof(MVd) * (size_t)(FS / 64), sizeof(MVd) * (size_t)(FS / 256), 16 * (size_t)(FS / 256),
+                   8 * (size_t)(FS / 64), (size_t)4, 4 * (size_t)N, (size_t)j.nbp * (N / 8),
+                   sizeof(PlaneState) * (size_t)j.nbp, sizeof(int4) * 65 * (size_t)j.nbp, 4 * (size_t)N})
+    t = align_up(t + b);
+  return t;
+}
+
 // Carves the per-frame buffers out of one allocation.
 static int alloc_job(Job &j, Arena &arena) {
   const int W = j.W, H = j.H, N = j.N, FS = W * H;
@@ -136,8 +158,10 @@ static int alloc_job(Job &j, Arena &arena) {
                o_tl = take(4 * (size_t)N), o_acc = take((size_t)j.nbp * (N / 8)), o_state = take(sizeof(PlaneState) * (size_t)j.nbp),
                o_recd = take(sizeof(int4) * 65 * (size_t)j.nbp), o_llr = take(4 * (size_t)N);
   uint8_t *base = nullptr;
-  DVC_HIP(arena.alloc(&base, off));
+  (void)arena;
+  DVC_HIP(hipMalloc((void **)&base, off));
   j.mem = base;
+  j.bytes = off;
   FrameDev &f = j.fd;
   f.si = base + o_si; f.residue = (int *)(base + o_res); f.sideTrans = (int *)(base + o_st);
   f.reconsTrans = (int *)(base + o_rt); f.alpha = (float *)(base + o_al); f.oct = (int *)(base + o_oct);
@@ -257,189 +281,298 @@ int dvc::bp_step(dvc_ctx *c, int N, const std::vector<PlaneDesc> &planes, bool s
   return DVC_OK;
 }
 
-extern "C" int dvc_decode_wz_frames(dvc_ctx *c, int n, const dvc_wz_frame *frames, int batched) {
-  DVC_CHECK(c, DVC_EINVAL, "dvc_decode_wz_frames: NULL context");
-  DVC_CHECK(n >= 0 && (n == 0 || frames), DVC_EINVAL, "dvc_decode_wz_frames: bad frame list");
-  if (n == 0) return DVC_OK;
-  DVC_HIP(hipSetDevice(c->device));
-  int rc = ensure_cost_tables(c);
-  if (rc) return rc;
-  hipStream_t s = c->stream;
-  Arena arena;
-  // ---- key frames: upload and filter each distinct host frame once
-  std::map<const uint8_t *, KeyFrameDev> keys;
-  std::vector<Job> jobs(n);
-  for (int i = 0; i < n; i++) {
-    const dvc_wz_frame &sp = frames[i];
-    DVC_CHECK((sp.width == 176 && sp.height == 144) || (sp.width == 352 && sp.height == 288), DVC_EINVAL,
-              "dvc_decode_wz_frames: frame size must be QCIF or CIF");
-    DVC_CHECK(sp.qindex >= 1 && sp.qindex <= 8, DVC_EINVAL, "dvc_decode_wz_frames: qindex must be 1..8");
-    DVC_CHECK(sp.past && sp.fut && sp.ac_range && sp.crc && sp.acc && sp.recons, DVC_EINVAL,
-              "dvc_decode_wz_frames: NULL frame pointer");
-    DVC_CHECK(sp.past_idx < sp.cur_idx && sp.cur_idx < sp.fut_idx, DVC_EINVAL,
-              "dvc_decode_wz_frames: need past_idx < cur_idx < fut_idx");
-    Job &j = jobs[i];
-    j.spec = &sp;
-    j.W = sp.width; j.H = sp.height; j.N = j.W * j.H / 16;
-    j.bands = band_plan(sp.qindex, sp.ac_range, &j.bitrate0);
-    for (auto &b : j.bands) j.nbp += b.qbits;
-    for (const uint8_t *kp : {sp.past, sp.fut}) {
-      if (keys.count(kp)) continue;
-      KeyFrameDev k;
-      const int FS = j.W * j.H;
-      DVC_HIP(arena.alloc(&k.full, FS));
-      DVC_HIP(arena.alloc(&k.lp, FS));
-      DVC_HIP(arena.alloc(&k.half, 4 * (size_t)FS));
-      DVC_HIP(hipMemcpyAsync(k.full, kp, FS, hipMemcpyHostToDevice, s));
-      TIMED_LAUNCH(c, "keyframe_filters", s, launch_keyframe_filters(k.full, k.lp, k.half, j.W, j.H, s));
-      keys[kp] = k;
-    }
-    rc = alloc_job(j, arena);
-    if (rc) return rc;
-    FrameDev &f = j.fd;
-    f.W = j.W; f.H = j.H; f.N = j.N; f.qindex = sp.qindex;
-    f.iPC = (double)(sp.cur_idx - sp.past_idx) / (double)(sp.fut_idx - sp.past_idx);
-    f.iFC = (double)(sp.fut_idx - sp.cur_idx) / (double)(sp.fut_idx - sp.past_idx);
-    const KeyFrameDev &kp = keys[sp.past], &kf = keys[sp.fut];
-    f.past = kp.full; f.past_lp = kp.lp; f.past_half = kp.half;
-    f.fut = kf.full; f.fut_lp = kf.lp; f.fut_half = kf.half;
-    DVC_HIP(hipMemcpyAsync(j.d_acc, sp.acc, (size_t)j.nbp * (j.N / 8), hipMemcpyHostToDevice, s));
-    DVC_HIP(hipMemsetAsync(f.reconsTrans, 0, sizeof(int) * j.W * j.H, s));
-    std::vector<int4> recInit((size_t)65 * j.nbp, make_int4(0, REC_NOT_RUN, -1, 0));
-    DVC_HIP(hipMemcpy(j.d_rec, recInit.data(), recInit.size() * sizeof(int4), hipMemcpyHostToDevice));
+
+// ---------------------------------------------------------------- engine
+namespace dvc {
+struct KeyEntry {
+  KeyFrameDev k;
+  int W = 0, H = 0;
+  int refs = 0;
+  bool named = false;     // kept resident after its last use (a stream key)
+  uint64_t lastUse = 0;
+  size_t bytes = 0;
+};
+
+}  // namespace dvc
+
+struct dvc_engine {
+  dvc_ctx *c = nullptr;
+  size_t budget = 0, used = 0;
+  int maxInflight = 1;
+  bool sequentialBp = false;
+  uint64_t nextTicket = 1, clock = 0;
+  std::deque<dvc::Job *> pending;
+  std::vector<dvc::Job *> inflight;
+  std::map<uint64_t, int> doneTickets;  // ticket -> 1
+  std::map<uint64_t, dvc::KeyEntry> keys;
+  // device scratch sized for maxInflight frames
+  dvc::FrameDev *d_frames = nullptr;
+  int *d_lists = nullptr;
+  dvc::PlaneDesc *d_planes = nullptr;
+  dvc::BpFrameLink *d_links = nullptr;
+  uint8_t *d_slots = nullptr;
+  int *d_counters = nullptr;
+  int groupCap = 0;
+  long hostFixups = 0;
+};
+
+namespace dvc {
+
+static void free_job(dvc_engine *e, Job *j) {
+  if (j->mem) (void)hipFree(j->mem);
+  e->used -= j->bytes;
+  for (uint64_t k : {j->pastKey, j->futKey}) {
+    auto it = e->keys.find(k);
+    if (it == e->keys.end()) continue;
+    it->second.refs--;
   }
-  // ---- descriptors
-  FrameDev *d_frames;
-  int *d_lists;  // up to 4 lists of n entries
-  DVC_HIP(arena.alloc(&d_frames, n));
-  DVC_HIP(arena.alloc(&d_lists, 4 * (size_t)n));
+  delete j;
+}
+
+// Drops unreferenced anonymous key frames, then the least recently used named
+// ones while the budget is exceeded.
+static void trim_keys(dvc_engine *e, size_t need) {
+  for (auto it = e->keys.begin(); it != e->keys.end();) {
+    if (it->second.refs == 0 && !it->second.named) {
+      KeyFrameDev &k = it->second.k;
+      (void)hipFree(k.full);
+      e->used -= it->second.bytes;
+      it = e->keys.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  while (e->used + need > e->budget) {
+    auto victim = e->keys.end();
+    for (auto it = e->keys.begin(); it != e->keys.end(); ++it)
+      if (it->second.refs == 0 && (victim == e->keys.end() || it->second.lastUse < victim->second.lastUse)) victim = it;
+    if (victim == e->keys.end()) break;
+    (void)hipFree(victim->second.k.full);
+    e->used -= victim->second.bytes;
+    e->keys.erase(victim);
+  }
+}
+
+static int key_frame(dvc_engine *e, uint64_t key, bool named, const uint8_t *host, int W, int H, hipStream_t s,
+                     KeyFrameDev *out) {
+  auto it = e->keys.find(key);
+  if (it != e->keys.end()) {
+    DVC_CHECK(it->second.W == W && it->second.H == H, DVC_EINVAL, "key frame reused with another size");
+    it->second.refs++;
+    it->second.lastUse = ++e->clock;
+    *out = it->second.k;
+    return DVC_OK;
+  }
+  KeyEntry ke;
+  const size_t FS = (size_t)W * H;
+  ke.bytes = 6 * FS;
+  uint8_t *p = nullptr;
+  DVC_HIP(hipMalloc((void **)&p, ke.bytes));
+  ke.k.full = p; ke.k.lp = p + FS; ke.k.half = p + 2 * FS;
+  ke.W = W; ke.H = H; ke.refs = 1; ke.named = named; ke.lastUse = ++e->clock;
+  DVC_HIP(hipMemcpyAsync(ke.k.full, host, FS, hipMemcpyHostToDevice, s));
+  TIMED_LAUNCH(e->c, "keyframe_filters", s, launch_keyframe_filters(ke.k.full, ke.k.lp, ke.k.half, W, H, s));
+  e->used += ke.bytes;
+  e->keys[key] = ke;
+  *out = ke.k;
+  return DVC_OK;
+}
+
+static int admit(dvc_engine *e, Job *j, hipStream_t s) {
+  dvc_ctx *c = e->c;
+  Arena unused;
+  int rc = alloc_job(*j, unused);
+  if (rc) return rc;
+  e->used += j->bytes;
+  FrameDev &f = j->fd;
+  const dvc_wz_frame &sp = j->spec;
+  f.W = j->W; f.H = j->H; f.N = j->N; f.qindex = sp.qindex;
+  f.iPC = (double)(sp.cur_idx - sp.past_idx) / (double)(sp.fut_idx - sp.past_idx);
+  f.iFC = (double)(sp.fut_idx - sp.cur_idx) / (double)(sp.fut_idx - sp.past_idx);
+  KeyFrameDev kp, kf;
+  const bool pn = j->pastKey != (uint64_t)(uintptr_t)sp.past, fn = j->futKey != (uint64_t)(uintptr_t)sp.fut;
+  if ((rc = key_frame(e, j->pastKey, pn, sp.past, j->W, j->H, s, &kp))) return rc;
+  if ((rc = key_frame(e, j->futKey, fn, sp.fut, j->W, j->H, s, &kf))) return rc;
+  f.past = kp.full; f.past_lp = kp.lp; f.past_half = kp.half;
+  f.fut = kf.full; f.fut_lp = kf.lp; f.fut_half = kf.half;
+  DVC_HIP(hipMemcpyAsync(j->d_acc, j->acc.data(), j->acc.size(), hipMemcpyHostToDevice, s));
+  DVC_HIP(hipMemsetAsync(f.reconsTrans, 0, sizeof(int) * j->W * j->H, s));
+  std::vector<int4> recInit((size_t)65 * j->nbp, make_int4(0, REC_NOT_RUN, -1, 0));
+  DVC_HIP(hipMemcpyAsync(j->d_rec, recInit.data(), recInit.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+  DVC_HIP(hipStreamSynchronize(s));  // recInit lives on this stack frame
+  j->bandPos = 0;
+  j->bit = j->bands.empty() ? 0 : j->bands[0].qbits;
+  j->bp = 0;
+  (void)c;
+  return DVC_OK;
+}
+
+static int finish_job(dvc_engine *e, Job &j, hipStream_t s) {
+  dvc_ctx *c = e->c;
+  DVC_HIP(hipMemcpyAsync(j.spec.recons, j.fd.recons, j.W * j.H, hipMemcpyDeviceToHost, s));
+  std::vector<PlaneState> ps(j.nbp);
+  std::vector<int4> rec((size_t)65 * j.nbp);
+  DVC_HIP(hipMemcpyAsync(ps.data(), j.d_state, sizeof(PlaneState) * j.nbp, hipMemcpyDeviceToHost, s));
+  DVC_HIP(hipMemcpyAsync(rec.data(), j.d_rec, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost, s));
+  DVC_HIP(hipStreamSynchronize(s));
+  dvc_frame_stats *st = j.spec.stats;
+  if (!st) return DVC_OK;
+  memset(st, 0, sizeof(*st));
+  st->nbitplanes = j.nbp;
+  st->bitrate = j.bitrate0;
+  Ladder *L = c->ladder(j.N);
+  for (int b = 0; b < j.nbp && b < 128; b++) {
+    const int4 *r = &rec[(size_t)65 * b];
+    int lvl = -1, it = 0;
+    for (int l = std::max(ps[b].start, 0); l < 65; l++) {
+      if (r[l].y == REC_ACCEPTED) { lvl = l; break; }
+      it += r[l].x;
+    }
+    DVC_CHECK(lvl >= 0, DVC_ESTATE, "WZ frame decode: bitplane without accepted code");
+    st->levels[b] = lvl;
+    st->start_levels[b] = ps[b].start;
+    st->iterations[b] = it + r[lvl].x;
+    st->used_inverse[b] = r[lvl].w;
+    st->entropy[b] = ps[b].entropy;
+    st->bitrate += L->syndLength[lvl] + 8;
+  }
+  st->recon_host_fixups = (int)e->hostFixups;
+  if (j.spec.orig) {
+    double mse = 0.0;
+    const int FS = j.W * j.H;
+    for (int p = 0; p < FS; p++) {
+      const double d = (double)j.spec.recons[p] - (double)j.spec.orig[p];
+      mse += d * d;
+    }
+    mse /= (double)FS;
+    st->psnr = mse == 0.0 ? 0.0 : 10.0 * log10(255.0 * 255.0 / mse);  // frame.cpp:115-124
+  }
+  return DVC_OK;
+}
+
+// One engine step: admit queued frames while the HBM budget allows, set up the
+// newcomers, run one bitplane of every in-flight frame (with the band starts
+// and ends it implies), and retire finished frames.
+static int engine_step(dvc_engine *e, int *completed) {
+  dvc_ctx *c = e->c;
+  hipStream_t s = c->stream;
+  int rc;
+  *completed = 0;
+  // ---- admission
+  std::vector<Job *> fresh;
+  while (!e->pending.empty() && (int)e->inflight.size() < e->maxInflight) {
+    Job *j = e->pending.front();
+    const size_t keyNeed = 12 * (size_t)j->W * j->H;
+    trim_keys(e, j->bytes + keyNeed);
+    if (!e->inflight.empty() && e->used + j->bytes + keyNeed > e->budget) break;
+    e->pending.pop_front();
+    if ((rc = admit(e, j, s))) { delete j; return rc; }
+    e->inflight.push_back(j);
+    fresh.push_back(j);
+  }
+  const int n = (int)e->inflight.size();
+  if (n == 0) return DVC_OK;
   std::vector<FrameDev> hf(n);
-  auto upload_frames = [&]() -> int {
-    for (int i = 0; i < n; i++) hf[i] = jobs[i].fd;
-    DVC_HIP(hipMemcpyAsync(d_frames, hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
-    return DVC_OK;
-  };
-  std::vector<int> hlists(4 * (size_t)n);
-  auto upload_list = [&](int slot, const std::vector<int> &l) -> int {
-    std::copy(l.begin(), l.end(), hlists.begin() + (size_t)slot * n);
-    DVC_HIP(hipMemcpyAsync(d_lists + (size_t)slot * n, l.data(), sizeof(int) * l.size(), hipMemcpyHostToDevice, s));
-    return DVC_OK;
-  };
-  CostTables ct{c->d_costFme, c->d_costUsi};
-  std::vector<int> all(n);
+  std::vector<int> hl0, hl1, hl2, hl3;
   int maxN = 0, maxMB = 0, maxPos = 0;
   for (int i = 0; i < n; i++) {
-    all[i] = i;
-    maxN = std::max(maxN, jobs[i].N);
-    maxMB = std::max(maxMB, jobs[i].W * jobs[i].H / 256);
-    maxPos = std::max(maxPos, (jobs[i].W - 3) * (jobs[i].H - 3));
+    Job *j = e->inflight[i];
+    maxN = std::max(maxN, j->N);
+    maxMB = std::max(maxMB, j->W * j->H / 256);
+    maxPos = std::max(maxPos, (j->W - 3) * (j->H - 3));
   }
-  if ((rc = upload_frames()) || (rc = upload_list(0, all))) return rc;
-  // ---- side information, noise parameters, over-complete transform, SMF
-  TIMED_LAUNCH(c, "create_si", s, launch_create_si(d_frames, d_lists, n, maxMB, ct, s));
-  TIMED_LAUNCH(c, "noise_param", s, launch_noise_param(d_frames, d_lists, n, maxN, s));
-  TIMED_LAUNCH(c, "oct", s, launch_oct(d_frames, d_lists, n, maxPos, 1, s));
-  TIMED_LAUNCH(c, "smf_init", s, launch_smf_init(d_frames, d_lists, n, maxN, s));
-
-  // ---- bitplane loop
-  // every (bitplane, code) item accepts at most once, plus one full-rate slot:
-  // 65 slots per frame and code-length group can never overflow
-  const int groupCap = 65 * n + 1;
-  int slotCap = 2 * groupCap;
-  uint8_t *d_slots;
-  int *d_counters;
-  PlaneDesc *d_planes;
-  BpFrameLink *d_links;
-  DVC_HIP(arena.alloc(&d_slots, (size_t)slotCap * 6336));
-  DVC_HIP(arena.alloc(&d_counters, 2));
-  DVC_HIP(arena.alloc(&d_planes, n));
-  DVC_HIP(arena.alloc(&d_links, n));
-  for (auto &j : jobs) { j.bandPos = 0; j.bit = j.bands.empty() ? 0 : j.bands[0].qbits; j.bp = 0; }
-  int host_fixups = 0;
-  for (;;) {
-    std::vector<int> active, starting, ending;
-    for (int i = 0; i < n; i++) {
-      Job &j = jobs[i];
-      if (j.done()) continue;
-      const BandPlan &b = j.bands[j.bandPos];
-      if (j.bit == b.qbits) {
-        starting.push_back(i);
-        j.fd.band = b.band; j.fd.index = b.index; j.fd.range = b.range; j.fd.step = b.step;
-        j.fd.levelShift = b.levelShift; j.fd.distStride = b.stride;
-      }
-      j.fd.bit = j.bit;
-      active.push_back(i);
-      if (j.bit == 1) ending.push_back(i);
+  CostTables ct{c->d_costFme, c->d_costUsi};
+  const int *L0 = e->d_lists, *L1 = e->d_lists + e->maxInflight, *L2 = e->d_lists + 2 * e->maxInflight,
+            *L3 = e->d_lists + 3 * e->maxInflight;
+  auto up_frames = [&]() -> int {
+    for (int i = 0; i < n; i++) hf[i] = e->inflight[i]->fd;
+    DVC_HIP(hipMemcpyAsync(e->d_frames, hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
+    return DVC_OK;
+  };
+  auto up_list = [&](const int *dst, const std::vector<int> &l) -> int {
+    DVC_HIP(hipMemcpyAsync((void *)dst, l.data(), sizeof(int) * l.size(), hipMemcpyHostToDevice, s));
+    return DVC_OK;
+  };
+  // ---- setup of newly admitted frames: SI, noise parameters, OCT, SMF
+  for (int i = 0; i < n; i++)
+    if (!e->inflight[i]->setup) hl3.push_back(i);
+  // band parameters of this step
+  std::vector<int> active, starting, ending;
+  for (int i = 0; i < n; i++) {
+    Job &j = *e->inflight[i];
+    if (j.done()) continue;
+    const BandPlan &b = j.bands[j.bandPos];
+    if (j.bit == b.qbits) {
+      starting.push_back(i);
+      j.fd.band = b.band; j.fd.index = b.index; j.fd.range = b.range; j.fd.step = b.step;
+      j.fd.levelShift = b.levelShift; j.fd.distStride = b.stride;
     }
-    if (active.empty()) break;
-    if ((rc = upload_frames())) return rc;
+    j.fd.bit = j.bit;
+    active.push_back(i);
+    if (j.bit == 1) ending.push_back(i);
+  }
+  if ((rc = up_frames())) return rc;
+  if (!hl3.empty()) {
+    const int nf = (int)hl3.size();
+    if ((rc = up_list(L3, hl3))) return rc;
+    TIMED_LAUNCH(c, "create_si", s, launch_create_si(e->d_frames, L3, nf, maxMB, ct, s));
+    TIMED_LAUNCH(c, "noise_param", s, launch_noise_param(e->d_frames, L3, nf, maxN, s));
+    TIMED_LAUNCH(c, "oct", s, launch_oct(e->d_frames, L3, nf, maxPos, 1, s));
+    TIMED_LAUNCH(c, "smf_init", s, launch_smf_init(e->d_frames, L3, nf, maxN, s));
+    for (int i : hl3) e->inflight[i]->setup = true;
+  }
+  if (!active.empty()) {
     if (!starting.empty()) {
       int maxStride = 0;
-      for (int i : starting) {
-        maxStride = std::max(maxStride, jobs[i].fd.distStride);
-        DVC_CHECK(jobs[i].fd.index == 0 || jobs[i].fd.range >= 0, DVC_EINVAL, "bad coefficient range");
-      }
-      if ((rc = upload_list(1, starting))) return rc;
-      TIMED_LAUNCH(c, "noise_dist", s, launch_noise_dist(d_frames, d_lists + n, (int)starting.size(), maxN, maxStride, s));
+      for (int i : starting) maxStride = std::max(maxStride, e->inflight[i]->fd.distStride);
+      if ((rc = up_list(L1, starting))) return rc;
+      TIMED_LAUNCH(c, "noise_dist", s, launch_noise_dist(e->d_frames, L1, (int)starting.size(), maxN, maxStride, s));
     }
-    if ((rc = upload_list(0, active))) return rc;
-    TIMED_LAUNCH(c, "condbit", s, launch_condbit(d_frames, d_lists, (int)active.size(), maxN, s));
-    // LDPCA for this bitplane of every active frame, grouped by code length
+    if ((rc = up_list(L0, active))) return rc;
+    TIMED_LAUNCH(c, "condbit", s, launch_condbit(e->d_frames, L0, (int)active.size(), maxN, s));
     std::vector<PlaneDesc> groups[2];
     std::vector<BpFrameLink> links(active.size());
     std::vector<PlaneState> init(active.size(), PlaneState{-1, 64, 0.0f, 0});
     for (size_t a = 0; a < active.size(); a++) {
-      Job &j = jobs[active[a]];
+      Job &j = *e->inflight[active[a]];
       DVC_HIP(hipMemcpyAsync(j.d_state + j.bp, &init[a], sizeof(PlaneState), hipMemcpyHostToDevice, s));
-      PlaneDesc pd{j.N, j.fd.LR, j.d_LLR, j.d_acc + (size_t)j.bp * (j.N / 8), (uint32_t)j.spec->crc[j.bp],
+      PlaneDesc pd{j.N, j.fd.LR, j.d_LLR, j.d_acc + (size_t)j.bp * (j.N / 8), (uint32_t)j.crc[j.bp],
                    j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp};
       groups[j.N == 6336].push_back(pd);
-      links[a] = BpFrameLink{j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp, d_slots, groupCap};
+      links[a] = BpFrameLink{j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp,
+                             e->d_slots + (size_t)(j.N == 6336 ? e->groupCap : 0) * 6336, e->groupCap};
     }
     DVC_HIP(hipStreamSynchronize(s));
     int planeOff = 0;
     for (int g = 0; g < 2; g++) {
       if (groups[g].empty()) continue;
-      // each group uses its own slice of the slot pool
-      const int cap = groupCap;
-      rc = bp_step(c, g ? 6336 : 1584, groups[g], !batched, d_slots + (size_t)(g ? cap : 0) * 6336, cap,
-                   d_counters, d_planes + planeOff, s);
+      rc = bp_step(c, g ? 6336 : 1584, groups[g], e->sequentialBp, e->d_slots + (size_t)(g ? e->groupCap : 0) * 6336,
+                   e->groupCap, e->d_counters, e->d_planes + planeOff, s);
       if (rc) return rc;
       planeOff += (int)groups[g].size();
       DVC_HIP(hipStreamSynchronize(s));
     }
-    // slots of group 1 start at cap * 6336 bytes; k_put_bitplane indexes slots by N, so give it the
-    // group-specific base pointer and per-N stride through the link.
-    for (size_t a = 0; a < active.size(); a++) {
-      Job &j = jobs[active[a]];
-      links[a].slots = d_slots + (size_t)(j.N == 6336 ? groupCap : 0) * 6336;
-      links[a].slot_capacity = groupCap;
-    }
-    DVC_HIP(hipMemcpyAsync(d_links, links.data(), sizeof(BpFrameLink) * links.size(), hipMemcpyHostToDevice, s));
-    TIMED_LAUNCH(c, "put_bitplane", s, launch_put_bitplane(d_frames, d_lists, d_links, (int)active.size(), maxN, s));
+    DVC_HIP(hipMemcpyAsync(e->d_links, links.data(), sizeof(BpFrameLink) * links.size(), hipMemcpyHostToDevice, s));
+    TIMED_LAUNCH(c, "put_bitplane", s, launch_put_bitplane(e->d_frames, L0, e->d_links, (int)active.size(), maxN, s));
     for (int i : active) {
-      Job &j = jobs[i];
-      j.bp++;
-      j.bit--;
+      e->inflight[i]->bp++;
+      e->inflight[i]->bit--;
     }
     if (!ending.empty()) {
-      if ((rc = upload_list(2, ending))) return rc;
-      const int *le = d_lists + 2 * (size_t)n;
       const int ne = (int)ending.size();
-      for (int i : ending) DVC_HIP(hipMemsetAsync(jobs[i].fd.tieCount, 0, sizeof(int), s));
-      TIMED_LAUNCH(c, "reconstruct", s, launch_reconstruct(d_frames, le, ne, maxN, s));
-      // entries near a rounding boundary: recompute with the C library's exp
+      if ((rc = up_list(L2, ending))) return rc;
+      for (int i : ending) DVC_HIP(hipMemsetAsync(e->inflight[i]->fd.tieCount, 0, sizeof(int), s));
+      TIMED_LAUNCH(c, "reconstruct", s, launch_reconstruct(e->d_frames, L2, ne, maxN, s));
       std::vector<int> ties(ne);
       for (int k = 0; k < ne; k++)
-        DVC_HIP(hipMemcpyAsync(&ties[k], jobs[ending[k]].fd.tieCount, sizeof(int), hipMemcpyDeviceToHost, s));
+        DVC_HIP(hipMemcpyAsync(&ties[k], e->inflight[ending[k]]->fd.tieCount, sizeof(int), hipMemcpyDeviceToHost, s));
       DVC_HIP(hipStreamSynchronize(s));
       for (int k = 0; k < ne; k++) {
-        Job &j = jobs[ending[k]];
-        const int cnt = std::min(ties[k], j.N);
+        Job &j = *e->inflight[ending[k]];
         DVC_CHECK(ties[k] <= j.N, DVC_ESTATE, "reconstruction tie list overflow");
-        if (cnt == 0) continue;
-        std::vector<int> idx(cnt);
-        DVC_HIP(hipMemcpy(idx.data(), j.fd.tieList, sizeof(int) * cnt, hipMemcpyDeviceToHost));
+        if (ties[k] == 0) continue;
+        std::vector<int> idx(ties[k]);
+        DVC_HIP(hipMemcpy(idx.data(), j.fd.tieList, sizeof(int) * ties[k], hipMemcpyDeviceToHost));
         for (int ci : idx) {
           int rt, side;
           float al;
@@ -448,69 +581,175 @@ extern "C" int dvc_decode_wz_frames(dvc_ctx *c, int n, const dvc_wz_frame *frame
           DVC_HIP(hipMemcpy(&al, j.fd.alpha + ci, sizeof(float), hipMemcpyDeviceToHost));
           int v = recon_host(rt, side, al, j.fd.index, j.fd.step, j.fd.levelShift);
           DVC_HIP(hipMemcpy(j.fd.sideTrans + ci, &v, sizeof(int), hipMemcpyHostToDevice));
-          host_fixups++;
+          e->hostFixups++;
         }
       }
-      TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(d_frames, le, ne, maxN, s));
-      TIMED_LAUNCH(c, "update_si", s, launch_update_si(d_frames, le, ne, maxN, ct, s));
-      TIMED_LAUNCH(c, "oct", s, launch_oct(d_frames, le, ne, maxPos, 0, s));
-      TIMED_LAUNCH(c, "update_smf", s, launch_update_smf(d_frames, le, ne, maxN, s));
-      TIMED_LAUNCH(c, "update_si_ml", s, launch_update_si_ml(d_frames, le, ne, maxN, 0, s));
+      TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(e->d_frames, L2, ne, maxN, s));
+      TIMED_LAUNCH(c, "update_si", s, launch_update_si(e->d_frames, L2, ne, maxN, ct, s));
+      TIMED_LAUNCH(c, "oct", s, launch_oct(e->d_frames, L2, ne, maxPos, 0, s));
+      TIMED_LAUNCH(c, "update_smf", s, launch_update_smf(e->d_frames, L2, ne, maxN, s));
+      TIMED_LAUNCH(c, "update_si_ml", s, launch_update_si_ml(e->d_frames, L2, ne, maxN, 0, s));
       for (int i : ending) {
-        Job &j = jobs[i];
+        Job &j = *e->inflight[i];
         j.bandPos++;
         if (!j.done()) j.bit = j.bands[j.bandPos].qbits;
       }
     }
-    // host buffers of this step (lists, links, descriptors) are reused next step
-    DVC_HIP(hipStreamSynchronize(s));
   }
-  // ---- final inverse transform (DVC_Decoder.cpp:411-423), outputs and statistics
-  if ((rc = upload_list(3, all))) return rc;
-  TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(d_frames, d_lists + 3 * (size_t)n, n, maxN, s));
-  for (int i = 0; i < n; i++) {
-    Job &j = jobs[i];
-    DVC_HIP(hipMemcpyAsync(j.spec->recons, j.fd.recons, j.W * j.H, hipMemcpyDeviceToHost, s));
-  }
-  DVC_HIP(hipStreamSynchronize(s));
-  for (int i = 0; i < n; i++) {
-    Job &j = jobs[i];
-    dvc_frame_stats *st = j.spec->stats;
-    if (!st) continue;
-    memset(st, 0, sizeof(*st));
-    st->nbitplanes = j.nbp;
-    st->bitrate = j.bitrate0;
-    std::vector<PlaneState> ps(j.nbp);
-    std::vector<int4> rec((size_t)65 * j.nbp);
-    DVC_HIP(hipMemcpy(ps.data(), j.d_state, sizeof(PlaneState) * j.nbp, hipMemcpyDeviceToHost));
-    DVC_HIP(hipMemcpy(rec.data(), j.d_rec, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost));
-    Ladder *L = c->ladder(j.N);
-    for (int b = 0; b < j.nbp && b < 128; b++) {
-      const int4 *r = &rec[(size_t)65 * b];
-      int lvl = -1, it = 0;
-      for (int l = ps[b].start; l < 65; l++) {
-        if (r[l].y == REC_ACCEPTED) { lvl = l; break; }
-        it += r[l].x;
+  // ---- retire: final inverse transform (DVC_Decoder.cpp:411-423) and outputs
+  std::vector<int> fin;
+  for (int i = 0; i < n; i++)
+    if (e->inflight[i]->setup && e->inflight[i]->done()) fin.push_back(i);
+  if (!fin.empty()) {
+    if ((rc = up_list(L3, fin))) return rc;
+    TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(e->d_frames, L3, (int)fin.size(), maxN, s));
+    for (int i : fin)
+      if ((rc = finish_job(e, *e->inflight[i], s))) return rc;
+    std::vector<Job *> keep;
+    for (int i = 0; i < n; i++) {
+      Job *j = e->inflight[i];
+      if (j->setup && j->done()) {
+        e->doneTickets[j->ticket] = 1;
+        free_job(e, j);
+        (*completed)++;
+      } else {
+        keep.push_back(j);
       }
-      DVC_CHECK(lvl >= 0, DVC_ESTATE, "dvc_decode_wz_frames: bitplane without accepted code");
-      st->levels[b] = lvl;
-      st->start_levels[b] = ps[b].start;
-      st->iterations[b] = it + r[lvl].x;
-      st->used_inverse[b] = r[lvl].w;
-      st->entropy[b] = ps[b].entropy;
-      st->bitrate += L->syndLength[lvl] + 8;
     }
-    st->recon_host_fixups = host_fixups;
-    if (j.spec->orig) {
-      double mse = 0.0;
-      const int FS = j.W * j.H;
-      for (int p = 0; p < FS; p++) {
-        const double d = (double)j.spec->recons[p] - (double)j.spec->orig[p];
-        mse += d * d;
-      }
-      mse /= (double)FS;
-      st->psnr = mse == 0.0 ? 0.0 : 10.0 * log10(255.0 * 255.0 / mse);  // frame.cpp:115-124
-    }
+    e->inflight.swap(keep);
   }
+  DVC_HIP(hipStreamSynchronize(s));  // host-side step buffers are reused next step
   return DVC_OK;
 }
+
+static int engine_submit(dvc_engine *e, const dvc_wz_frame *sp, uint64_t pastKey, uint64_t futKey, uint64_t *ticket) {
+  DVC_CHECK(sp, DVC_EINVAL, "dvc_engine_submit: NULL frame");
+  DVC_CHECK((sp->width == 176 && sp->height == 144) || (sp->width == 352 && sp->height == 288), DVC_EINVAL,
+            "WZ frame size must be QCIF or CIF");
+  DVC_CHECK(sp->qindex >= 1 && sp->qindex <= 8, DVC_EINVAL, "qindex must be 1..8");
+  DVC_CHECK(sp->past && sp->fut && sp->ac_range && sp->crc && sp->acc && sp->recons, DVC_EINVAL,
+            "WZ frame: NULL pointer");
+  DVC_CHECK(sp->past_idx < sp->cur_idx && sp->cur_idx < sp->fut_idx, DVC_EINVAL, "need past_idx < cur_idx < fut_idx");
+  Job *j = new Job();
+  j->spec = *sp;
+  j->W = sp->width; j->H = sp->height; j->N = j->W * j->H / 16;
+  j->acRange.assign(sp->ac_range, sp->ac_range + 16);
+  j->bands = band_plan(sp->qindex, j->acRange.data(), &j->bitrate0);
+  for (auto &b : j->bands) j->nbp += b.qbits;
+  j->crc.assign(sp->crc, sp->crc + j->nbp);
+  j->acc.assign(sp->acc, sp->acc + (size_t)j->nbp * (j->N / 8));
+  j->bytes = job_bytes(*j);
+  j->pastKey = pastKey ? pastKey : (uint64_t)(uintptr_t)sp->past;
+  j->futKey = futKey ? futKey : (uint64_t)(uintptr_t)sp->fut;
+  j->ticket = e->nextTicket++;
+  e->pending.push_back(j);
+  if (ticket) *ticket = j->ticket;
+  return DVC_OK;
+}
+
+static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **out) {
+  DVC_CHECK(c && out, DVC_EINVAL, "dvc_engine_create: NULL argument");
+  DVC_HIP(hipSetDevice(c->device));
+  int rc = ensure_cost_tables(c);
+  if (rc) return rc;
+  dvc_engine *e = new dvc_engine();
+  e->c = c;
+  e->maxInflight = (p && p->max_inflight > 0) ? p->max_inflight : 64;
+  e->sequentialBp = p && p->sequential_bp;
+  size_t freeB = 0, totalB = 0;
+  (void)hipMemGetInfo(&freeB, &totalB);
+  e->budget = (p && p->hbm_budget_bytes) ? (size_t)p->hbm_budget_bytes : (size_t)(0.8 * (double)freeB);
+  const int M = e->maxInflight;
+  e->groupCap = 65 * M + 1;
+  hipError_t err = hipSuccess;
+  if (err == hipSuccess) err = hipMalloc((void **)&e->d_frames, sizeof(FrameDev) * M);
+  if (err == hipSuccess) err = hipMalloc((void **)&e->d_lists, sizeof(int) * 4 * (size_t)M);
+  if (err == hipSuccess) err = hipMalloc((void **)&e->d_planes, sizeof(PlaneDesc) * M);
+  if (err == hipSuccess) err = hipMalloc((void **)&e->d_links, sizeof(BpFrameLink) * M);
+  if (err == hipSuccess) err = hipMalloc((void **)&e->d_slots, (size_t)2 * e->groupCap * 6336);
+  if (err == hipSuccess) err = hipMalloc((void **)&e->d_counters, 2 * sizeof(int));
+  if (err != hipSuccess) {
+    set_error(std::string("dvc_engine_create: ") + hipGetErrorString(err));
+    delete e;
+    return DVC_ENOMEM;
+  }
+  *out = e;
+  return DVC_OK;
+}
+
+static void engine_destroy(dvc_engine *e) {
+  if (!e) return;
+  (void)hipSetDevice(e->c->device);
+  (void)hipStreamSynchronize(e->c->stream);
+  for (Job *j : e->pending) delete j;
+  for (Job *j : e->inflight) { if (j->mem) (void)hipFree(j->mem); delete j; }
+  for (auto &kv : e->keys) (void)hipFree(kv.second.k.full);
+  void *ptrs[] = {e->d_frames, e->d_lists, e->d_planes, e->d_links, e->d_slots, e->d_counters};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  delete e;
+}
+
+}  // namespace dvc
+
+extern "C" {
+
+int dvc_engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **out) { return engine_create(c, p, out); }
+int dvc_engine_destroy(dvc_engine *e) { engine_destroy(e); return DVC_OK; }
+int dvc_engine_submit(dvc_engine *e, const dvc_wz_frame *f, uint64_t past_key, uint64_t fut_key, uint64_t *ticket) {
+  DVC_CHECK(e, DVC_EINVAL, "dvc_engine_submit: NULL engine");
+  return engine_submit(e, f, past_key, fut_key, ticket);
+}
+int dvc_engine_step(dvc_engine *e, int *completed) {
+  DVC_CHECK(e && completed, DVC_EINVAL, "dvc_engine_step: NULL argument");
+  DVC_HIP(hipSetDevice(e->c->device));
+  return engine_step(e, completed);
+}
+int dvc_engine_status(dvc_engine *e, int *queued, int *inflight, uint64_t *hbm_used) {
+  DVC_CHECK(e, DVC_EINVAL, "dvc_engine_status: NULL engine");
+  if (queued) *queued = (int)e->pending.size();
+  if (inflight) *inflight = (int)e->inflight.size();
+  if (hbm_used) *hbm_used = e->used;
+  return DVC_OK;
+}
+int dvc_engine_done(dvc_engine *e, uint64_t ticket, int *done) {
+  DVC_CHECK(e && done, DVC_EINVAL, "dvc_engine_done: NULL argument");
+  auto it = e->doneTickets.find(ticket);
+  *done = it != e->doneTickets.end();
+  if (*done) e->doneTickets.erase(it);
+  return DVC_OK;
+}
+int dvc_engine_release_key(dvc_engine *e, uint64_t key) {
+  DVC_CHECK(e, DVC_EINVAL, "dvc_engine_release_key: NULL engine");
+  auto it = e->keys.find(key);
+  if (it != e->keys.end()) it->second.named = false;
+  trim_keys(e, 0);
+  return DVC_OK;
+}
+
+int dvc_decode_wz_frames(dvc_ctx *c, int n, const dvc_wz_frame *frames, int batched) {
+  DVC_CHECK(c, DVC_EINVAL, "dvc_decode_wz_frames: NULL context");
+  DVC_CHECK(n >= 0 && (n == 0 || frames), DVC_EINVAL, "dvc_decode_wz_frames: bad frame list");
+  if (n == 0) return DVC_OK;
+  dvc_engine_params p{};
+  p.max_inflight = batched ? n : 1;
+  p.sequential_bp = batched ? 0 : 1;
+  dvc_engine *e = nullptr;
+  int rc = engine_create(c, &p, &e);
+  if (rc) return rc;
+  for (int i = 0; i < n && !rc; i++) rc = engine_submit(e, &frames[i], 0, 0, nullptr);
+  int left = n;
+  while (!rc && left > 0) {
+    int done = 0;
+    rc = engine_step(e, &done);
+    left -= done;
+    if (!rc && done == 0 && e->inflight.empty() && !e->pending.empty()) {
+      set_error("dvc_decode_wz_frames: frame does not fit the HBM budget");
+      rc = DVC_ENOMEM;
+    }
+  }
+  engine_destroy(e);
+  return rc;
+}
+
+}  // extern "C"

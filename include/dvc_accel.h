@@ -149,6 +149,34 @@ typedef struct {
  * and equal the CPU path's (ParallelMode 0). */
 int dvc_decode_wz_frames(dvc_ctx *ctx, int n, const dvc_wz_frame *frames, int batched);
 
+/* ---------------------------------------------------------------- serving engine
+ * Continuous batching (no counterpart in the reference, which decodes frames
+ * strictly one after another, main.cpp:132-142): requests are queued by
+ * dvc_engine_submit and admitted between steps while their device footprint
+ * fits the HBM budget; every dvc_engine_step advances each in-flight frame by
+ * one bitplane (setting up newcomers first) and retires finished frames, whose
+ * recons/stats are then written.  Key frames are kept resident under a
+ * caller-chosen key (e.g. stream id << 32 | frame number; 0 = the host
+ * pointer) and shared by the frames that name it, until released or evicted
+ * (least recently used, only when unreferenced).  The request's host buffers
+ * (past, fut, orig, recons, stats) must stay valid until its ticket is done;
+ * ac_range/crc/acc are copied at submit.  Each frame's result is the same as
+ * when decoded alone. */
+typedef struct dvc_engine dvc_engine;
+typedef struct {
+  uint64_t hbm_budget_bytes;  /* 0: 80% of the free device memory */
+  int max_inflight;           /* frames decoded concurrently (0: 64) */
+  int sequential_bp;          /* 1: plain per-code rate search instead of the speculative launch */
+} dvc_engine_params;
+int dvc_engine_create(dvc_ctx *ctx, const dvc_engine_params *params, dvc_engine **out);
+int dvc_engine_destroy(dvc_engine *eng);
+int dvc_engine_submit(dvc_engine *eng, const dvc_wz_frame *frame, uint64_t past_key, uint64_t fut_key,
+                      uint64_t *ticket);
+int dvc_engine_step(dvc_engine *eng, int *completed);
+int dvc_engine_status(dvc_engine *eng, int *queued, int *inflight, uint64_t *hbm_used);
+int dvc_engine_done(dvc_engine *eng, uint64_t ticket, int *done);
+int dvc_engine_release_key(dvc_engine *eng, uint64_t key);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,3 +66,33 @@ def test_frames_against_oracle_fresh_seed(ctx):
         assert np.array_equal(rec, orec)
         assert st["levels"] == oinfo["levels"] and st["bitrate"] == oinfo["bitrate"]
         assert st["iterations"] == oinfo["iters"]
+
+
+def test_engine_staggered_arrivals_equal_solo(ctx):
+    """Continuous batching: frames submitted while others are mid-decode
+    (different sizes, quantisers and arrival steps) give the same results as
+    when decoded alone."""
+    from dvc_amd import _lib
+    cases = json.loads((GOLD / "frames.json").read_text())["cases"]
+    inputs = [_case_inputs(c) for c in cases]
+    eng = _lib.Engine(ctx, max_inflight=3)
+    tickets = {}
+    step = 0
+    pending = list(range(len(inputs)))
+    arrive = {0: 0, 1: 3, 2: 7, 3: 20}
+    results = {}
+    while pending or tickets:
+        for i in list(pending):
+            if arrive[i] <= step:
+                tickets[eng.submit(inputs[i], past_key=(i + 1) << 32, fut_key=((i + 1) << 32) | 2)] = i
+                pending.remove(i)
+        eng.step()
+        step += 1
+        for t in list(tickets):
+            if eng.done(t):
+                results[tickets.pop(t)] = eng.result(t)
+        assert step < 2000
+    eng.close()
+    for i, c in enumerate(cases):
+        rec, st = results[i]
+        _check_against_golden(rec, st, c)
