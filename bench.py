@@ -50,7 +50,7 @@ def bench_ldpca(args):
     lrs = [p["LR"] for p in planes]
     accs = [p["acc"] for p in planes]
     crcs = [p["crc"] for p in planes]
-    ctx = _lib.Context(device=0, max_iter=args.max_iter)
+    ctx = _lib.Context(device=0, max_iter=args.max_iter, bp_threads=args.bp_threads)
     for i in range(args.warmup):
         decs, sts = ctx.ldpca_decode_batch(lrs, accs, crcs)
         log(f"[bench] warmup {i} done")
@@ -131,7 +131,7 @@ def bench_wz(args):
     t0 = time.time()
     jobs = make_wz_inputs(args.streams)
     log(f"[bench] encoded {len(jobs)} CIF Q8 WZ frames in {time.time() - t0:.1f}s")
-    ctx = _lib.Context(device=0, max_iter=args.max_iter)
+    ctx = _lib.Context(device=0, max_iter=args.max_iter, bp_threads=args.bp_threads)
     for i in range(args.warmup):
         ctx.decode_wz_frames(jobs[:16], batched=True)
         log(f"[bench] warmup {i} done")
@@ -162,10 +162,11 @@ def bench_wz(args):
         t = time.perf_counter()
         ctx.decode_wz_frames([j], batched=False)
         lat.append(time.perf_counter() - t)
-    log(f"[bench] plain: {plain_n} frames, mean {np.mean(lat) * 1e3:.1f} ms/frame")
+    if plain_n:
+        log(f"[bench] plain: {plain_n} frames, mean {np.mean(lat) * 1e3:.1f} ms/frame")
     t_step = float(np.mean(times))
     value = len(jobs) / t_step
-    plain_fps = plain_n / float(np.sum(lat))
+    plain_fps = plain_n / float(np.sum(lat)) if plain_n else None
     eu = sum(sum(st["iterations"]) * 3 * 6336 for _, st in outs)
     dom = max(kern.items(), key=lambda kv: kv[1]["total_ms"]) if kern else ("", {"total_ms": 0, "launches": 1})
     res = {
@@ -176,9 +177,10 @@ def bench_wz(args):
         "config": {"workload": f"wz_cif_q8: CIF 352x288, 33 frames, GOP 2, Qindex 8, {args.streams} sequences "
                                f"({len(jobs)} WZ frames, 63 bitplanes each) per step, batched, max_iter={args.max_iter}",
                    "parallelism": "single GPU"},
-        "plain_fps": plain_fps, "batched_over_plain": value / plain_fps,
+        "plain_fps": plain_fps, "batched_over_plain": value / plain_fps if plain_fps else None,
         "bp_edge_updates_per_s": eu / t_step,
-        "latency_ms": {"plain_p50": float(np.percentile(lat, 50) * 1e3), "plain_p99": float(np.percentile(lat, 99) * 1e3),
+        "latency_ms": {"plain_p50": float(np.percentile(lat, 50) * 1e3) if lat else None,
+                       "plain_p99": float(np.percentile(lat, 99) * 1e3) if lat else None,
                        "batch_p50": float(np.percentile(times, 50) * 1e3), "batch_p99": float(np.percentile(times, 99) * 1e3)},
         "psnr_mean": float(np.mean(psnrs)),
         "kernels_ms_per_step": kern, "dominant_kernel": dom[0],
@@ -221,18 +223,31 @@ def roofline_wz(dom, kern, args, nframes, outs):
 
 
 def cpu_baseline_wz(jobs, args):
-    """Oracle (CPU restatement of the reference decoder, ParallelMode 1 with
-    OpenMP on every host core) on one frame of the same workload."""
+    """Oracle (CPU restatement of the reference decoder) on a bounded sample of
+    the same workload: ParallelMode 1 (OpenMP over the host cores the run is
+    given) on consecutive frames until about `cpu_seconds` have passed, and
+    ParallelMode 0 on one core for one frame."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_lib as O
-    cores = os.cpu_count() or 1
-    j = jobs[0]
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     t = time.perf_counter()
-    O.decode_wz_frame(j["past"], j["fut"], j["orig"], j["wz"], 8, mode=1, nthreads=cores, max_iter=args.max_iter,
-                      idx=j["idx"])
+    n = 0
+    for j in jobs:
+        O.decode_wz_frame(j["past"], j["fut"], j["orig"], j["wz"], 8, mode=1, nthreads=cores,
+                          max_iter=args.max_iter, idx=j["idx"])
+        n += 1
+        if time.perf_counter() - t > args.cpu_seconds:
+            break
     dt = time.perf_counter() - t
-    return {"value": 1.0 / dt, "unit": "WZ frames/s", "cores": cores, "kind": "port",
-            "sample": f"1 CIF Q8 WZ frame of the same workload, ParallelMode 1 (OpenMP), {dt:.1f}s"}
+    j = jobs[0]
+    t0 = time.perf_counter()
+    O.decode_wz_frame(j["past"], j["fut"], j["orig"], j["wz"], 8, mode=0, nthreads=1, max_iter=args.max_iter,
+                      idx=j["idx"])
+    dt0 = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "WZ frames/s", "cores": cores, "kind": "port",
+            "sample": f"{n} CIF Q8 WZ frames of the same workload, ParallelMode 1 (OpenMP, {cores} threads), "
+                      f"{dt:.1f}s; ParallelMode 0 on 1 core: 1 frame in {dt0:.1f}s",
+            "mode0_1core_fps": 1.0 / dt0}
 
 
 def cpu_baseline_ldpca(planes, args):
@@ -253,24 +268,50 @@ def cpu_baseline_ldpca(planes, args):
                       f"sequential rate search, {dt:.1f}s"}
 
 
+def bench_copy(args):
+    """HBM calibration: a device-to-device copy of a known size (1 GiB read +
+    1 GiB written per step).  Its FETCH_SIZE / WRITE_SIZE readings scale the
+    same counters of the decoding kernels to bytes."""
+    import torch
+    nbytes = 1 << 30
+    src = torch.randint(0, 255, (nbytes,), dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(args.warmup):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / args.steps
+    assert torch.equal(dst[:4096], src[:4096])
+    gbs = 2 * nbytes / dt / 1e9
+    return {"metric": "HBM copy bandwidth (calibration)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+            "config": {"workload": "hbm_copy: 1 GiB device-to-device copy per step"},
+            "bytes_read_per_step": nbytes, "bytes_written_per_step": nbytes}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch"])
+    ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch", "hbm_copy"])
     ap.add_argument("--streams", type=int, default=4)
     ap.add_argument("--plain-frames", type=int, default=4)
-    ap.add_argument("--max-iter", type=int, default=50)
+    ap.add_argument("--max-iter", type=int, default=100)
+    ap.add_argument("--bp-threads", type=int, default=0, help="threads per CIF BP workgroup (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-planes", type=int, default=150)
-    ap.add_argument("--cpu-seconds", type=float, default=25.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(streams shard across GPUs without any exchange; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
-    res = bench_wz(args) if args.workload == "wz_cif_q8" else bench_ldpca(args)
+    res = {"wz_cif_q8": bench_wz, "ldpca_batch": bench_ldpca, "hbm_copy": bench_copy}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

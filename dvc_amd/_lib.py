@@ -183,7 +183,7 @@ class Context:
     def selftest(self, func, x):
         """Device evaluation of a scalar routine (see dvc_selftest_unary)."""
         x = np.ascontiguousarray(x)
-        out = np.zeros(x.size, np.float64 if func >= 4 else np.float32)
+        out = np.zeros(x.size, np.float64 if func in (4, 5) else np.float32)
         _check(lib.dvc_selftest_unary(self.h, func, _ptr(x), _ptr(out), x.size), "dvc_selftest_unary")
         return out
 

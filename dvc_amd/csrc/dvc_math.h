@@ -232,3 +232,98 @@ DVC_MATH_FN float dvc_bp_phi(float x) {
   DVCM_NOCONTRACT
   return -dvc_logf(dvc_tanhf((x + (float)0.000000001) / 2.0f));
 }
+
+// The general path of dvc_logf for positive normal finite x, reading the
+// (invc, logc) table through `tab` (tab[i] = invc[i], tab[16 + i] = logc[i])
+// so that a kernel can keep it in LDS.  x = 1.0f gives +0 here as well.
+DVC_MATH_FN float dvc_logf_normal(float x, const double *tab) {
+  const double A0 = -0x1.00ea348b88334p-2, A1 = 0x1.5575b0be00b6ap-2, A2 = -0x1.ffffef20a4123p-2;
+  const double Ln2 = 0x1.62e42fefa39efp-1;
+  const uint32_t ix = dvcm_asu(x);
+  const uint32_t tmp = ix - 0x3f330000u;
+  const int i = (int)((tmp >> 19) % 16u);
+  const int k = (int32_t)tmp >> 23;
+  const uint32_t iz = ix - (tmp & 0xff800000u);
+  const double invc = tab[i], logc = tab[16 + i];
+  const double z = (double)dvcm_asf(iz);
+  const double r = fma(z, invc, -1.0);
+  const double y0 = fma((double)k, Ln2, logc);
+  const double r2 = r * r;
+  double y = fma(A1, r, A2);
+  y = fma(A0, r2, y);
+  y = fma(y, r2, y0 + r);
+  return (float)y;
+}
+
+// dvc_bp_phi for x >= 0 (the magnitudes BP applies it to), bit-identical to
+// it, without data-dependent branches, so the lanes of a wave never diverge.
+// With y = (x + 1e-9f)/2 >= 5e-10:
+//   * tanhf: y >= 22 gives 1; otherwise its two expm1f calls (argument 2y for
+//     y >= 1, -2y below) become one call on the selected argument, and its
+//     two quotients one division with a selected numerator;
+//   * expm1f on a in (-2, -1e-9] or [2, 44): every reduction case and every
+//     result case is computed and the reached one selected (|a| < 2^-25
+//     returns a; k = 0, -1, 1, <= -2 or > 56, 2..22, 23..56);
+//   * logf of z in (0, 1] is its general path (dvc_logf_normal).
+// Arguments outside these ranges only reach lanes whose value the final
+// selects discard.
+DVC_MATH_FN float dvc_bp_phi_nonneg(float x, const double *logtab) {
+  DVCM_NOCONTRACT
+  const float ln2_hi = 6.9313812256e-01f, ln2_lo = 9.0580006145e-06f, invln2 = 1.4426950216e+00f,
+              Q1 = -3.3333335072e-02f, Q2 = 1.5873016091e-03f, Q3 = -7.9365076090e-05f,
+              Q4 = 4.0082177293e-06f, Q5 = -2.0109921195e-07f;
+  const float y = (x + (float)0.000000001) / 2.0f;
+  const uint32_t iy = dvcm_asu(y) & 0x7fffffffu;
+  const int big = iy >= 0x3f800000u;
+  const float a = big ? 2.0f * y : -2.0f * y;
+  // ---- expm1f(a)
+  const uint32_t ha = dvcm_asu(a);
+  const int neg = (ha >> 31) != 0;
+  const uint32_t hx = ha & 0x7fffffffu;
+  const int red = hx > 0x3eb17218u;
+  const int near1 = hx < 0x3F851592u;
+  const float hi1 = neg ? a + ln2_hi : a - ln2_hi;
+  const float lo1 = neg ? -ln2_lo : ln2_lo;
+  float kf = invln2 * a + (neg ? -0.5f : 0.5f);
+  kf = kf > 256.0f ? 256.0f : (kf < -256.0f ? -256.0f : kf);
+  const int kg = (int)kf;
+  const float tg = (float)kg;
+  const float hig = a - tg * ln2_hi;
+  const float log_ = tg * ln2_lo;
+  const float hi = near1 ? hi1 : hig, lo = near1 ? lo1 : log_;
+  const int kr = near1 ? (neg ? -1 : 1) : kg;
+  const float xr = hi - lo;
+  const float c = (hi - xr) - lo;
+  const float xx = red ? xr : a;
+  const int k = red ? kr : 0;
+  const float hfx = 0.5f * xx;
+  const float hxs = xx * hfx;
+  const float r1 = 1.0f + hxs * (Q1 + hxs * (Q2 + hxs * (Q3 + hxs * (Q4 + hxs * Q5))));
+  const float t3 = 3.0f - r1 * hfx;
+  const float e = hxs * ((r1 - t3) / (6.0f - xx * t3));
+  const float r_k0 = xx - (xx * e - hxs);
+  float e2 = (xx * (e - c) - c);
+  e2 -= hxs;
+  const float r_km1 = 0.5f * (xx - e2) - 0.5f;
+  const float r_k1 = (xx < -0.25f) ? -2.0f * (e2 - (xx + 0.5f)) : 1.0f + 2.0f * (xx - e2);
+  const uint32_t ksh = (uint32_t)k << 23;
+  const float r_big = dvcm_asf(dvcm_asu(1.0f - (e2 - xx)) + ksh) - 1.0f;
+  const int kc = k < 0 ? 0 : (k > 31 ? 31 : k);
+  const float t23 = dvcm_asf(0x3f800000u - (0x1000000u >> kc));
+  const float r_lt23 = dvcm_asf(dvcm_asu(t23 - (e2 - xx)) + ksh);
+  const float tge = dvcm_asf((uint32_t)(0x7f - k) << 23);
+  float yge = xx - (e2 + tge);
+  yge += 1.0f;
+  const float r_ge23 = dvcm_asf(dvcm_asu(yge) + ksh);
+  float em1 = (k == 0) ? r_k0
+            : (k == -1) ? r_km1
+            : (k == 1) ? r_k1
+            : (k <= -2 || k > 56) ? r_big
+            : (k < 23) ? r_lt23 : r_ge23;
+  em1 = (!red && hx < 0x33000000u) ? a : em1;
+  // ---- tanhf(y)
+  const float q = (big ? 2.0f : -em1) / (em1 + 2.0f);
+  float z = big ? 1.0f - q : q;
+  z = iy >= 0x41b00000u ? 1.0f : z;
+  return -dvc_logf_normal(z, logtab);
+}

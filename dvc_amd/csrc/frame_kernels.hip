@@ -489,44 +489,73 @@ __global__ void k_smf_init(const FrameDev *F, const int *list) {
 // noiseDistribution (noiseModel.cpp:77-132): DC prob[k] = expf(-a*|k - SI_DC|);
 // AC prob[k] = sum_m p_m * expf(-a*|k - range - SI_m|), m in window raster
 // order.  A window term whose probability is exactly +0 adds +0 to a
-// non-negative sum and is skipped when a is finite.  Grid: (k chunks, block, frame).
+// non-negative sum and is skipped when a is finite.  The exponent depends on
+// the integer distance d = |k - range - SI_m| only, so each workgroup first
+// tabulates expf(-a*(float)d) for the distances its block can reach (the same
+// values, bit for bit) and the mixture sums then read the table.  One
+// workgroup per (4x4 block, frame).
+#define ND_TAB 6144  // table entries (24 KB of LDS); larger distances call expf directly
 __global__ __launch_bounds__(256) void k_noise_dist(const FrameDev *F, const int *list) {
-  const FrameDev &f = F[list[blockIdx.z]];
-  const int b = blockIdx.y;
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int b = blockIdx.x;
   if (b >= f.N) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = threadIdx.x;
   const int ow = f.W - 3, index = f.index, range = f.range;
   const SmfWin s = smf_window(f, b);
   const float a = f.alpha[(s.top + (index >> 2)) * f.W + s.left + (index & 3)];
+  float *out = f.dist + (size_t)b * f.distStride;
   if (index == 0) {
-    if (k >= range) return;
     const int side = f.oct[(size_t)16 * (s.top * ow + s.left)];
-    f.dist[(size_t)b * f.distStride + k] = dvc_expf(-a * (float)abs(k - side));
+    for (int k = tid; k < range; k += 256) out[k] = dvc_expf(-a * (float)abs(k - side));
     return;
   }
   __shared__ float s_p[81];
   __shared__ int s_side[81];
-  __shared__ int s_n;
-  if (threadIdx.x == 0) {
+  __shared__ int s_n, s_maxabs;
+  __shared__ float s_tab[ND_TAB];
+  if (tid < 64) {
+    // wave 0 compacts the window's non-zero terms in raster order
     const bool fin = isfinite(a);
-    int n = 0;
-    for (int p = 0; p < s.len; p++) {
-      const float pr = f.smf[(size_t)81 * b + p];
-      if (fin && pr == 0.0f) continue;
-      const int m = s.sT + p / s.w, q = s.sL + p % s.w;
-      s_p[n] = pr;
-      s_side[n] = f.oct[(size_t)16 * (m * ow + q) + index];
-      n++;
+    int n = 0, mx = 0;
+    for (int base = 0; base < s.len; base += 64) {
+      const int p = base + tid;
+      float pr = 0.0f;
+      bool keep = false;
+      if (p < s.len) {
+        pr = f.smf[(size_t)81 * b + p];
+        keep = !(fin && pr == 0.0f);
+      }
+      const unsigned long long mask = __ballot(keep);
+      if (keep) {
+        const int pos = n + __popcll(mask & ((1ull << tid) - 1ull));
+        const int m = s.sT + p / s.w, q = s.sL + p % s.w;
+        const int sd = f.oct[(size_t)16 * (m * ow + q) + index];
+        s_p[pos] = pr;
+        s_side[pos] = sd;
+        mx = imax(mx, abs(sd));
+      }
+      n += __popcll(mask);
     }
-    s_n = n;
+    for (int o = 32; o > 0; o >>= 1) mx = imax(mx, __shfl_xor(mx, o));
+    if (tid == 0) { s_n = n; s_maxabs = mx; }
   }
   __syncthreads();
+  const int n = s_n;
+  // |k - range - SI_m| <= range + max|SI_m| for k in [0, 2*range]
+  const int tl = imin(range + s_maxabs + 1, ND_TAB);
+  for (int d = tid; d < tl; d += 256) s_tab[d] = dvc_expf(-a * (float)d);
+  __syncthreads();
   const int rmax = 2 * range + 1;
-  if (k >= rmax) return;
-  const int pc = k - range, n = s_n;
-  float sum = 0.0f;
-  for (int m = 0; m < n; m++) sum += s_p[m] * dvc_expf(-a * (float)abs(pc - s_side[m]));
-  f.dist[(size_t)b * f.distStride + k] = sum;
+  for (int k = tid; k < rmax; k += 256) {
+    const int pc = k - range;
+    float sum = 0.0f;
+    for (int m = 0; m < n; m++) {
+      const int d = abs(pc - s_side[m]);
+      const float e = d < tl ? s_tab[d] : dvc_expf(-a * (float)d);
+      sum += s_p[m] * e;
+    }
+    out[k] = sum;
+  }
 }
 
 // ------------------------------------------------------------ conditional bit probability
@@ -660,43 +689,102 @@ __global__ void k_inverse_transform(const FrameDev *F, const int *list) {
 // the partially reconstructed frame, separate full-pel argmins on the past
 // and future key frames (cost SAD*(1+K*|2v|) in double), 9-position half-pel
 // refinement on each interpolated frame, then SI = (int)(P*iPC + F*iFC).
-// One wave per block, 4 blocks per workgroup.
+// One wave per block, 4 horizontally adjacent blocks per workgroup (the frame
+// width is a multiple of 16, so the 4 share a block row).  The union of their
+// search windows (up to 68 rows x 80 bytes of each key frame) is staged in LDS
+// as 32-bit words; lane l sweeps candidate column sL+l down the window,
+// forming each candidate's 4-pixel rows with v_alignbyte and its SAD with
+// v_sad_u8 (exact integer SADs, as the reference's double sums of integer
+// differences are).  The 65th column of a full window is spread over the lanes
+// afterwards.  Ties resolve to the first candidate in raster order.
+#define USI_LW 21   // staged words per window row: 20 + one zero word
+#define USI_ROWS 68
+__device__ __forceinline__ uint32_t usi_word(const uint32_t *R, int r, int j, int sh) {
+  return __builtin_amdgcn_alignbyte(R[r * USI_LW + j + 1], R[r * USI_LW + j], sh);
+}
+__device__ __forceinline__ uint32_t sad4(const uint32_t *bk, uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3) {
+  uint32_t s = __builtin_amdgcn_sad_u8(bk[0], r0, 0u);
+  s = __builtin_amdgcn_sad_u8(bk[1], r1, s);
+  s = __builtin_amdgcn_sad_u8(bk[2], r2, s);
+  return __builtin_amdgcn_sad_u8(bk[3], r3, s);
+}
+__device__ __forceinline__ void argmin_take(double c, int i, double &best, int &bi) {
+  if (c < best || (c == best && i < bi)) { best = c; bi = i; }
+}
 __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int *list, CostTables ct) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int b = blockIdx.x * 4 + wave;
-  if (b >= f.N) return;
+  const int b0 = blockIdx.x * 4;
+  if (b0 >= f.N) return;  // uniform over the workgroup
   const int W = f.W, H = f.H, W2 = 2 * W, H2 = 2 * H, bw = W / 4;
-  const int top = (b / bw) * 4, left = (b % bw) * 4;
+  __shared__ uint32_t s_win[2][USI_ROWS * USI_LW];
+  __shared__ double s_fac[33 * 33];
+  // union window of the 4 blocks: rows [wT, wB], byte columns [cL, cR]
+  const int row0 = (b0 / bw) * 4, left0 = (b0 % bw) * 4;
+  const int wT = imax(0, row0 - 32), wB = imin(H - 4, row0 + 32) + 3;
+  const int cL = imax(0, left0 - 32) & ~3, cR = imin(W - 4, left0 + 12 + 32) + 3;
+  const int nrows = wB - wT + 1, nw = (cR + 1 - cL) >> 2;
+  for (int t = threadIdx.x; t < 2 * nrows * USI_LW; t += 256) {
+    const int ref = t / (nrows * USI_LW), rem = t - ref * nrows * USI_LW;
+    const int r = rem / USI_LW, w = rem - r * USI_LW;
+    const uint8_t *src = ref ? f.fut : f.past;
+    s_win[ref][r * USI_LW + w] = w < nw ? *(const uint32_t *)(src + (size_t)(wT + r) * W + cL + 4 * w) : 0u;
+  }
+  for (int t = threadIdx.x; t < 33 * 33; t += 256) {
+    const int dy = t / 33, dx = t - dy * 33;
+    s_fac[t] = ct.usi[(dy + 32) * 65 + dx + 32];
+  }
+  __syncthreads();
+  const int b = b0 + wave;
+  if (b >= f.N) return;
+  const int top = row0, left = (b % bw) * 4;
   const int sT = imax(0, top - 32), sB = imin(H - 4, top + 32), sL = imax(0, left - 32), sR = imin(W - 4, left + 32);
-  const int ww = sR - sL + 1, nc = ww * (sB - sT + 1);
+  const int ww = sR - sL + 1, nr = sB - sT + 1;
+  uint32_t bk[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) bk[m] = *(const uint32_t *)(f.recons + (size_t)(top + m) * W + left);
+  const uint32_t *P = s_win[0], *Fu = s_win[1];
+  double pBest = DBL_MAX, fBest = DBL_MAX;
+  int pIdx = 0x7fffffff, fIdx = 0x7fffffff;
+  if (lane < ww) {
+    const int x = sL + lane, xo = x - cL, j = xo >> 2, sh = xo & 3;
+    const double *facRow = s_fac + abs(x - left);
+    int r = sT - wT;
+    uint32_t p0 = usi_word(P, r, j, sh), p1 = usi_word(P, r + 1, j, sh), p2 = usi_word(P, r + 2, j, sh);
+    uint32_t f0 = usi_word(Fu, r, j, sh), f1 = usi_word(Fu, r + 1, j, sh), f2 = usi_word(Fu, r + 2, j, sh);
+    int idx = lane;
+    for (int y = sT; y <= sB; y++, r++, idx += ww) {
+      const uint32_t p3 = usi_word(P, r + 3, j, sh), f3 = usi_word(Fu, r + 3, j, sh);
+      const uint32_t ps = sad4(bk, p0, p1, p2, p3), fs = sad4(bk, f0, f1, f2, f3);
+      const double fac = facRow[33 * abs(y - top)];
+      const double pc = (double)ps * fac, fc = (double)fs * fac;
+      if (pc < pBest) { pBest = pc; pIdx = idx; }
+      if (fc < fBest) { fBest = fc; fIdx = idx; }
+      p0 = p1; p1 = p2; p2 = p3;
+      f0 = f1; f1 = f2; f2 = f3;
+    }
+  }
+  if (ww > 64) {
+    // column sL+64: one candidate row per lane
+    const int x = sL + 64, xo = x - cL, j = xo >> 2, sh = xo & 3;
+    for (int i = lane; i < nr; i += 64) {
+      const int y = sT + i, r = y - wT;
+      const uint32_t ps = sad4(bk, usi_word(P, r, j, sh), usi_word(P, r + 1, j, sh), usi_word(P, r + 2, j, sh),
+                               usi_word(P, r + 3, j, sh));
+      const uint32_t fs = sad4(bk, usi_word(Fu, r, j, sh), usi_word(Fu, r + 1, j, sh), usi_word(Fu, r + 2, j, sh),
+                               usi_word(Fu, r + 3, j, sh));
+      const double fac = s_fac[33 * abs(y - top) + abs(x - left)];
+      argmin_take((double)ps * fac, i * ww + 64, pBest, pIdx);
+      argmin_take((double)fs * fac, i * ww + 64, fBest, fIdx);
+    }
+  }
+  wave_argmin(pBest, pIdx);
+  wave_argmin(fBest, fIdx);
   int blk[16];
 #pragma unroll
   for (int m = 0; m < 4; m++)
 #pragma unroll
-    for (int n = 0; n < 4; n++) blk[4 * m + n] = f.recons[(top + m) * W + left + n];
-  double pBest = DBL_MAX, fBest = DBL_MAX;
-  int pIdx = 0x7fffffff, fIdx = 0x7fffffff;
-  for (int c = lane; c < nc; c += 64) {
-    const int y = sT + c / ww, x = sL + c % ww;
-    int ps = 0, fs = 0;
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-      const uint8_t *pr = f.past + (y + m) * W + x;
-      const uint8_t *fr = f.fut + (y + m) * W + x;
-#pragma unroll
-      for (int n = 0; n < 4; n++) {
-        ps += abs(blk[4 * m + n] - (int)pr[n]);
-        fs += abs(blk[4 * m + n] - (int)fr[n]);
-      }
-    }
-    const double fac = ct.usi[(y - top + 32) * 65 + (x - left + 32)];
-    const double pc = (double)ps * fac, fc = (double)fs * fac;
-    if (pc < pBest) { pBest = pc; pIdx = c; }
-    if (fc < fBest) { fBest = fc; fIdx = c; }
-  }
-  wave_argmin(pBest, pIdx);
-  wave_argmin(fBest, fIdx);
+    for (int n = 0; n < 4; n++) blk[4 * m + n] = (bk[m] >> (8 * n)) & 0xff;
   int pbx = sL + pIdx % ww, pby = sT + pIdx / ww, fbx = sL + fIdx % ww, fby = sT + fIdx / ww;
   // subPelRefine: lanes 0-8 evaluate the 9 positions of the past frame, 16-24 the future
   const int which = lane >> 4, pos = lane & 15;
@@ -733,70 +821,72 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
 
 // ------------------------------------------------------------ motion learning
 // updateSMF (motionLearning.cpp:70-127): p = expf(-MU*SSE) over the +-4
-// window, normalised by the float sum in raster order.
-__global__ void k_update_smf(const FrameDev *F, const int *list) {
+// window, normalised by the float sum in raster order.  One wave per 4x4
+// block: lane l evaluates window positions l and l+64; the raster-order sum
+// is then formed identically on every lane from shuffled values.
+__global__ __launch_bounds__(256) void k_update_smf(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 4 + wave;
   if (b >= f.N) return;
   const int W = f.W;
   const SmfWin s = smf_window(f, b);
   float blk[16];
+#pragma unroll
   for (int m = 0; m < 4; m++)
+#pragma unroll
     for (int n = 0; n < 4; n++) blk[4 * m + n] = (float)f.recons[(s.top + m) * W + s.left + n];
-  float *pr = f.smf + (size_t)81 * b;
-  float sum = 0.0f;
-  int p = 0;
-  for (int m = s.sT; m <= s.sB; m++)
-    for (int q = s.sL; q <= s.sR; q++) {
+  float v[2];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int p = lane + 64 * h;
+    v[h] = 0.0f;
+    if (p < s.len) {
+      const int m = s.sT + p / s.w, q = s.sL + p % s.w;
       float sse = 0.0f;
+#pragma unroll
       for (int ii = 0; ii < 4; ii++)
+#pragma unroll
         for (int jj = 0; jj < 4; jj++) {
           const float d = blk[4 * ii + jj] - (float)f.si[(m + ii) * W + q + jj];
           sse += d * d;
         }
-      pr[p] = dvc_expf(-0.01f * sse);
-      sum += pr[p];
-      p++;
+      v[h] = dvc_expf(-0.01f * sse);
     }
-  if ((double)sum > 0.0) {
-    for (int q = 0; q < p; q++) pr[q] = pr[q] / sum;
-  } else {
-    pr[s.self] = 1.0f;
+  }
+  float sum = 0.0f;
+  for (int p = 0; p < s.len; p++) sum += __shfl(p < 64 ? v[0] : v[1], p & 63);
+  float *pr = f.smf + (size_t)81 * b;
+  const bool norm = (double)sum > 0.0;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int p = lane + 64 * h;
+    if (p < s.len) pr[p] = norm ? v[h] / sum : (p == s.self ? 1.0f : v[h]);
   }
 }
 
 // update_si_ML (motionLearning.cpp:129-160): SMF-weighted over-complete
 // coefficients for the bands after the frame's current band, accumulated in
-// window raster order and truncated.
-__global__ void k_update_si_ml(const FrameDev *F, const int *list, int) {
+// window raster order and truncated.  16 lanes per 4x4 block, one per
+// coefficient index, so a window position's 16 coefficients are one
+// contiguous 64-byte load; lanes of already decoded bands idle.
+__constant__ int c_zigpos[16] = {0, 1, 5, 6, 2, 4, 7, 12, 3, 8, 11, 13, 9, 10, 14, 15};  // inverse of c_zigzag
+__global__ __launch_bounds__(256) void k_update_si_ml(const FrameDev *F, const int *list, int) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int bandStart = f.band + 1;  // update_si_ML(..., band+1, ...) (DVC_Decoder.cpp:369)
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= f.N) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = t >> 4, index = t & 15;
+  if (b >= f.N || c_zigpos[index] < bandStart) return;
   const int W = f.W, ow = W - 3;
   const SmfWin s = smf_window(f, b);
-  float acc[16];
-#pragma unroll
-  for (int i = 0; i < 16; i++) acc[i] = 0.0f;
   const float *pr = f.smf + (size_t)81 * b;
+  float acc = 0.0f;
   int p = 0;
-  for (int m = s.sT; m <= s.sB; m++)
-    for (int q = s.sL; q <= s.sR; q++) {
-      const float prob = pr[p++];
-      const int *o = f.oct + (size_t)16 * (m * ow + q);
-#pragma unroll
-      for (int k = 0; k < 16; k++)
-        if (k >= bandStart) {
-          const int index = c_zigzag[k];
-          acc[index] += prob * (float)o[index];
-        }
-    }
-#pragma unroll
-  for (int k = 0; k < 16; k++)
-    if (k >= bandStart) {
-      const int index = c_zigzag[k];
-      f.sideTrans[(s.top + (index >> 2)) * W + s.left + (index & 3)] = (int)acc[index];
-    }
+  for (int m = s.sT; m <= s.sB; m++) {
+    const int *o = f.oct + (size_t)16 * (m * ow + s.sL) + index;
+    for (int q = 0; q < s.w; q++) acc += pr[p++] * (float)o[16 * q];
+  }
+  f.sideTrans[(s.top + (index >> 2)) * W + s.left + (index & 3)] = (int)acc;
 }
 
 // ------------------------------------------------------------ launchers
@@ -818,7 +908,7 @@ hipError_t launch_oct(const FrameDev *F, const int *list, int n, int maxPos, int
   return hipGetLastError();
 }
 hipError_t launch_noise_dist(const FrameDev *F, const int *list, int n, int maxN, int maxStride, hipStream_t s) {
-  hipLaunchKernelGGL(k_noise_dist, dim3((maxStride + 255) / 256, maxN, n), dim3(256), 0, s, F, list);
+  hipLaunchKernelGGL(k_noise_dist, dim3(maxN, n), dim3(256), 0, s, F, list);
   return hipGetLastError();
 }
 hipError_t launch_condbit(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s) {
@@ -847,11 +937,11 @@ hipError_t launch_smf_init(const FrameDev *F, const int *list, int n, int maxN, 
   return hipGetLastError();
 }
 hipError_t launch_update_smf(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s) {
-  hipLaunchKernelGGL(k_update_smf, dim3((maxN + 127) / 128, n), dim3(128), 0, s, F, list);
+  hipLaunchKernelGGL(k_update_smf, dim3((maxN + 3) / 4, n), dim3(256), 0, s, F, list);
   return hipGetLastError();
 }
 hipError_t launch_update_si_ml(const FrameDev *F, const int *list, int n, int maxN, int bandStart, hipStream_t s) {
-  hipLaunchKernelGGL(k_update_si_ml, dim3((maxN + 127) / 128, n), dim3(128), 0, s, F, list, bandStart);
+  hipLaunchKernelGGL(k_update_si_ml, dim3((16 * maxN + 255) / 256, n), dim3(256), 0, s, F, list, bandStart);
   return hipGetLastError();
 }
 

@@ -7,7 +7,9 @@
 // for step, so every message, decision and iteration count equals the CPU
 // path's:
 //   * phi(x) = -logf(tanhf((x + 1e-9f)/2)) with the glibc-identical routines
-//     of dvc_math.h;
+//     of dvc_math.h, in their branch-free form for x >= 0 (every argument
+//     here is a magnitude: a row's sum of |phi| is at least each of its
+//     terms, so |sum| - |phi_e| >= 0);
 //   * check-node sums accumulate in ascending edge order (= ascending column),
 //     the order of `rowTotal[ir[k]] += ...` (:514-515);
 //   * variable-node sums accumulate LLR + c2v0 + c2v1 + c2v2 (:527-539);
@@ -105,9 +107,14 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
   __shared__ uint32_t synBits[S::SYN_WORDS];
   __shared__ int s_item, s_cancel;
   __shared__ uint32_t s_crc[T / 64];
+  __shared__ double s_logtab[32];  // logf's (invc, logc) table, read with per-lane indices
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const LadderDev L = a.lad;
+  if (tid < 16) {
+    s_logtab[tid] = dvcm_logf_invc[tid];
+    s_logtab[16 + tid] = dvcm_logf_logc[tid];
+  }
 
   for (;;) {
     // ---- grab the next item (lane 0 of the workgroup), broadcast through LDS
@@ -159,7 +166,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         float v = pd.LLR[k];
         uint32_t s = f2u(v) >> 31;
         float mag = (s ? -1.0f : 1.0f) * v;
-        uint32_t f = (f2u(dvc_bp_phi(mag)) & 0x7fffffffu) | (s << 31);
+        uint32_t f = (f2u(dvc_bp_phi_nonneg(mag, s_logtab)) & 0x7fffffffu) | (s << 31);
         phi[3 * k] = u2f(f); phi[3 * k + 1] = u2f(f); phi[3 * k + 2] = u2f(f);
       }
     }
@@ -206,7 +213,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
             const uint32_t R = f2u(rowTot[gir[e]]);
             ph[q] = f2u(phi[e]);
             const float x = u2f(R & 0x7fffffffu) - u2f(ph[q] & 0x7fffffffu);
-            const float v = dvc_bp_phi(x);
+            const float v = dvc_bp_phi_nonneg(x, s_logtab);
             c2v[q] = ((R ^ ph[q]) >> 31) ? v * -1.0f : v * 1.0f;
           }
           float tmp = pd.LLR[k];
@@ -219,7 +226,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
             const float v2c = tmp - c2v[q];
             const uint32_t s = f2u(v2c) >> 31;
             const float mag = (s ? -1.0f : 1.0f) * v2c;
-            phi[3 * k + q] = u2f((f2u(dvc_bp_phi(mag)) & 0x7fffffffu) | (s << 31));
+            phi[3 * k + q] = u2f((f2u(dvc_bp_phi_nonneg(mag, s_logtab)) & 0x7fffffffu) | (s << 31));
           }
         }
         const uint64_t bal = __ballot(dec);

@@ -8,6 +8,12 @@
 namespace dvc {
 
 __global__ void k_selftest(int func, const void *in, void *out, int n) {
+  __shared__ double s_logtab[32];  // as the BP kernel stages it
+  if (threadIdx.x < 16) {
+    s_logtab[threadIdx.x] = dvcm_logf_invc[threadIdx.x];
+    s_logtab[16 + threadIdx.x] = dvcm_logf_logc[threadIdx.x];
+  }
+  __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   switch (func) {
@@ -17,6 +23,7 @@ __global__ void k_selftest(int func, const void *in, void *out, int n) {
     case 3: ((float *)out)[i] = dvc_bp_phi(((const float *)in)[i]); break;
     case 4: ((double *)out)[i] = sqrt((double)((const int *)in)[i]); break;
     case 5: ((double *)out)[i] = exp(((const double *)in)[i]); break;
+    case 6: ((float *)out)[i] = dvc_bp_phi_nonneg(((const float *)in)[i], s_logtab); break;
     default: break;
   }
 }
@@ -24,9 +31,9 @@ __global__ void k_selftest(int func, const void *in, void *out, int n) {
 }  // namespace dvc
 
 extern "C" int dvc_selftest_unary(dvc_ctx *c, int func, const void *in, void *out, int n) {
-  DVC_CHECK(c && in && out && n > 0 && func >= 0 && func <= 5, DVC_EINVAL, "dvc_selftest_unary: bad argument");
+  DVC_CHECK(c && in && out && n > 0 && func >= 0 && func <= 6, DVC_EINVAL, "dvc_selftest_unary: bad argument");
   DVC_HIP(hipSetDevice(c->device));
-  const size_t isz = (func == 4 || func <= 3) ? 4 : 8, osz = func >= 4 ? 8 : 4;
+  const size_t isz = func == 5 ? 8 : 4, osz = (func == 4 || func == 5) ? 8 : 4;
   void *din = nullptr, *dout = nullptr;
   DVC_HIP(hipMalloc(&din, isz * (size_t)n));
   hipError_t e = hipMalloc(&dout, osz * (size_t)n);

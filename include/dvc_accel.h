@@ -112,7 +112,8 @@ int dvc_ldpca_decode_batch(dvc_ctx *ctx, int n, dvc_ldpca_item *items);
 /* Verification entry (no counterpart in the reference): evaluates one of the
  * device's scalar routines on n inputs.  func 0..3: float logf, expf, tanhf,
  * the BP kernel phi(x) = -logf(tanhf((x+1e-9f)/2)) (float in, float out);
- * 4: double sqrt of int32 inputs; 5: double exp (double in, double out). */
+ * 4: double sqrt of int32 inputs; 5: double exp (double in, double out);
+ * 6: the branch-free phi the BP kernel evaluates for x >= 0 (float in, float out). */
 int dvc_selftest_unary(dvc_ctx *ctx, int func, const void *in, void *out, int n);
 
 /* ---------------------------------------------------------------- WZ frames */

@@ -68,6 +68,29 @@ def test_frames_against_oracle_fresh_seed(ctx):
         assert st["iterations"] == oinfo["iters"]
 
 
+@pytest.mark.parametrize("size", [(176, 144, 3), (352, 288, 6)])
+def test_frames_against_oracle_max_iter_100(size):
+    """The measured configuration caps BP at 100 iterations per code (the
+    reference's compiled cap is 50): GPU plain and batched decodes equal the
+    oracle run with the same cap, at Qindex 8."""
+    from dvc_amd import _lib
+    w, h, shift = size
+    vid = W.synthetic_video(w, h, 3, shift, seed=1100)
+    wz = W.encode_wz_frame(vid[1], 8)
+    orec, oinfo = O.decode_wz_frame(vid[0], vid[2], vid[1], wz, 8, max_iter=100)
+    c100 = _lib.Context(max_iter=100)
+    try:
+        for batched in (False, True):
+            (rec, st), = c100.decode_wz_frames([dict(past=vid[0], fut=vid[2], orig=vid[1], wz=wz)], batched=batched)
+            print(size, batched, "levels", st["levels"][:8], flush=True)
+            assert np.array_equal(rec, orec)
+            assert st["levels"] == oinfo["levels"] and st["bitrate"] == oinfo["bitrate"]
+            assert st["iterations"] == oinfo["iters"]
+            assert st["psnr"] == oinfo["psnr"]
+    finally:
+        c100.close()
+
+
 def test_engine_staggered_arrivals_equal_solo(ctx):
     """Continuous batching: frames submitted while others are mid-decode
     (different sizes, quantisers and arrival steps) give the same results as

@@ -34,11 +34,14 @@ def test_float_routines_match_host(ctx, func, name, lo, hi):
     assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32))
 
 
-def test_bp_phi_full_grid_of_messages(ctx):
-    # every float in [0, 64) on a stride, plus the small magnitudes bit by bit
-    b = np.concatenate([np.arange(0, 0x42800000, 4099, dtype=np.uint32), np.arange(0, 1 << 20, dtype=np.uint32)])
+@pytest.mark.parametrize("func", [3, 6])
+def test_bp_phi_full_grid_of_messages(ctx, func):
+    # every float in [0, 64) on a stride, plus the small magnitudes bit by bit;
+    # func 3 is the branching form, 6 the branch-free one the BP kernel runs
+    b = np.concatenate([np.arange(0, 0x42800000, 4099, dtype=np.uint32), np.arange(0, 1 << 20, dtype=np.uint32),
+                        np.arange(0x42800000, 0x7f800001, 65537, dtype=np.uint32)])
     x = b.view(np.float32)
-    dev = ctx.selftest(3, x)
+    dev = ctx.selftest(func, x)
     logf = LIBM.logf; logf.restype = ctypes.c_float; logf.argtypes = [ctypes.c_float]
     tanhf = LIBM.tanhf; tanhf.restype = ctypes.c_float; tanhf.argtypes = [ctypes.c_float]
     eps = np.float32(0.000000001)
