@@ -36,6 +36,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "WZ frames/sec and BP edge-updates/s (batched vs plain, 1 GPU); p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
+LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
 
 
 def log(*a):
@@ -89,7 +90,7 @@ def bench_ldpca(args):
             hbm_bytes += 4 * n + (int(m_of[n][lvl]) + 7) // 8 + (n + 7) // 8 + 2 * 3 * n
         lds_bytes += 16 * st["iterations"] * 3 * n
     bp_avg_ms = bp_ms / max(bp_n, 1)
-    achieved = hbm_bytes / (bp_avg_ms * 1e-3) / 1e9 if bp_n else None
+    per_step = max(bp_n // max(args.steps, 1), 1)  # bp_6336 launches per step
     res = {
         "metric": METRIC,
         "value": value,
@@ -100,10 +101,7 @@ def bench_ldpca(args):
         "dtype": "f32", "data": "synthetic (seeded Bernoulli bitplanes, Laplacian soft inputs)",
         "config": {"workload": "ldpca_batch: 512 bitplanes (384 N=6336 + 128 N=1584), all codes speculative, "
                                f"max_iter={args.max_iter}", "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "bp_persistent_kernel<6336>", "kernel_ms_avg": bp_avg_ms,
-                     "lds_GBps": lds_bytes / (bp_avg_ms * 1e-3) / 1e9 if bp_n else None},
+        "roofline": bp_roofline(bp_avg_ms, hbm_bytes / per_step, lds_bytes / 16.0 / per_step, "ldpca_batch"),
         "kernels_ms": {"bp_6336": bp_ms / max(bp_n, 1), "bp_1584": bpq_ms / max(bpq_n, 1),
                        "ce_llr": ce_ms / max(ce_n, 1)},
         "wrong_bitplanes": bad,
@@ -191,35 +189,51 @@ def bench_wz(args):
     return res
 
 
+def profiled_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE and WRITE_SIZE passes scaled by the
+    copy calibration), or None when there is none."""
+    f = ROOT / "profiles" / f"{kernel}_{workload}.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith(f"traffic_bytes_per_launch {kernel} "):
+            return float(line.split()[-1])
+    return None
+
+
+def bp_roofline(kernel_ms_avg, hbm_bytes_per_launch, edge_updates_per_launch, workload):
+    """Roofline of the BP kernel.  HBM: per (bitplane, code) the sequential
+    search runs, 4N (LLR) + ceil(m/8) (syndrome) + ceil(N/8) (decisions) + 2E
+    (16-bit row lists).  LDS: 16 bytes per edge-update (v2c read and write,
+    c2v read and write)."""
+    t = kernel_ms_avg * 1e-3
+    hbm = hbm_bytes_per_launch / t / 1e9
+    lds = 16.0 * edge_updates_per_launch / t / 1e9
+    return {"bound": "hbm", "kernel": "bp_6336", "achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": hbm / HBM_PEAK_GBS, "traffic": profiled_traffic("bp_6336", workload),
+            "kernel_ms_avg": kernel_ms_avg, "algorithmic_bytes_per_launch": hbm_bytes_per_launch,
+            "lds": {"achieved": lds, "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": lds / LDS_PEAK_GBS,
+                    "bytes_per_launch": 16.0 * edge_updates_per_launch},
+            "profile": f"profiles/bp_6336_{workload}.md"}
+
+
 def roofline_wz(dom, kern, args, nframes, outs):
-    """Roofline of the dominant kernel, from its algorithmic traffic or work."""
+    """Roofline of the dominant kernel (the BP launch of each bitplane step)."""
     name, d = dom
-    avg_ms = d["total_ms"] / max(d["launches"], 1)
-    out = {"bound": "hbm", "kernel": name, "kernel_ms_avg": avg_ms, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "traffic": None}
-    if name == "bp_6336":
-        # per launch (one bitplane step of all frames): per code run 4N LLR +
-        # m/8 syndrome + N/8 decisions + 2E row indices
-        byts = 0
-        for _, st in outs:
-            for b in range(st["nbitplanes"]):
-                for lvl in range(st["start_levels"][b], st["levels"][b] + 1):
-                    if lvl < 64:
-                        byts += 4 * 6336 + (96 * (lvl + 2) + 7) // 8 + 792 + 2 * 19008
-        per_launch = byts / max(d["launches"], 1)
-    elif name == "noise_dist":
-        # per launch: the distribution written once (4 bytes per value) plus
-        # the SMF window and SI coefficients read per block
-        per_launch = None
-    else:
-        per_launch = None
-    if per_launch:
-        out["achieved"] = per_launch / (avg_ms * 1e-3) / 1e9
-        out["frac"] = out["achieved"] / HBM_PEAK_GBS
-    else:
-        out["achieved"] = None
-        out["frac"] = None
-    return out
+    if name != "bp_6336":
+        return {"bound": "hbm", "kernel": name, "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": None, "traffic": None}
+    launches = max(d["launches"], 1)
+    byts = 0
+    eu = 0
+    for _, st in outs:
+        for b in range(st["nbitplanes"]):
+            for lvl in range(st["start_levels"][b], st["levels"][b] + 1):
+                if lvl < 64:
+                    byts += 4 * 6336 + (96 * (lvl + 2) + 7) // 8 + 792 + 2 * 19008
+            eu += st["iterations"][b] * 3 * 6336
+    return bp_roofline(d["total_ms"] / launches, byts / launches, eu / launches, "wz_cif_q8")
 
 
 def cpu_baseline_wz(jobs, args):
