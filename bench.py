@@ -7,7 +7,8 @@ Workloads
                 CIF 352x288, 33 frames, GOP 2, Qindex 8 (16 WZ frames, 63
                 bitplanes each, N = 6336), synthetic sequence as described in
                 dvc_amd.workload.  One step decodes `--streams` such sequences
-                (16 WZ frames each) as one batch with the speculative
+                (16 WZ frames each, 16 sequences = 256 frames by default) through
+                the continuous-batching engine (2 lanes) with the speculative
                 persistent LDPCA launch; value = WZ frames/s.  The plain path
                 (one frame at a time, sequential rate search) is timed on one
                 sequence beside it.
@@ -131,13 +132,17 @@ def bench_wz(args):
     log(f"[bench] encoded {len(jobs)} CIF Q8 WZ frames in {time.time() - t0:.1f}s")
     ctx = _lib.Context(device=0, max_iter=args.max_iter, bp_threads=args.bp_threads)
     for i in range(args.warmup):
-        ctx.decode_wz_frames(jobs[:16], batched=True)
+        eng = _lib.Engine(ctx, max_inflight=len(jobs), lanes=args.lanes)
+        eng.decode_all(jobs[:16])
+        eng.close()
         log(f"[bench] warmup {i} done")
     ctx.profile(True)
     times = []
     for i in range(args.steps):
         t = time.perf_counter()
-        outs = ctx.decode_wz_frames(jobs, batched=True)
+        eng = _lib.Engine(ctx, max_inflight=len(jobs), lanes=args.lanes)
+        outs = eng.decode_all(jobs)
+        eng.close()
         times.append(time.perf_counter() - t)
         log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms for {len(jobs)} frames")
     names = ["create_si", "keyframe_filters", "noise_param", "oct", "smf_init", "noise_dist", "condbit", "ce_llr",
@@ -313,10 +318,11 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch", "hbm_copy"])
-    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--streams", type=int, default=16)
     ap.add_argument("--plain-frames", type=int, default=4)
     ap.add_argument("--max-iter", type=int, default=100)
     ap.add_argument("--bp-threads", type=int, default=0, help="threads per CIF BP workgroup (0: library default)")
+    ap.add_argument("--lanes", type=int, default=2, help="engine lanes (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-planes", type=int, default=150)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

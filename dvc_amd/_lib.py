@@ -73,7 +73,8 @@ class WZFrameSpec(ctypes.Structure):
 
 
 class EngineParams(ctypes.Structure):
-    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int), ("sequential_bp", ctypes.c_int)]
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int), ("sequential_bp", ctypes.c_int),
+                ("lanes", ctypes.c_int)]
 
 
 # Every symbol include/dvc_accel.h declares, with its ctypes signature.
@@ -253,8 +254,8 @@ class Engine:
     """Continuous-batching decoder (dvc_engine_*): submit frames at any time,
     step, and collect results as their tickets complete."""
 
-    def __init__(self, ctx, max_inflight=64, hbm_budget_bytes=0, sequential_bp=False):
-        p = EngineParams(int(hbm_budget_bytes), int(max_inflight), 1 if sequential_bp else 0)
+    def __init__(self, ctx, max_inflight=64, hbm_budget_bytes=0, sequential_bp=False, lanes=0):
+        p = EngineParams(int(hbm_budget_bytes), int(max_inflight), 1 if sequential_bp else 0, int(lanes))
         h = ctypes.c_void_p()
         _check(lib.dvc_engine_create(ctx.h, ctypes.byref(p), ctypes.byref(h)), "dvc_engine_create")
         self.h = h
@@ -287,6 +288,18 @@ class Engine:
         q, i, u = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
         _check(lib.dvc_engine_status(self.h, ctypes.byref(q), ctypes.byref(i), ctypes.byref(u)), "dvc_engine_status")
         return q.value, i.value, u.value
+
+    def decode_all(self, frames):
+        """Submits every frame, steps until all are done, and returns their
+        (recons, stats) in submission order."""
+        tickets = [self.submit(fr) for fr in frames]
+        pending = set(tickets)
+        while pending:
+            self.step()
+            for t in list(pending):
+                if self.done(t):
+                    pending.discard(t)
+        return [self.result(t) for t in tickets]
 
     def release_key(self, key):
         _check(lib.dvc_engine_release_key(self.h, key), "dvc_engine_release_key")

@@ -59,6 +59,7 @@ using namespace dvc;
 hipEvent_t dvc_ctx::timer_begin(const char *name, hipStream_t s, int *slot) {
   *slot = -1;
   if (!profiling) return nullptr;
+  std::lock_guard<std::mutex> lock(timer_mu);
   int k = -1;
   for (size_t i = 0; i < timers.size(); i++)
     if (timers[i].name == name) { k = (int)i; break; }
@@ -75,10 +76,12 @@ void dvc_ctx::timer_end(int slot, hipEvent_t start, hipStream_t s) {
   hipEvent_t e;
   if (hipEventCreate(&e) != hipSuccess) return;
   (void)hipEventRecord(e, s);
+  std::lock_guard<std::mutex> lock(timer_mu);
   timers[slot].pending.push_back({start, e});
 }
 
 void dvc_ctx::timer_collect() {
+  std::lock_guard<std::mutex> lock(timer_mu);
   for (auto &t : timers) {
     for (auto &pe : t.pending) {
       float ms = 0.0f;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "../../include/dvc_accel.h"
@@ -66,6 +67,7 @@ struct KernelTimer {
 struct dvc_ctx {
   bool profiling = false;
   std::vector<KernelTimer> timers;
+  std::mutex timer_mu;  // engine lanes launch from several host threads
   // Records start/stop events around a launch when profiling is on.
   hipEvent_t timer_begin(const char *name, hipStream_t s, int *slot);
   void timer_end(int slot, hipEvent_t start, hipStream_t s);

@@ -20,6 +20,8 @@
 #include <algorithm>
 #include <deque>
 #include <map>
+#include <string>
+#include <thread>
 #include <vector>
 #include "dvc_internal.h"
 #include "frame_kernels.h"
@@ -108,6 +110,9 @@ struct Job {
   float *d_LLR = nullptr;
   // cursor
   int bandPos = 0, bit = 0, bp = 0;
+  int fixups = 0;                       // reconstruction entries recomputed on the host
+  std::vector<PlaneState> initState;    // upload sources, alive until the frame retires
+  std::vector<int4> initRec;
   bool done() const { return bandPos >= (int)bands.size(); }
 };
 
@@ -293,6 +298,30 @@ struct KeyEntry {
   size_t bytes = 0;
 };
 
+// One lane of the engine: a stream, the frames it advances in lockstep and
+// its device scratch.  Lanes step concurrently from host threads, so the tail
+// of one lane's LDPCA launch and its host round trips overlap the other
+// lanes' kernels.
+struct Lane {
+  hipStream_t s = nullptr;
+  int cap = 0, groupCap = 0;
+  std::vector<Job *> inflight;
+  std::vector<Job *> finished;  // retired by the engine after the step
+  FrameDev *d_frames = nullptr;
+  int *d_lists = nullptr;
+  PlaneDesc *d_planes = nullptr;
+  BpFrameLink *d_links = nullptr;
+  uint8_t *d_slots = nullptr;
+  int *d_counters = nullptr;
+  int *d_ties = nullptr;
+  int rc = 0;
+  std::string err;
+  // host staging of one step, alive until the step's last synchronisation
+  std::vector<FrameDev> hf;
+  std::vector<int> l0, l1, l2, l3, ties;
+  std::vector<BpFrameLink> links;
+  std::vector<PlaneDesc> groups[2];
+};
 }  // namespace dvc
 
 struct dvc_engine {
@@ -302,18 +331,14 @@ struct dvc_engine {
   bool sequentialBp = false;
   uint64_t nextTicket = 1, clock = 0;
   std::deque<dvc::Job *> pending;
-  std::vector<dvc::Job *> inflight;
+  std::vector<dvc::Lane> lanes;
   std::map<uint64_t, int> doneTickets;  // ticket -> 1
   std::map<uint64_t, dvc::KeyEntry> keys;
-  // device scratch sized for maxInflight frames
-  dvc::FrameDev *d_frames = nullptr;
-  int *d_lists = nullptr;
-  dvc::PlaneDesc *d_planes = nullptr;
-  dvc::BpFrameLink *d_links = nullptr;
-  uint8_t *d_slots = nullptr;
-  int *d_counters = nullptr;
-  int groupCap = 0;
-  long hostFixups = 0;
+  int inflight() const {
+    int n = 0;
+    for (auto &l : lanes) n += (int)l.inflight.size();
+    return n;
+  }
 };
 
 namespace dvc {
@@ -378,8 +403,9 @@ static int key_frame(dvc_engine *e, uint64_t key, bool named, const uint8_t *hos
   return DVC_OK;
 }
 
+// Allocates a frame's device state and uploads its inputs on stream s (the
+// caller synchronises s before any lane uses the frame).
 static int admit(dvc_engine *e, Job *j, hipStream_t s) {
-  dvc_ctx *c = e->c;
   Arena unused;
   int rc = alloc_job(*j, unused);
   if (rc) return rc;
@@ -397,13 +423,15 @@ static int admit(dvc_engine *e, Job *j, hipStream_t s) {
   f.fut = kf.full; f.fut_lp = kf.lp; f.fut_half = kf.half;
   DVC_HIP(hipMemcpyAsync(j->d_acc, j->acc.data(), j->acc.size(), hipMemcpyHostToDevice, s));
   DVC_HIP(hipMemsetAsync(f.reconsTrans, 0, sizeof(int) * j->W * j->H, s));
-  std::vector<int4> recInit((size_t)65 * j->nbp, make_int4(0, REC_NOT_RUN, -1, 0));
-  DVC_HIP(hipMemcpyAsync(j->d_rec, recInit.data(), recInit.size() * sizeof(int4), hipMemcpyHostToDevice, s));
-  DVC_HIP(hipStreamSynchronize(s));  // recInit lives on this stack frame
+  // per-bitplane rate-search state: start level from the entropy (computed on
+  // the device), no accepted code yet, and no record of any code run
+  j->initState.assign(j->nbp, PlaneState{-1, 64, 0.0f, 0});
+  j->initRec.assign((size_t)65 * j->nbp, make_int4(0, REC_NOT_RUN, -1, 0));
+  DVC_HIP(hipMemcpyAsync(j->d_state, j->initState.data(), sizeof(PlaneState) * j->nbp, hipMemcpyHostToDevice, s));
+  DVC_HIP(hipMemcpyAsync(j->d_rec, j->initRec.data(), j->initRec.size() * sizeof(int4), hipMemcpyHostToDevice, s));
   j->bandPos = 0;
   j->bit = j->bands.empty() ? 0 : j->bands[0].qbits;
   j->bp = 0;
-  (void)c;
   return DVC_OK;
 }
 
@@ -436,7 +464,7 @@ static int finish_job(dvc_engine *e, Job &j, hipStream_t s) {
     st->entropy[b] = ps[b].entropy;
     st->bitrate += L->syndLength[lvl] + 8;
   }
-  st->recon_host_fixups = (int)e->hostFixups;
+  st->recon_host_fixups = j.fixups;
   if (j.spec.orig) {
     double mse = 0.0;
     const int FS = j.W * j.H;
@@ -450,56 +478,38 @@ static int finish_job(dvc_engine *e, Job &j, hipStream_t s) {
   return DVC_OK;
 }
 
-// One engine step: admit queued frames while the HBM budget allows, set up the
-// newcomers, run one bitplane of every in-flight frame (with the band starts
-// and ends it implies), and retire finished frames.
-static int engine_step(dvc_engine *e, int *completed) {
+// One bitplane of every frame of a lane (with the band starts and ends it
+// implies), setting up the lane's new frames first and finishing the frames
+// whose last band ended.  Runs on the lane's stream only.
+static int lane_step(dvc_engine *e, Lane &ln) {
   dvc_ctx *c = e->c;
-  hipStream_t s = c->stream;
+  hipStream_t s = ln.s;
   int rc;
-  *completed = 0;
-  // ---- admission
-  std::vector<Job *> fresh;
-  while (!e->pending.empty() && (int)e->inflight.size() < e->maxInflight) {
-    Job *j = e->pending.front();
-    const size_t keyNeed = 12 * (size_t)j->W * j->H;
-    trim_keys(e, j->bytes + keyNeed);
-    if (!e->inflight.empty() && e->used + j->bytes + keyNeed > e->budget) break;
-    e->pending.pop_front();
-    if ((rc = admit(e, j, s))) { delete j; return rc; }
-    e->inflight.push_back(j);
-    fresh.push_back(j);
-  }
-  const int n = (int)e->inflight.size();
+  const int n = (int)ln.inflight.size();
   if (n == 0) return DVC_OK;
-  std::vector<FrameDev> hf(n);
-  std::vector<int> hl0, hl1, hl2, hl3;
+  ln.hf.resize(n);
+  ln.l0.clear(); ln.l1.clear(); ln.l2.clear(); ln.l3.clear();
   int maxN = 0, maxMB = 0, maxPos = 0;
   for (int i = 0; i < n; i++) {
-    Job *j = e->inflight[i];
+    Job *j = ln.inflight[i];
     maxN = std::max(maxN, j->N);
     maxMB = std::max(maxMB, j->W * j->H / 256);
     maxPos = std::max(maxPos, (j->W - 3) * (j->H - 3));
+    j->fd.tieCount = ln.d_ties + i;
   }
   CostTables ct{c->d_costFme, c->d_costUsi};
-  const int *L0 = e->d_lists, *L1 = e->d_lists + e->maxInflight, *L2 = e->d_lists + 2 * e->maxInflight,
-            *L3 = e->d_lists + 3 * e->maxInflight;
-  auto up_frames = [&]() -> int {
-    for (int i = 0; i < n; i++) hf[i] = e->inflight[i]->fd;
-    DVC_HIP(hipMemcpyAsync(e->d_frames, hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
-    return DVC_OK;
-  };
+  const int *L0 = ln.d_lists, *L1 = ln.d_lists + ln.cap, *L2 = ln.d_lists + 2 * ln.cap, *L3 = ln.d_lists + 3 * ln.cap;
   auto up_list = [&](const int *dst, const std::vector<int> &l) -> int {
     DVC_HIP(hipMemcpyAsync((void *)dst, l.data(), sizeof(int) * l.size(), hipMemcpyHostToDevice, s));
     return DVC_OK;
   };
   // ---- setup of newly admitted frames: SI, noise parameters, OCT, SMF
+  std::vector<int> &hl3 = ln.l3, &active = ln.l0, &starting = ln.l1, &ending = ln.l2;
   for (int i = 0; i < n; i++)
-    if (!e->inflight[i]->setup) hl3.push_back(i);
+    if (!ln.inflight[i]->setup) hl3.push_back(i);
   // band parameters of this step
-  std::vector<int> active, starting, ending;
   for (int i = 0; i < n; i++) {
-    Job &j = *e->inflight[i];
+    Job &j = *ln.inflight[i];
     if (j.done()) continue;
     const BandPlan &b = j.bands[j.bandPos];
     if (j.bit == b.qbits) {
@@ -511,86 +521,87 @@ static int engine_step(dvc_engine *e, int *completed) {
     active.push_back(i);
     if (j.bit == 1) ending.push_back(i);
   }
-  if ((rc = up_frames())) return rc;
+  for (int i = 0; i < n; i++) ln.hf[i] = ln.inflight[i]->fd;
+  DVC_HIP(hipMemcpyAsync(ln.d_frames, ln.hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
   if (!hl3.empty()) {
     const int nf = (int)hl3.size();
     if ((rc = up_list(L3, hl3))) return rc;
-    TIMED_LAUNCH(c, "create_si", s, launch_create_si(e->d_frames, L3, nf, maxMB, ct, s));
-    TIMED_LAUNCH(c, "noise_param", s, launch_noise_param(e->d_frames, L3, nf, maxN, s));
-    TIMED_LAUNCH(c, "oct", s, launch_oct(e->d_frames, L3, nf, maxPos, 1, s));
-    TIMED_LAUNCH(c, "smf_init", s, launch_smf_init(e->d_frames, L3, nf, maxN, s));
-    for (int i : hl3) e->inflight[i]->setup = true;
+    TIMED_LAUNCH(c, "create_si", s, launch_create_si(ln.d_frames, L3, nf, maxMB, ct, s));
+    TIMED_LAUNCH(c, "noise_param", s, launch_noise_param(ln.d_frames, L3, nf, maxN, s));
+    TIMED_LAUNCH(c, "oct", s, launch_oct(ln.d_frames, L3, nf, maxPos, 1, s));
+    TIMED_LAUNCH(c, "smf_init", s, launch_smf_init(ln.d_frames, L3, nf, maxN, s));
+    for (int i : hl3) ln.inflight[i]->setup = true;
   }
   if (!active.empty()) {
     if (!starting.empty()) {
       int maxStride = 0;
-      for (int i : starting) maxStride = std::max(maxStride, e->inflight[i]->fd.distStride);
+      for (int i : starting) maxStride = std::max(maxStride, ln.inflight[i]->fd.distStride);
       if ((rc = up_list(L1, starting))) return rc;
-      TIMED_LAUNCH(c, "noise_dist", s, launch_noise_dist(e->d_frames, L1, (int)starting.size(), maxN, maxStride, s));
+      TIMED_LAUNCH(c, "noise_dist", s, launch_noise_dist(ln.d_frames, L1, (int)starting.size(), maxN, maxStride, s));
     }
     if ((rc = up_list(L0, active))) return rc;
-    TIMED_LAUNCH(c, "condbit", s, launch_condbit(e->d_frames, L0, (int)active.size(), maxN, s));
-    std::vector<PlaneDesc> groups[2];
-    std::vector<BpFrameLink> links(active.size());
-    std::vector<PlaneState> init(active.size(), PlaneState{-1, 64, 0.0f, 0});
+    TIMED_LAUNCH(c, "condbit", s, launch_condbit(ln.d_frames, L0, (int)active.size(), maxN, s));
+    ln.groups[0].clear();
+    ln.groups[1].clear();
+    ln.links.resize(active.size());
     for (size_t a = 0; a < active.size(); a++) {
-      Job &j = *e->inflight[active[a]];
-      DVC_HIP(hipMemcpyAsync(j.d_state + j.bp, &init[a], sizeof(PlaneState), hipMemcpyHostToDevice, s));
+      Job &j = *ln.inflight[active[a]];
       PlaneDesc pd{j.N, j.fd.LR, j.d_LLR, j.d_acc + (size_t)j.bp * (j.N / 8), (uint32_t)j.crc[j.bp],
                    j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp};
-      groups[j.N == 6336].push_back(pd);
-      links[a] = BpFrameLink{j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp,
-                             e->d_slots + (size_t)(j.N == 6336 ? e->groupCap : 0) * 6336, e->groupCap};
+      ln.groups[j.N == 6336].push_back(pd);
+      ln.links[a] = BpFrameLink{j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp,
+                                ln.d_slots + (size_t)(j.N == 6336 ? ln.groupCap : 0) * 6336, ln.groupCap};
     }
-    DVC_HIP(hipStreamSynchronize(s));
     int planeOff = 0;
     for (int g = 0; g < 2; g++) {
-      if (groups[g].empty()) continue;
-      rc = bp_step(c, g ? 6336 : 1584, groups[g], e->sequentialBp, e->d_slots + (size_t)(g ? e->groupCap : 0) * 6336,
-                   e->groupCap, e->d_counters, e->d_planes + planeOff, s);
+      if (ln.groups[g].empty()) continue;
+      rc = bp_step(c, g ? 6336 : 1584, ln.groups[g], e->sequentialBp, ln.d_slots + (size_t)(g ? ln.groupCap : 0) * 6336,
+                   ln.groupCap, ln.d_counters + 2 * g, ln.d_planes + planeOff, s);
       if (rc) return rc;
-      planeOff += (int)groups[g].size();
-      DVC_HIP(hipStreamSynchronize(s));
+      planeOff += (int)ln.groups[g].size();
     }
-    DVC_HIP(hipMemcpyAsync(e->d_links, links.data(), sizeof(BpFrameLink) * links.size(), hipMemcpyHostToDevice, s));
-    TIMED_LAUNCH(c, "put_bitplane", s, launch_put_bitplane(e->d_frames, L0, e->d_links, (int)active.size(), maxN, s));
+    DVC_HIP(hipMemcpyAsync(ln.d_links, ln.links.data(), sizeof(BpFrameLink) * ln.links.size(), hipMemcpyHostToDevice, s));
+    TIMED_LAUNCH(c, "put_bitplane", s, launch_put_bitplane(ln.d_frames, L0, ln.d_links, (int)active.size(), maxN, s));
     for (int i : active) {
-      e->inflight[i]->bp++;
-      e->inflight[i]->bit--;
+      ln.inflight[i]->bp++;
+      ln.inflight[i]->bit--;
     }
     if (!ending.empty()) {
       const int ne = (int)ending.size();
       if ((rc = up_list(L2, ending))) return rc;
-      for (int i : ending) DVC_HIP(hipMemsetAsync(e->inflight[i]->fd.tieCount, 0, sizeof(int), s));
-      TIMED_LAUNCH(c, "reconstruct", s, launch_reconstruct(e->d_frames, L2, ne, maxN, s));
-      std::vector<int> ties(ne);
-      for (int k = 0; k < ne; k++)
-        DVC_HIP(hipMemcpyAsync(&ties[k], e->inflight[ending[k]]->fd.tieCount, sizeof(int), hipMemcpyDeviceToHost, s));
+      DVC_HIP(hipMemsetAsync(ln.d_ties, 0, sizeof(int) * n, s));
+      TIMED_LAUNCH(c, "reconstruct", s, launch_reconstruct(ln.d_frames, L2, ne, maxN, s));
+      ln.ties.assign(n, 0);
+      DVC_HIP(hipMemcpyAsync(ln.ties.data(), ln.d_ties, sizeof(int) * n, hipMemcpyDeviceToHost, s));
       DVC_HIP(hipStreamSynchronize(s));
-      for (int k = 0; k < ne; k++) {
-        Job &j = *e->inflight[ending[k]];
-        DVC_CHECK(ties[k] <= j.N, DVC_ESTATE, "reconstruction tie list overflow");
-        if (ties[k] == 0) continue;
-        std::vector<int> idx(ties[k]);
-        DVC_HIP(hipMemcpy(idx.data(), j.fd.tieList, sizeof(int) * ties[k], hipMemcpyDeviceToHost));
+      for (int i : ending) {
+        Job &j = *ln.inflight[i];
+        const int nt = ln.ties[i];
+        DVC_CHECK(nt <= j.N, DVC_ESTATE, "reconstruction tie list overflow");
+        if (nt == 0) continue;
+        std::vector<int> idx(nt);
+        DVC_HIP(hipMemcpyAsync(idx.data(), j.fd.tieList, sizeof(int) * nt, hipMemcpyDeviceToHost, s));
+        DVC_HIP(hipStreamSynchronize(s));
         for (int ci : idx) {
           int rt, side;
           float al;
-          DVC_HIP(hipMemcpy(&rt, j.fd.reconsTrans + ci, sizeof(int), hipMemcpyDeviceToHost));
-          DVC_HIP(hipMemcpy(&side, j.fd.sideTrans + ci, sizeof(int), hipMemcpyDeviceToHost));
-          DVC_HIP(hipMemcpy(&al, j.fd.alpha + ci, sizeof(float), hipMemcpyDeviceToHost));
-          int v = recon_host(rt, side, al, j.fd.index, j.fd.step, j.fd.levelShift);
-          DVC_HIP(hipMemcpy(j.fd.sideTrans + ci, &v, sizeof(int), hipMemcpyHostToDevice));
-          e->hostFixups++;
+          DVC_HIP(hipMemcpyAsync(&rt, j.fd.reconsTrans + ci, sizeof(int), hipMemcpyDeviceToHost, s));
+          DVC_HIP(hipMemcpyAsync(&side, j.fd.sideTrans + ci, sizeof(int), hipMemcpyDeviceToHost, s));
+          DVC_HIP(hipMemcpyAsync(&al, j.fd.alpha + ci, sizeof(float), hipMemcpyDeviceToHost, s));
+          DVC_HIP(hipStreamSynchronize(s));
+          const int v = recon_host(rt, side, al, j.fd.index, j.fd.step, j.fd.levelShift);
+          DVC_HIP(hipMemcpyAsync(j.fd.sideTrans + ci, &v, sizeof(int), hipMemcpyHostToDevice, s));
+          DVC_HIP(hipStreamSynchronize(s));
+          j.fixups++;
         }
       }
-      TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(e->d_frames, L2, ne, maxN, s));
-      TIMED_LAUNCH(c, "update_si", s, launch_update_si(e->d_frames, L2, ne, maxN, ct, s));
-      TIMED_LAUNCH(c, "oct", s, launch_oct(e->d_frames, L2, ne, maxPos, 0, s));
-      TIMED_LAUNCH(c, "update_smf", s, launch_update_smf(e->d_frames, L2, ne, maxN, s));
-      TIMED_LAUNCH(c, "update_si_ml", s, launch_update_si_ml(e->d_frames, L2, ne, maxN, 0, s));
+      TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(ln.d_frames, L2, ne, maxN, s));
+      TIMED_LAUNCH(c, "update_si", s, launch_update_si(ln.d_frames, L2, ne, maxN, ct, s));
+      TIMED_LAUNCH(c, "oct", s, launch_oct(ln.d_frames, L2, ne, maxPos, 0, s));
+      TIMED_LAUNCH(c, "update_smf", s, launch_update_smf(ln.d_frames, L2, ne, maxN, s));
+      TIMED_LAUNCH(c, "update_si_ml", s, launch_update_si_ml(ln.d_frames, L2, ne, maxN, 0, s));
       for (int i : ending) {
-        Job &j = *e->inflight[i];
+        Job &j = *ln.inflight[i];
         j.bandPos++;
         if (!j.done()) j.bit = j.bands[j.bandPos].qbits;
       }
@@ -599,26 +610,73 @@ static int engine_step(dvc_engine *e, int *completed) {
   // ---- retire: final inverse transform (DVC_Decoder.cpp:411-423) and outputs
   std::vector<int> fin;
   for (int i = 0; i < n; i++)
-    if (e->inflight[i]->setup && e->inflight[i]->done()) fin.push_back(i);
+    if (ln.inflight[i]->setup && ln.inflight[i]->done()) fin.push_back(i);
   if (!fin.empty()) {
-    if ((rc = up_list(L3, fin))) return rc;
-    TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(e->d_frames, L3, (int)fin.size(), maxN, s));
+    ln.l3 = fin;
+    if ((rc = up_list(L3, ln.l3))) return rc;
+    TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(ln.d_frames, L3, (int)fin.size(), maxN, s));
     for (int i : fin)
-      if ((rc = finish_job(e, *e->inflight[i], s))) return rc;
+      if ((rc = finish_job(e, *ln.inflight[i], s))) return rc;
     std::vector<Job *> keep;
     for (int i = 0; i < n; i++) {
-      Job *j = e->inflight[i];
-      if (j->setup && j->done()) {
-        e->doneTickets[j->ticket] = 1;
-        free_job(e, j);
-        (*completed)++;
-      } else {
-        keep.push_back(j);
-      }
+      Job *j = ln.inflight[i];
+      if (j->setup && j->done()) ln.finished.push_back(j);
+      else keep.push_back(j);
     }
-    e->inflight.swap(keep);
+    ln.inflight.swap(keep);
   }
-  DVC_HIP(hipStreamSynchronize(s));  // host-side step buffers are reused next step
+  DVC_HIP(hipStreamSynchronize(s));  // the step's host staging is reused next step
+  return DVC_OK;
+}
+
+// One engine step: admit queued frames while the HBM budget and the lanes'
+// capacity allow (uploads on the context stream, finished before any lane
+// runs), step every lane with frames (concurrently when several have), then
+// retire the lanes' finished frames.
+static int engine_step(dvc_engine *e, int *completed) {
+  dvc_ctx *c = e->c;
+  int rc;
+  *completed = 0;
+  bool admitted = false;
+  while (!e->pending.empty() && e->inflight() < e->maxInflight) {
+    Job *j = e->pending.front();
+    Lane *dst = nullptr;
+    for (auto &l : e->lanes)
+      if ((int)l.inflight.size() < l.cap && (!dst || l.inflight.size() < dst->inflight.size())) dst = &l;
+    if (!dst) break;
+    const size_t keyNeed = 12 * (size_t)j->W * j->H;
+    trim_keys(e, j->bytes + keyNeed);
+    if (e->inflight() > 0 && e->used + j->bytes + keyNeed > e->budget) break;
+    e->pending.pop_front();
+    if ((rc = admit(e, j, c->stream))) { delete j; return rc; }
+    dst->inflight.push_back(j);
+    admitted = true;
+  }
+  if (admitted) DVC_HIP(hipStreamSynchronize(c->stream));
+  std::vector<Lane *> busy;
+  for (auto &l : e->lanes)
+    if (!l.inflight.empty()) busy.push_back(&l);
+  if (busy.empty()) return DVC_OK;
+  auto run = [e, c](Lane *l) {
+    l->rc = DVC_OK;
+    if (hipSetDevice(c->device) != hipSuccess) { l->rc = DVC_EDEVICE; l->err = "hipSetDevice failed"; return; }
+    l->rc = lane_step(e, *l);
+    if (l->rc) l->err = dvc_last_error();
+  };
+  std::vector<std::thread> th;
+  for (size_t k = 1; k < busy.size(); k++) th.emplace_back(run, busy[k]);
+  run(busy[0]);
+  for (auto &t : th) t.join();
+  for (Lane *l : busy)
+    if (l->rc) { set_error(l->err); return l->rc; }
+  for (auto &l : e->lanes) {
+    for (Job *j : l.finished) {
+      e->doneTickets[j->ticket] = 1;
+      free_job(e, j);
+      (*completed)++;
+    }
+    l.finished.clear();
+  }
   return DVC_OK;
 }
 
@@ -647,6 +705,8 @@ static int engine_submit(dvc_engine *e, const dvc_wz_frame *sp, uint64_t pastKey
   return DVC_OK;
 }
 
+static void engine_destroy(dvc_engine *e);
+
 static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **out) {
   DVC_CHECK(c && out, DVC_EINVAL, "dvc_engine_create: NULL argument");
   DVC_HIP(hipSetDevice(c->device));
@@ -659,18 +719,28 @@ static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **ou
   size_t freeB = 0, totalB = 0;
   (void)hipMemGetInfo(&freeB, &totalB);
   e->budget = (p && p->hbm_budget_bytes) ? (size_t)p->hbm_budget_bytes : (size_t)(0.8 * (double)freeB);
-  const int M = e->maxInflight;
-  e->groupCap = 65 * M + 1;
+  // lanes: as requested, else one per 16 frames in flight, at most 4
+  int nl = (p && p->lanes > 0) ? p->lanes : std::min(4, std::max(1, e->maxInflight / 16));
+  nl = std::min(nl, e->maxInflight);
+  e->lanes.resize(nl);
   hipError_t err = hipSuccess;
-  if (err == hipSuccess) err = hipMalloc((void **)&e->d_frames, sizeof(FrameDev) * M);
-  if (err == hipSuccess) err = hipMalloc((void **)&e->d_lists, sizeof(int) * 4 * (size_t)M);
-  if (err == hipSuccess) err = hipMalloc((void **)&e->d_planes, sizeof(PlaneDesc) * M);
-  if (err == hipSuccess) err = hipMalloc((void **)&e->d_links, sizeof(BpFrameLink) * M);
-  if (err == hipSuccess) err = hipMalloc((void **)&e->d_slots, (size_t)2 * e->groupCap * 6336);
-  if (err == hipSuccess) err = hipMalloc((void **)&e->d_counters, 2 * sizeof(int));
+  for (int k = 0; k < nl && err == hipSuccess; k++) {
+    Lane &l = e->lanes[k];
+    l.cap = (e->maxInflight + nl - 1) / nl;
+    l.groupCap = 65 * l.cap + 1;
+    const int M = l.cap;
+    err = hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipMalloc((void **)&l.d_frames, sizeof(FrameDev) * M);
+    if (err == hipSuccess) err = hipMalloc((void **)&l.d_lists, sizeof(int) * 4 * (size_t)M);
+    if (err == hipSuccess) err = hipMalloc((void **)&l.d_planes, sizeof(PlaneDesc) * M);
+    if (err == hipSuccess) err = hipMalloc((void **)&l.d_links, sizeof(BpFrameLink) * M);
+    if (err == hipSuccess) err = hipMalloc((void **)&l.d_slots, (size_t)2 * l.groupCap * 6336);
+    if (err == hipSuccess) err = hipMalloc((void **)&l.d_counters, 4 * sizeof(int));
+    if (err == hipSuccess) err = hipMalloc((void **)&l.d_ties, sizeof(int) * M);
+  }
   if (err != hipSuccess) {
     set_error(std::string("dvc_engine_create: ") + hipGetErrorString(err));
-    delete e;
+    engine_destroy(e);
     return DVC_ENOMEM;
   }
   *out = e;
@@ -681,12 +751,18 @@ static void engine_destroy(dvc_engine *e) {
   if (!e) return;
   (void)hipSetDevice(e->c->device);
   (void)hipStreamSynchronize(e->c->stream);
+  for (auto &l : e->lanes)
+    if (l.s) (void)hipStreamSynchronize(l.s);
   for (Job *j : e->pending) delete j;
-  for (Job *j : e->inflight) { if (j->mem) (void)hipFree(j->mem); delete j; }
+  for (auto &l : e->lanes) {
+    for (Job *j : l.inflight) { if (j->mem) (void)hipFree(j->mem); delete j; }
+    for (Job *j : l.finished) { if (j->mem) (void)hipFree(j->mem); delete j; }
+    void *ptrs[] = {l.d_frames, l.d_lists, l.d_planes, l.d_links, l.d_slots, l.d_counters, l.d_ties};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
+    if (l.s) (void)hipStreamDestroy(l.s);
+  }
   for (auto &kv : e->keys) (void)hipFree(kv.second.k.full);
-  void *ptrs[] = {e->d_frames, e->d_lists, e->d_planes, e->d_links, e->d_slots, e->d_counters};
-  for (void *p : ptrs)
-    if (p) (void)hipFree(p);
   delete e;
 }
 
@@ -708,7 +784,7 @@ int dvc_engine_step(dvc_engine *e, int *completed) {
 int dvc_engine_status(dvc_engine *e, int *queued, int *inflight, uint64_t *hbm_used) {
   DVC_CHECK(e, DVC_EINVAL, "dvc_engine_status: NULL engine");
   if (queued) *queued = (int)e->pending.size();
-  if (inflight) *inflight = (int)e->inflight.size();
+  if (inflight) *inflight = e->inflight();
   if (hbm_used) *hbm_used = e->used;
   return DVC_OK;
 }
@@ -743,7 +819,7 @@ int dvc_decode_wz_frames(dvc_ctx *c, int n, const dvc_wz_frame *frames, int batc
     int done = 0;
     rc = engine_step(e, &done);
     left -= done;
-    if (!rc && done == 0 && e->inflight.empty() && !e->pending.empty()) {
+    if (!rc && done == 0 && e->inflight() == 0 && !e->pending.empty()) {
       set_error("dvc_decode_wz_frames: frame does not fit the HBM budget");
       rc = DVC_ENOMEM;
     }

@@ -168,6 +168,8 @@ typedef struct {
   uint64_t hbm_budget_bytes;  /* 0: 80% of the free device memory */
   int max_inflight;           /* frames decoded concurrently (0: 64) */
   int sequential_bp;          /* 1: plain per-code rate search instead of the speculative launch */
+  int lanes;                  /* frame groups stepped concurrently, each on its own stream and host
+                                 thread (0: one per 16 frames in flight, at most 4) */
 } dvc_engine_params;
 int dvc_engine_create(dvc_ctx *ctx, const dvc_engine_params *params, dvc_engine **out);
 int dvc_engine_destroy(dvc_engine *eng);

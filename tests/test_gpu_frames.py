@@ -91,14 +91,15 @@ def test_frames_against_oracle_max_iter_100(size):
         c100.close()
 
 
-def test_engine_staggered_arrivals_equal_solo(ctx):
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_engine_staggered_arrivals_equal_solo(ctx, lanes):
     """Continuous batching: frames submitted while others are mid-decode
     (different sizes, quantisers and arrival steps) give the same results as
-    when decoded alone."""
+    when decoded alone, with one lane or with lanes stepping concurrently."""
     from dvc_amd import _lib
     cases = json.loads((GOLD / "frames.json").read_text())["cases"]
     inputs = [_case_inputs(c) for c in cases]
-    eng = _lib.Engine(ctx, max_inflight=3)
+    eng = _lib.Engine(ctx, max_inflight=3, lanes=lanes)
     tickets = {}
     step = 0
     pending = list(range(len(inputs)))
@@ -119,3 +120,23 @@ def test_engine_staggered_arrivals_equal_solo(ctx):
     for i, c in enumerate(cases):
         rec, st = results[i]
         _check_against_golden(rec, st, c)
+
+
+def test_engine_lanes_mixed_batch(ctx):
+    """Frames spread over several lanes (QCIF and CIF, Qindex 1..8, shared key
+    frames) decode to the same output as each frame alone on the plain path."""
+    from dvc_amd import _lib
+    frames = []
+    for k, (w, h, sh) in enumerate([(176, 144, 3), (352, 288, 6)]):
+        vid = W.synthetic_video(w, h, 5, sh, seed=900 + k)
+        for q in (1, 4, 8):
+            for cur in (1, 3):
+                frames.append(dict(past=vid[cur - 1], fut=vid[cur + 1], orig=vid[cur],
+                                   wz=W.encode_wz_frame(vid[cur], q), idx=(cur - 1, cur, cur + 1)))
+    eng = _lib.Engine(ctx, max_inflight=len(frames), lanes=4)
+    outs = eng.decode_all(frames)
+    eng.close()
+    for fr, (rec, st) in zip(frames, outs):
+        (rec1, st1), = ctx.decode_wz_frames([fr], batched=False)
+        assert np.array_equal(rec, rec1)
+        assert st["levels"] == st1["levels"] and st["bitrate"] == st1["bitrate"] and st["psnr"] == st1["psnr"]
