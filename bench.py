@@ -287,6 +287,34 @@ def cpu_baseline_ldpca(planes, args):
                       f"sequential rate search, {dt:.1f}s"}
 
 
+def bench_serve(args):
+    """Serving loop: 32 concurrent synthetic streams (24 QCIF + 8 CIF, GOP
+    from {2,4,8}, Qindex 1..8), Poisson arrivals at each rate of --rates;
+    value = frames/s at the highest rate, with p50/p99 latency per rate and a
+    sample of served frames checked against their solo decodes."""
+    from dvc_amd import _lib, serve
+    ctx = _lib.Context(device=0, max_iter=args.max_iter, bp_threads=args.bp_threads)
+    if args.warmup:
+        serve.run(ctx, serve.make_streams(n_qcif=2, n_cif=1, frames=5), rate=1000.0, lanes=args.lanes)
+    sweep = []
+    for rate in [float(r) for r in args.rates.split(",")]:
+        streams = serve.make_streams(frames=args.serve_frames)
+        res, _ = serve.run(ctx, streams, rate, max_inflight=args.serve_inflight, lanes=args.lanes,
+                           verify=serve.verify_solo(ctx, sample=8))
+        log(f"[bench] serve rate {rate:.0f}/s: {res['frames_per_s']:.1f} frames/s, p50 "
+            f"{res['latency_ms']['p50']:.0f} ms, p99 {res['latency_ms']['p99']:.0f} ms, {res['verified']}")
+        sweep.append(res)
+    top = sweep[-1]
+    return {"metric": METRIC, "value": top["frames_per_s"], "unit": "WZ frames/s", "n_gpus": 1,
+            "steps": len(sweep), "warmup": args.warmup, "ms_per_step": top["wall_s"] * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded QCIF/CIF sequences, Poisson arrivals)",
+            "config": {"workload": f"serve: 32 streams (24 QCIF + 8 CIF), {args.serve_frames} frames each, GOP U{{2,4,8}}, "
+                                   f"Qindex U{{1..8}}, Poisson arrivals at {args.rates} requests/s, max_iter={args.max_iter}"},
+            "latency_ms": {"p50": top["latency_ms"]["p50"], "p99": top["latency_ms"]["p99"]},
+            "sweep": sweep}
+
+
 def bench_copy(args):
     """HBM calibration: a device-to-device copy of a known size (1 GiB read +
     1 GiB written per step).  Its FETCH_SIZE / WRITE_SIZE readings scale the
@@ -317,12 +345,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch", "hbm_copy"])
+    ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch", "hbm_copy", "serve"])
     ap.add_argument("--streams", type=int, default=16)
     ap.add_argument("--plain-frames", type=int, default=4)
     ap.add_argument("--max-iter", type=int, default=100)
     ap.add_argument("--bp-threads", type=int, default=0, help="threads per CIF BP workgroup (0: library default)")
     ap.add_argument("--lanes", type=int, default=2, help="engine lanes (0: library default)")
+    ap.add_argument("--rates", default="50,200,800", help="serve: Poisson arrival rates (requests/s)")
+    ap.add_argument("--serve-frames", type=int, default=17, help="serve: frames per stream")
+    ap.add_argument("--serve-inflight", type=int, default=128, help="serve: engine frames in flight")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-planes", type=int, default=150)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -331,7 +362,8 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(streams shard across GPUs without any exchange; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
-    res = {"wz_cif_q8": bench_wz, "ldpca_batch": bench_ldpca, "hbm_copy": bench_copy}[args.workload](args)
+    res = {"wz_cif_q8": bench_wz, "ldpca_batch": bench_ldpca, "hbm_copy": bench_copy,
+           "serve": bench_serve}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -22,6 +22,7 @@
 #include <map>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 #include "dvc_internal.h"
 #include "frame_kernels.h"
@@ -235,7 +236,28 @@ int ensure_cost_tables(dvc_ctx *c) {
 // One bitplane step of LDPCA decoding for the planes of one code length.
 int bp_step(dvc_ctx *c, int N, const std::vector<PlaneDesc> &planes, bool sequential, uint8_t *d_slots, int slotCap,
             int *d_counters, PlaneDesc *d_planes, hipStream_t s);
+// The speculative step's launches alone (the plane descriptors already in
+// d_planes): entropy and LLR, counters, the persistent BP launch and the
+// full-rate inverse.  Kernel launches only, so a stream capture of it replays.
+int bp_step_launch(dvc_ctx *c, int N, int n, uint8_t *d_slots, int slotCap, int *d_counters, PlaneDesc *d_planes,
+                   hipStream_t s);
 }  // namespace dvc
+
+int dvc::bp_step_launch(dvc_ctx *c, int N, int n, uint8_t *d_slots, int slotCap, int *d_counters, PlaneDesc *d_planes,
+                        hipStream_t s) {
+  Ladder *L = c->ladder(N);
+  const int C = L->numCodes;
+  LadderDev ld = ladder_dev(*L);
+  const int threads = N == 1584 ? 256 : (c->bp_threads_cif > 0 ? c->bp_threads_cif : 1024);
+  const int maxOff = C - 1;
+  TIMED_LAUNCH(c, "ce_llr", s, launch_ce_llr(d_planes, n, 1, c->lad1584, c->lad6336, s));
+  DVC_HIP(launch_zero_ints(d_counters, 2, s));
+  BpArgs a{ld, d_planes, n, d_counters, n * maxOff, maxOff, c->max_iter, d_slots, d_counters + 1, slotCap};
+  const int grid = std::min(c->sm_count * (N == 6336 ? 1 : 4), n * maxOff);
+  TIMED_LAUNCH(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
+  TIMED_LAUNCH(c, "invh", s, launch_invh(d_planes, n, ld, d_slots, d_counters + 1, slotCap, s));
+  return DVC_OK;
+}
 
 int dvc::bp_step(dvc_ctx *c, int N, const std::vector<PlaneDesc> &planes, bool sequential, uint8_t *d_slots,
                  int slotCap, int *d_counters, PlaneDesc *d_planes, hipStream_t s) {
@@ -245,16 +267,8 @@ int dvc::bp_step(dvc_ctx *c, int N, const std::vector<PlaneDesc> &planes, bool s
   LadderDev ld = ladder_dev(*L);
   const int threads = N == 1584 ? 256 : (c->bp_threads_cif > 0 ? c->bp_threads_cif : 1024);
   DVC_HIP(hipMemcpyAsync(d_planes, planes.data(), sizeof(PlaneDesc) * n, hipMemcpyHostToDevice, s));
+  if (!sequential) return bp_step_launch(c, N, n, d_slots, slotCap, d_counters, d_planes, s);
   TIMED_LAUNCH(c, "ce_llr", s, launch_ce_llr(d_planes, n, 1, c->lad1584, c->lad6336, s));
-  if (!sequential) {
-    const int maxOff = C - 1;
-    DVC_HIP(hipMemsetAsync(d_counters, 0, 2 * sizeof(int), s));
-    BpArgs a{ld, d_planes, n, d_counters, n * maxOff, maxOff, c->max_iter, d_slots, d_counters + 1, slotCap};
-    int grid = std::min(c->sm_count * (N == 6336 ? 1 : 4), n * maxOff);
-    TIMED_LAUNCH(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
-    TIMED_LAUNCH(c, "invh", s, launch_invh(d_planes, n, ld, d_slots, d_counters + 1, slotCap, s));
-    return DVC_OK;
-  }
   // Plain path: per bitplane, one code per launch in increasing order, with
   // the host deciding the next code (the reference's beliefPropagation_CUDA loop).
   // Every host-side source below stays alive until the stream is drained.
@@ -321,6 +335,12 @@ struct Lane {
   std::vector<int> l0, l1, l2, l3, ties;
   std::vector<BpFrameLink> links;
   std::vector<PlaneDesc> groups[2];
+  // Captured launch sequences of steady steps (no frame starts or ends a
+  // band), keyed by their launch shape: frames stepped, planes of each code
+  // length, largest code length.  Their kernels read the step's frame list,
+  // frame descriptors and plane descriptors from this lane's buffers, which
+  // are uploaded before each replay.
+  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
 };
 }  // namespace dvc
 
@@ -329,6 +349,7 @@ struct dvc_engine {
   size_t budget = 0, used = 0;
   int maxInflight = 1;
   bool sequentialBp = false;
+  bool graphs = true;  // replay captured steady steps
   uint64_t nextTicket = 1, clock = 0;
   std::deque<dvc::Job *> pending;
   std::vector<dvc::Lane> lanes;
@@ -540,7 +561,6 @@ static int lane_step(dvc_engine *e, Lane &ln) {
       TIMED_LAUNCH(c, "noise_dist", s, launch_noise_dist(ln.d_frames, L1, (int)starting.size(), maxN, maxStride, s));
     }
     if ((rc = up_list(L0, active))) return rc;
-    TIMED_LAUNCH(c, "condbit", s, launch_condbit(ln.d_frames, L0, (int)active.size(), maxN, s));
     ln.groups[0].clear();
     ln.groups[1].clear();
     ln.links.resize(active.size());
@@ -552,16 +572,58 @@ static int lane_step(dvc_engine *e, Lane &ln) {
       ln.links[a] = BpFrameLink{j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp,
                                 ln.d_slots + (size_t)(j.N == 6336 ? ln.groupCap : 0) * 6336, ln.groupCap};
     }
-    int planeOff = 0;
-    for (int g = 0; g < 2; g++) {
-      if (ln.groups[g].empty()) continue;
-      rc = bp_step(c, g ? 6336 : 1584, ln.groups[g], e->sequentialBp, ln.d_slots + (size_t)(g ? ln.groupCap : 0) * 6336,
-                   ln.groupCap, ln.d_counters + 2 * g, ln.d_planes + planeOff, s);
-      if (rc) return rc;
-      planeOff += (int)ln.groups[g].size();
+    const int na = (int)active.size(), n0 = (int)ln.groups[0].size(), n1 = (int)ln.groups[1].size();
+    DVC_HIP(hipMemcpyAsync(ln.d_links, ln.links.data(), sizeof(BpFrameLink) * na, hipMemcpyHostToDevice, s));
+    if (!e->sequentialBp) {
+      if (n0) DVC_HIP(hipMemcpyAsync(ln.d_planes, ln.groups[0].data(), sizeof(PlaneDesc) * n0, hipMemcpyHostToDevice, s));
+      if (n1)
+        DVC_HIP(hipMemcpyAsync(ln.d_planes + n0, ln.groups[1].data(), sizeof(PlaneDesc) * n1, hipMemcpyHostToDevice, s));
     }
-    DVC_HIP(hipMemcpyAsync(ln.d_links, ln.links.data(), sizeof(BpFrameLink) * ln.links.size(), hipMemcpyHostToDevice, s));
-    TIMED_LAUNCH(c, "put_bitplane", s, launch_put_bitplane(ln.d_frames, L0, ln.d_links, (int)active.size(), maxN, s));
+    // the step's kernels: conditional bit probabilities, LDPCA per code
+    // length, bitplane insertion
+    auto launches = [&]() -> int {
+      TIMED_LAUNCH(c, "condbit", s, launch_condbit(ln.d_frames, L0, na, maxN, s));
+      int off = 0;
+      for (int g = 0; g < 2; g++) {
+        const int ng = g ? n1 : n0;
+        if (!ng) continue;
+        uint8_t *slots = ln.d_slots + (size_t)(g ? ln.groupCap : 0) * 6336;
+        const int r = e->sequentialBp
+                          ? bp_step(c, g ? 6336 : 1584, ln.groups[g], true, slots, ln.groupCap, ln.d_counters + 2 * g,
+                                    ln.d_planes + off, s)
+                          : bp_step_launch(c, g ? 6336 : 1584, ng, slots, ln.groupCap, ln.d_counters + 2 * g,
+                                           ln.d_planes + off, s);
+        if (r) return r;
+        off += ng;
+      }
+      TIMED_LAUNCH(c, "put_bitplane", s, launch_put_bitplane(ln.d_frames, L0, ln.d_links, na, maxN, s));
+      return DVC_OK;
+    };
+    const bool steady = e->graphs && !e->sequentialBp && !c->profiling && hl3.empty() && starting.empty() &&
+                        ending.empty();
+    if (steady) {
+      const auto key = std::make_tuple(na, n0, n1, maxN);
+      auto it = ln.graphs.find(key);
+      if (it == ln.graphs.end()) {
+        DVC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int lrc = launches();
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s, &graph);
+        if (lrc) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return lrc;
+        }
+        DVC_HIP(ce);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        DVC_HIP(ie);
+        it = ln.graphs.emplace(key, exec).first;
+      }
+      DVC_HIP(hipGraphLaunch(it->second, s));
+    } else if ((rc = launches())) {
+      return rc;
+    }
     for (int i : active) {
       ln.inflight[i]->bp++;
       ln.inflight[i]->bit--;
@@ -716,6 +778,7 @@ static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **ou
   e->c = c;
   e->maxInflight = (p && p->max_inflight > 0) ? p->max_inflight : 64;
   e->sequentialBp = p && p->sequential_bp;
+  e->graphs = !(p && p->eager_steps);
   size_t freeB = 0, totalB = 0;
   (void)hipMemGetInfo(&freeB, &totalB);
   e->budget = (p && p->hbm_budget_bytes) ? (size_t)p->hbm_budget_bytes : (size_t)(0.8 * (double)freeB);
@@ -757,6 +820,7 @@ static void engine_destroy(dvc_engine *e) {
   for (auto &l : e->lanes) {
     for (Job *j : l.inflight) { if (j->mem) (void)hipFree(j->mem); delete j; }
     for (Job *j : l.finished) { if (j->mem) (void)hipFree(j->mem); delete j; }
+    for (auto &g : l.graphs) (void)hipGraphExecDestroy(g.second);
     void *ptrs[] = {l.d_frames, l.d_lists, l.d_planes, l.d_links, l.d_slots, l.d_counters, l.d_ties};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);

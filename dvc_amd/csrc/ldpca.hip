@@ -322,7 +322,19 @@ __global__ void invh_kernel(const PlaneDesc *planes, int nplanes, LadderDev L, u
   }
 }
 
+// Clears n ints.  Used instead of hipMemsetAsync for the launch counters so a
+// captured step graph holds kernel nodes only.
+__global__ void zero_ints_kernel(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // ------------------------------------------------------------------ host side
+hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+  return hipGetLastError();
+}
+
 template <int N, int T>
 static hipError_t launch_bp(const BpArgs &a, int grid, hipStream_t s) {
   hipLaunchKernelGGL((bp_persistent_kernel<N, T>), dim3(grid), dim3(T), 0, s, a);

@@ -59,6 +59,7 @@ enum { REC_FAILED = 0, REC_CRC_MISMATCH = 1, REC_ACCEPTED = 2, REC_CANCELLED = 3
 
 LadderDev ladder_dev(const Ladder &L);
 hipError_t launch_bp_any(int N, int threads, const BpArgs &a, int grid, hipStream_t s);
+hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
 hipError_t launch_ce_llr(const PlaneDesc *d_planes, int n, int fixed_start, const Ladder &l1584,
                          const Ladder &l6336, hipStream_t s);
 hipError_t launch_invh(const PlaneDesc *d_planes, int n, const LadderDev &L, uint8_t *slots, int *slot_count,

@@ -10,7 +10,7 @@ the engine under per-stream keys; frames are admitted by HBM footprint and
 join the running batch between bitplane steps.  Latency is measured from
 arrival to completion.
 
-    python -m dvc_amd.serve --rate 200 --seconds 10
+    python -m dvc_amd.serve --rates 100,400,1600
 """
 import argparse
 import json
@@ -70,7 +70,7 @@ def key_id(sid, frame):
     return ((sid + 1) << 32) | frame
 
 
-def run(ctx, streams, rate, seed=1, max_inflight=64, duration_cap=120.0, verify=None):
+def run(ctx, streams, rate, seed=1, max_inflight=64, lanes=0, duration_cap=120.0, verify=None):
     """Feeds Poisson arrivals at `rate` requests/s and steps the engine.
     Returns a dict of throughput and latency figures."""
     rng = np.random.default_rng(seed)
@@ -87,7 +87,7 @@ def run(ctx, streams, rate, seed=1, max_inflight=64, duration_cap=120.0, verify=
         cur, past, fut = s.order[cursor[s.sid]]
         cursor[s.sid] += 1
         reqs.append(dict(t=t, s=s, cur=cur, past=past, fut=fut))
-    eng = _lib.Engine(ctx, max_inflight=max_inflight)
+    eng = _lib.Engine(ctx, max_inflight=max_inflight, lanes=lanes)
     t0 = time.perf_counter()
     nxt = 0
     waiting = []        # arrived, references not decoded yet
@@ -142,12 +142,16 @@ def run(ctx, streams, rate, seed=1, max_inflight=64, duration_cap=120.0, verify=
 
 
 def verify_solo(ctx, sample=8, seed=0):
-    """Checks a sample of served frames against solo decodes of the same frame
-    (with the references the serving loop used)."""
+    """Checks a sample of served frames (all of them when sample <= 0)
+    against solo decodes of the same frame with the references the serving
+    loop used."""
     def check(outputs, streams):
         rng = np.random.default_rng(seed)
         keys = list(outputs)
-        pick = [keys[i] for i in rng.choice(len(keys), min(sample, len(keys)), replace=False)]
+        if sample <= 0:
+            pick = keys
+        else:
+            pick = [keys[i] for i in rng.choice(len(keys), min(sample, len(keys)), replace=False)]
         bad = 0
         for sid, cur in pick:
             s = streams[sid]

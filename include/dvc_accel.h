@@ -170,6 +170,8 @@ typedef struct {
   int sequential_bp;          /* 1: plain per-code rate search instead of the speculative launch */
   int lanes;                  /* frame groups stepped concurrently, each on its own stream and host
                                  thread (0: one per 16 frames in flight, at most 4) */
+  int eager_steps;            /* 1: launch every step's kernels directly; 0: steps in which no frame
+                                 starts or ends a band replay a captured HIP graph of their launches */
 } dvc_engine_params;
 int dvc_engine_create(dvc_ctx *ctx, const dvc_engine_params *params, dvc_engine **out);
 int dvc_engine_destroy(dvc_engine *eng);

@@ -140,3 +140,24 @@ def test_engine_lanes_mixed_batch(ctx):
         (rec1, st1), = ctx.decode_wz_frames([fr], batched=False)
         assert np.array_equal(rec, rec1)
         assert st["levels"] == st1["levels"] and st["bitrate"] == st1["bitrate"] and st["psnr"] == st1["psnr"]
+
+
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_engine_graph_replay_equals_eager(ctx, lanes):
+    """Steps in which no frame starts or ends a band replay a captured graph
+    (dozens of replays per frame at Qindex 8, with eager band starts and ends
+    and synchronisations between them); every output equals the eager run."""
+    from dvc_amd import _lib
+    frames = []
+    for k, (w, h, sh, q) in enumerate([(352, 288, 6, 8), (176, 144, 3, 8), (352, 288, 6, 4), (176, 144, 3, 6)]):
+        vid = W.synthetic_video(w, h, 3, sh, seed=950 + k)
+        frames.append(dict(past=vid[0], fut=vid[2], orig=vid[1], wz=W.encode_wz_frame(vid[1], q)))
+    runs = []
+    for eager in (False, True):
+        eng = _lib.Engine(ctx, max_inflight=len(frames), lanes=lanes, eager_steps=eager)
+        runs.append(eng.decode_all(frames))
+        eng.close()
+    for (rg, sg), (re_, se) in zip(*runs):
+        assert np.array_equal(rg, re_)
+        assert sg["levels"] == se["levels"] and sg["iterations"] == se["iterations"]
+        assert sg["bitrate"] == se["bitrate"] and sg["psnr"] == se["psnr"]
