@@ -315,6 +315,58 @@ def bench_serve(args):
             "sweep": sweep}
 
 
+def bench_si(args):
+    """Batched side-information and noise-model generation: 64 CIF key-frame
+    pairs (16 consecutive pairs k, k+2 of 4 sequences of the CIF generator),
+    WZ frame at the midpoint; value = pairs/s of dvc_side_info_batch (host
+    buffers in and out), with a sample checked against the oracle."""
+    from dvc_amd import _lib, workload as W
+    pairs = []
+    for st in range(4):
+        vid = W.synthetic_video(352, 288, 33, 6, 1300 + st)
+        for k in range(0, 32, 2):
+            pairs.append(dict(past=vid[k], fut=vid[k + 2], idx=(k, k + 1, k + 2), qindex=8))
+    ctx = _lib.Context(device=0, max_iter=args.max_iter)
+    for _ in range(args.warmup):
+        ctx.side_info_batch(pairs)
+    ctx.profile(True)
+    times = []
+    for i in range(args.steps):
+        t = time.perf_counter()
+        outs = ctx.side_info_batch(pairs)
+        times.append(time.perf_counter() - t)
+        log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms for {len(pairs)} pairs")
+    kern = {}
+    for nm in ["keyframe_filters", "create_si", "noise_param"]:
+        ms, cnt = ctx.profile_get(nm)
+        if cnt:
+            kern[nm] = {"total_ms": ms / args.steps, "launches": cnt // args.steps}
+    ctx.profile(False)
+    res = {"metric": METRIC, "value": len(pairs) / float(np.mean(times)), "unit": "SI+alpha key-frame pairs/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": float(np.mean(times)) * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32/f64/f32",
+           "data": "synthetic (seeded CIF sequences)",
+           "config": {"workload": "si_batch: 64 CIF key-frame pairs, WZ frame at the midpoint, Qindex 8 bands for alpha"},
+           "kernels_ms_per_step": kern}
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        from test_gpu_si import oracle_si_alpha
+        t = time.perf_counter()
+        n = 0
+        bad = 0
+        for d, (si, r, al) in zip(pairs, outs):
+            osi, ores, oal = oracle_si_alpha(d["past"], d["fut"], d["idx"], 8)
+            bad += int(not (np.array_equal(si, osi) and np.array_equal(al.view(np.uint32), oal.view(np.uint32))))
+            n += 1
+            if time.perf_counter() - t > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t
+        res["cpu_baseline"] = {"value": n / dt, "unit": "SI+alpha key-frame pairs/s", "cores": 1, "kind": "port",
+                               "sample": f"{n} pairs of the same set on the oracle (sequential), {dt:.1f}s; "
+                                         f"{bad} mismatching outputs"}
+    return res
+
+
 def bench_copy(args):
     """HBM calibration: a device-to-device copy of a known size (1 GiB read +
     1 GiB written per step).  Its FETCH_SIZE / WRITE_SIZE readings scale the
@@ -345,7 +397,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch", "hbm_copy", "serve"])
+    ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch", "hbm_copy", "serve", "si_batch"])
     ap.add_argument("--streams", type=int, default=16)
     ap.add_argument("--plain-frames", type=int, default=4)
     ap.add_argument("--max-iter", type=int, default=100)
@@ -363,7 +415,7 @@ def main():
               "(streams shard across GPUs without any exchange; see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
     res = {"wz_cif_q8": bench_wz, "ldpca_batch": bench_ldpca, "hbm_copy": bench_copy,
-           "serve": bench_serve}[args.workload](args)
+           "serve": bench_serve, "si_batch": bench_si}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

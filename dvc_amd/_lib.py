@@ -72,6 +72,13 @@ class WZFrameSpec(ctypes.Structure):
                 ("recons", ctypes.c_void_p), ("stats", ctypes.POINTER(FrameStats))]
 
 
+class SIRequest(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("qindex", ctypes.c_int),
+                ("past_idx", ctypes.c_int), ("cur_idx", ctypes.c_int), ("fut_idx", ctypes.c_int),
+                ("past", ctypes.c_void_p), ("fut", ctypes.c_void_p), ("si", ctypes.c_void_p),
+                ("residue", ctypes.c_void_p), ("alpha", ctypes.c_void_p)]
+
+
 class EngineParams(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int), ("sequential_bp", ctypes.c_int),
                 ("lanes", ctypes.c_int), ("eager_steps", ctypes.c_int)]
@@ -93,6 +100,7 @@ SIGNATURES = {
     "dvc_ldpca_decode_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(LdpcaItem)]),
     "dvc_selftest_unary": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_int]),
     "dvc_decode_wz_frames": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WZFrameSpec), _c.c_int]),
+    "dvc_side_info_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(SIRequest)]),
     "dvc_engine_create": (_c.c_int, [_c.c_void_p, _c.POINTER(EngineParams), _c.POINTER(_c.c_void_p)]),
     "dvc_engine_destroy": (_c.c_int, [_c.c_void_p]),
     "dvc_engine_submit": (_c.c_int, [_c.c_void_p, _c.POINTER(WZFrameSpec), _c.c_uint64, _c.c_uint64,
@@ -214,6 +222,29 @@ class Context:
                                    acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats[i]))
         _check(lib.dvc_decode_wz_frames(self.h, k, specs, 1 if batched else 0), "dvc_decode_wz_frames")
         return [(outs[i], stats[i].as_dict()) for i in range(k)]
+
+    def side_info_batch(self, pairs, qindex=8):
+        """pairs: list of (past, fut) uint8 HxW key frames, or of dicts with
+        past, fut and optional idx=(past, cur, fut) and qindex.  Returns a list
+        of (si uint8 HxW, residue int32 HxW, alpha float32 HxW)."""
+        k = len(pairs)
+        reqs = (SIRequest * max(k, 1))()
+        keep, outs = [], []
+        for i, pr in enumerate(pairs):
+            d = pr if isinstance(pr, dict) else dict(past=pr[0], fut=pr[1])
+            past = np.ascontiguousarray(d["past"], np.uint8)
+            fut = np.ascontiguousarray(d["fut"], np.uint8)
+            h, w = past.shape
+            si = np.zeros((h, w), np.uint8)
+            res = np.zeros((h, w), np.int32)
+            al = np.zeros((h, w), np.float32)
+            keep += [past, fut]
+            outs.append((si, res, al))
+            pi, ci, fi = d.get("idx", (0, 1, 2))
+            reqs[i] = SIRequest(w, h, int(d.get("qindex", qindex)), pi, ci, fi, past.ctypes.data, fut.ctypes.data,
+                                si.ctypes.data, res.ctypes.data, al.ctypes.data)
+        _check(lib.dvc_side_info_batch(self.h, k, reqs), "dvc_side_info_batch")
+        return outs
 
     def ldpca_decode_batch(self, lrs, accs, crcs, start_levels=None):
         """Speculative batched decode of many bitplanes (mixed N allowed)."""

@@ -147,6 +147,7 @@ int dvc_ctx_destroy(dvc_ctx *c) {
   free_ladder(c->lad6336);
   if (c->d_costFme) (void)hipFree(c->d_costFme);
   if (c->d_costUsi) (void)hipFree(c->d_costUsi);
+  if (c->si_scratch) (void)hipFree(c->si_scratch);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DVC_OK;

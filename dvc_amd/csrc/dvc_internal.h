@@ -80,11 +80,7 @@ struct dvc_ctx {
   int bp_threads_cif = 0;
   dvc::Ladder lad1584, lad6336;
   hipStream_t stream = nullptr;
-  // scratch for the single-bitplane path
-  float *d_LR = nullptr, *d_LLR = nullptr;
-  uint8_t *d_acc = nullptr, *d_decoded = nullptr;
-  int *d_result = nullptr;  // [0] converged, [1] crc, [2] iterations, [3] start level, [4] entropy bits
-  void *h_pinned = nullptr;
-  size_t h_pinned_bytes = 0;
+  void *si_scratch = nullptr;  // dvc_side_info_batch's device buffers, kept between calls
+  size_t si_scratch_bytes = 0;
   dvc::Ladder *ladder(int N) { return N == 1584 ? &lad1584 : (N == 6336 ? &lad6336 : nullptr); }
 };

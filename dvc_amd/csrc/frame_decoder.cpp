@@ -867,6 +867,104 @@ int dvc_engine_release_key(dvc_engine *e, uint64_t key) {
   return DVC_OK;
 }
 
+int dvc_side_info_batch(dvc_ctx *c, int n, const dvc_si_request *reqs) {
+  DVC_CHECK(c, DVC_EINVAL, "dvc_side_info_batch: NULL context");
+  DVC_CHECK(n >= 0 && (n == 0 || reqs), DVC_EINVAL, "dvc_side_info_batch: bad request list");
+  for (int i = 0; i < n; i++) {
+    const dvc_si_request &r = reqs[i];
+    DVC_CHECK((r.width == 176 && r.height == 144) || (r.width == 352 && r.height == 288), DVC_EINVAL,
+              "dvc_side_info_batch: size must be QCIF or CIF");
+    DVC_CHECK(r.qindex >= 1 && r.qindex <= 8, DVC_EINVAL, "dvc_side_info_batch: qindex must be 1..8");
+    DVC_CHECK(r.past && r.fut && r.si, DVC_EINVAL, "dvc_side_info_batch: NULL pointer");
+    DVC_CHECK(r.past_idx < r.cur_idx && r.cur_idx < r.fut_idx, DVC_EINVAL,
+              "dvc_side_info_batch: need past_idx < cur_idx < fut_idx");
+  }
+  if (n == 0) return DVC_OK;
+  DVC_HIP(hipSetDevice(c->device));
+  int rc = ensure_cost_tables(c);
+  if (rc) return rc;
+  hipStream_t s = c->stream;
+  // one allocation: per request the two key frames (full, low-pass, 2x), SI,
+  // residue, its transform, alpha and the motion-field scratch
+  std::vector<size_t> base(n + 1, 0);
+  int maxN = 0, maxMB = 0;
+  for (int i = 0; i < n; i++) {
+    const size_t FS = (size_t)reqs[i].width * reqs[i].height;
+    size_t t = 0;
+    for (size_t b : {6 * FS, 6 * FS, FS, 4 * FS, 4 * FS, 4 * FS, sizeof(MVd) * (FS / 256), sizeof(MVd) * (FS / 64),
+                     sizeof(MVd) * (FS / 64), sizeof(MVd) * (FS / 256), 16 * (FS / 256), 8 * (FS / 64)})
+      t = align_up(t + b);
+    base[i + 1] = base[i] + t;
+    maxN = std::max(maxN, (int)(FS / 16));
+    maxMB = std::max(maxMB, (int)(FS / 256));
+  }
+  // the context keeps the allocation for later calls (grown when too small)
+  const size_t need = base[n] + sizeof(FrameDev) * n + sizeof(int) * n;
+  if (c->si_scratch_bytes < need) {
+    if (c->si_scratch) DVC_HIP(hipFree(c->si_scratch));
+    c->si_scratch = nullptr;
+    c->si_scratch_bytes = 0;
+    DVC_HIP(hipMalloc(&c->si_scratch, need));
+    c->si_scratch_bytes = need;
+  }
+  uint8_t *mem = (uint8_t *)c->si_scratch;
+  FrameDev *d_frames = (FrameDev *)(mem + base[n]);
+  int *d_list = (int *)(d_frames + n);
+  std::vector<FrameDev> hf(n);
+  std::vector<int> list(n);
+  auto run = [&]() -> int {
+    for (int i = 0; i < n; i++) {
+      const dvc_si_request &r = reqs[i];
+      const int W = r.width, H = r.height;
+      const size_t FS = (size_t)W * H;
+      uint8_t *p = mem + base[i];
+      size_t off = 0;
+      auto take = [&](size_t bytes) { uint8_t *q = p + off; off = align_up(off + bytes); return q; };
+      uint8_t *kp = take(6 * FS), *kf = take(6 * FS);
+      FrameDev &f = hf[i];
+      f = FrameDev{};
+      f.W = W; f.H = H; f.N = (int)(FS / 16); f.qindex = r.qindex;
+      f.iPC = (double)(r.cur_idx - r.past_idx) / (double)(r.fut_idx - r.past_idx);
+      f.iFC = (double)(r.fut_idx - r.cur_idx) / (double)(r.fut_idx - r.past_idx);
+      f.past = kp; f.past_lp = kp + FS; f.past_half = kp + 2 * FS;
+      f.fut = kf; f.fut_lp = kf + FS; f.fut_half = kf + 2 * FS;
+      f.si = take(FS);
+      f.residue = (int *)take(4 * FS);
+      f.sideTrans = (int *)take(4 * FS);
+      f.alpha = (float *)take(4 * FS);
+      f.mbmv = (MVd *)take(sizeof(MVd) * (FS / 256));
+      f.bmv = (MVd *)take(sizeof(MVd) * (FS / 64));
+      f.bmv_tmp = (MVd *)take(sizeof(MVd) * (FS / 64));
+      f.fwd = (MVd *)take(sizeof(MVd) * (FS / 256));
+      f.holes = (double *)take(16 * (FS / 256));
+      f.mse = (double *)take(8 * (FS / 64));
+      list[i] = i;
+      DVC_HIP(hipMemcpyAsync(kp, r.past, FS, hipMemcpyHostToDevice, s));
+      DVC_HIP(hipMemcpyAsync(kf, r.fut, FS, hipMemcpyHostToDevice, s));
+      DVC_HIP(hipMemsetAsync(f.alpha, 0, 4 * FS, s));
+      TIMED_LAUNCH(c, "keyframe_filters", s, launch_keyframe_filters(kp, kp + FS, kp + 2 * FS, W, H, s));
+      TIMED_LAUNCH(c, "keyframe_filters", s, launch_keyframe_filters(kf, kf + FS, kf + 2 * FS, W, H, s));
+    }
+    DVC_HIP(hipMemcpyAsync(d_frames, hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
+    DVC_HIP(hipMemcpyAsync(d_list, list.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    CostTables ct{c->d_costFme, c->d_costUsi};
+    TIMED_LAUNCH(c, "create_si", s, launch_create_si(d_frames, d_list, n, maxMB, ct, s));
+    TIMED_LAUNCH(c, "noise_param", s, launch_noise_param(d_frames, d_list, n, maxN, s));
+    for (int i = 0; i < n; i++) {
+      const dvc_si_request &r = reqs[i];
+      const size_t FS = (size_t)r.width * r.height;
+      DVC_HIP(hipMemcpyAsync(r.si, hf[i].si, FS, hipMemcpyDeviceToHost, s));
+      if (r.residue) DVC_HIP(hipMemcpyAsync(r.residue, hf[i].residue, 4 * FS, hipMemcpyDeviceToHost, s));
+      if (r.alpha) DVC_HIP(hipMemcpyAsync(r.alpha, hf[i].alpha, 4 * FS, hipMemcpyDeviceToHost, s));
+    }
+    DVC_HIP(hipStreamSynchronize(s));
+    return DVC_OK;
+  };
+  rc = run();
+  (void)hipStreamSynchronize(s);
+  return rc;
+}
+
 int dvc_decode_wz_frames(dvc_ctx *c, int n, const dvc_wz_frame *frames, int batched) {
   DVC_CHECK(c, DVC_EINVAL, "dvc_decode_wz_frames: NULL context");
   DVC_CHECK(n >= 0 && (n == 0 || frames), DVC_EINVAL, "dvc_decode_wz_frames: bad frame list");

@@ -150,6 +150,26 @@ typedef struct {
  * and equal the CPU path's (ParallelMode 0). */
 int dvc_decode_wz_frames(dvc_ctx *ctx, int n, const dvc_wz_frame *frames, int batched);
 
+/* ---------------------------------------------------------------- side information
+ * createSideInfo (sideInfoCreation.cpp:438-551: key-frame low-pass and
+ * half-pel interpolation, forward ME, 16x16 and 8x8 bidirectional half-pel
+ * refinement, weighted vector-median smoothing, motion compensation), the
+ * residue's forward transform (transform.cpp:153-190) and noiseParameter
+ * (noiseModel.cpp:9-75) -- steps 2-4 of decodeGOP, DVC_Decoder.cpp:51-80 --
+ * for n frames in one batch.  Replaces forwardME_CUDA (forwardME_kernel.cu:
+ * 187-234) and the CPU steps around it.  Outputs equal the CPU path's. */
+typedef struct {
+  int width, height;          /* 176x144 or 352x288 */
+  int qindex;                 /* 1..8: the bands that get alpha (QtableBits > 0) */
+  int past_idx, cur_idx, fut_idx;
+  const uint8_t *past, *fut;  /* key frames, luma W*H */
+  uint8_t *si;                /* out: W*H side information */
+  int *residue;               /* out (may be NULL): W*H motion-compensated residue (F - P) >> 1 */
+  float *alpha;               /* out (may be NULL): W*H Laplacian parameter at the coefficient
+                                 positions of the bands, 0 elsewhere */
+} dvc_si_request;
+int dvc_side_info_batch(dvc_ctx *ctx, int n, const dvc_si_request *reqs);
+
 /* ---------------------------------------------------------------- serving engine
  * Continuous batching (no counterpart in the reference, which decodes frames
  * strictly one after another, main.cpp:132-142): requests are queued by
