@@ -81,7 +81,7 @@ class SIRequest(ctypes.Structure):
 
 class EngineParams(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int), ("sequential_bp", ctypes.c_int),
-                ("lanes", ctypes.c_int), ("eager_steps", ctypes.c_int)]
+                ("lanes", ctypes.c_int), ("eager_steps", ctypes.c_int), ("device_buffers", ctypes.c_int)]
 
 
 # Every symbol include/dvc_accel.h declares, with its ctypes signature.
@@ -101,6 +101,9 @@ SIGNATURES = {
     "dvc_selftest_unary": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_int]),
     "dvc_decode_wz_frames": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WZFrameSpec), _c.c_int]),
     "dvc_side_info_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(SIRequest)]),
+    "dvc_buffer_alloc": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_void_p)]),
+    "dvc_buffer_free": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
+    "dvc_buffer_copy": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_size_t, _c.c_int]),
     "dvc_engine_create": (_c.c_int, [_c.c_void_p, _c.POINTER(EngineParams), _c.POINTER(_c.c_void_p)]),
     "dvc_engine_destroy": (_c.c_int, [_c.c_void_p]),
     "dvc_engine_submit": (_c.c_int, [_c.c_void_p, _c.POINTER(WZFrameSpec), _c.c_uint64, _c.c_uint64,
@@ -127,6 +130,50 @@ def _ptr(a):
 
 def prepare_data(data_dir=None):
     _check(lib.dvc_prepare_data(str(data_dir).encode() if data_dir else None), "dvc_prepare_data")
+
+
+class DeviceBuffer:
+    """A device allocation of the context's device (dvc_buffer_*)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib.dvc_buffer_alloc(ctx.h, self.nbytes, ctypes.byref(p)), "dvc_buffer_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        b = cls(ctx, a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib.dvc_buffer_copy(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes, 0), "dvc_buffer_copy")
+
+    def download(self, shape, dtype):
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib.dvc_buffer_copy(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes, 1), "dvc_buffer_copy")
+        return out
+
+    def copy_from(self, other, nbytes=None):
+        n = self.nbytes if nbytes is None else int(nbytes)
+        _check(lib.dvc_buffer_copy(self.ctx.h, self.ptr, other.ptr, n, 2), "dvc_buffer_copy")
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            lib.dvc_buffer_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Context:
@@ -285,9 +332,10 @@ class Engine:
     """Continuous-batching decoder (dvc_engine_*): submit frames at any time,
     step, and collect results as their tickets complete."""
 
-    def __init__(self, ctx, max_inflight=64, hbm_budget_bytes=0, sequential_bp=False, lanes=0, eager_steps=False):
+    def __init__(self, ctx, max_inflight=64, hbm_budget_bytes=0, sequential_bp=False, lanes=0, eager_steps=False,
+                 device_buffers=False):
         p = EngineParams(int(hbm_budget_bytes), int(max_inflight), 1 if sequential_bp else 0, int(lanes),
-                         1 if eager_steps else 0)
+                         1 if eager_steps else 0, 1 if device_buffers else 0)
         h = ctypes.c_void_p()
         _check(lib.dvc_engine_create(ctx.h, ctypes.byref(p), ctypes.byref(h)), "dvc_engine_create")
         self.h = h
@@ -300,6 +348,22 @@ class Engine:
         t = ctypes.c_uint64()
         _check(lib.dvc_engine_submit(self.h, ctypes.byref(spec), past_key, fut_key, ctypes.byref(t)), "dvc_engine_submit")
         self.live[t.value] = (spec, keep, rec, stats)
+        return t.value
+
+    def submit_device(self, width, height, wz, past_ptr, fut_ptr, acc_ptr, recons_ptr, idx=(0, 1, 2),
+                      past_key=0, fut_key=0):
+        """Submits a frame whose key frames, accumulated syndromes and output
+        live in device memory (an engine made with device_buffers=True); the
+        pointers are integers (e.g. torch tensors' data_ptr()).  ac_range and
+        crc come from the host-side workload.WZFrame `wz`."""
+        stats = FrameStats()
+        acr = np.ascontiguousarray(wz.ac_range, np.int32)
+        crc = np.ascontiguousarray(wz.crc, np.uint8)
+        spec = WZFrameSpec(width, height, wz.qindex, idx[0], idx[1], idx[2], past_ptr, fut_ptr, None,
+                           acr.ctypes.data, crc.ctypes.data, acc_ptr, recons_ptr, ctypes.pointer(stats))
+        t = ctypes.c_uint64()
+        _check(lib.dvc_engine_submit(self.h, ctypes.byref(spec), past_key, fut_key, ctypes.byref(t)), "dvc_engine_submit")
+        self.live[t.value] = (spec, [acr, crc], None, stats)
         return t.value
 
     def step(self):

@@ -350,6 +350,7 @@ struct dvc_engine {
   int maxInflight = 1;
   bool sequentialBp = false;
   bool graphs = true;  // replay captured steady steps
+  bool deviceBuffers = false;  // past, fut, acc and recons of requests are device pointers
   uint64_t nextTicket = 1, clock = 0;
   std::deque<dvc::Job *> pending;
   std::vector<dvc::Lane> lanes;
@@ -399,7 +400,7 @@ static void trim_keys(dvc_engine *e, size_t need) {
   }
 }
 
-static int key_frame(dvc_engine *e, uint64_t key, bool named, const uint8_t *host, int W, int H, hipStream_t s,
+static int key_frame(dvc_engine *e, uint64_t key, bool named, const uint8_t *src, int W, int H, hipStream_t s,
                      KeyFrameDev *out) {
   auto it = e->keys.find(key);
   if (it != e->keys.end()) {
@@ -416,7 +417,7 @@ static int key_frame(dvc_engine *e, uint64_t key, bool named, const uint8_t *hos
   DVC_HIP(hipMalloc((void **)&p, ke.bytes));
   ke.k.full = p; ke.k.lp = p + FS; ke.k.half = p + 2 * FS;
   ke.W = W; ke.H = H; ke.refs = 1; ke.named = named; ke.lastUse = ++e->clock;
-  DVC_HIP(hipMemcpyAsync(ke.k.full, host, FS, hipMemcpyHostToDevice, s));
+  DVC_HIP(hipMemcpyAsync(ke.k.full, src, FS, e->deviceBuffers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
   TIMED_LAUNCH(e->c, "keyframe_filters", s, launch_keyframe_filters(ke.k.full, ke.k.lp, ke.k.half, W, H, s));
   e->used += ke.bytes;
   e->keys[key] = ke;
@@ -442,7 +443,10 @@ static int admit(dvc_engine *e, Job *j, hipStream_t s) {
   if ((rc = key_frame(e, j->futKey, fn, sp.fut, j->W, j->H, s, &kf))) return rc;
   f.past = kp.full; f.past_lp = kp.lp; f.past_half = kp.half;
   f.fut = kf.full; f.fut_lp = kf.lp; f.fut_half = kf.half;
-  DVC_HIP(hipMemcpyAsync(j->d_acc, j->acc.data(), j->acc.size(), hipMemcpyHostToDevice, s));
+  if (e->deviceBuffers)
+    DVC_HIP(hipMemcpyAsync(j->d_acc, j->spec.acc, (size_t)j->nbp * (j->N / 8), hipMemcpyDeviceToDevice, s));
+  else
+    DVC_HIP(hipMemcpyAsync(j->d_acc, j->acc.data(), j->acc.size(), hipMemcpyHostToDevice, s));
   DVC_HIP(hipMemsetAsync(f.reconsTrans, 0, sizeof(int) * j->W * j->H, s));
   // per-bitplane rate-search state: start level from the entropy (computed on
   // the device), no accepted code yet, and no record of any code run
@@ -458,7 +462,18 @@ static int admit(dvc_engine *e, Job *j, hipStream_t s) {
 
 static int finish_job(dvc_engine *e, Job &j, hipStream_t s) {
   dvc_ctx *c = e->c;
-  DVC_HIP(hipMemcpyAsync(j.spec.recons, j.fd.recons, j.W * j.H, hipMemcpyDeviceToHost, s));
+  const size_t FS = (size_t)j.W * j.H;
+  std::vector<uint8_t> hostRec;  // device mode: the frame on the host for the PSNR
+  if (e->deviceBuffers) {
+    DVC_HIP(hipMemcpyAsync(j.spec.recons, j.fd.recons, FS, hipMemcpyDeviceToDevice, s));
+    if (j.spec.orig && j.spec.stats) {
+      hostRec.resize(FS);
+      DVC_HIP(hipMemcpyAsync(hostRec.data(), j.fd.recons, FS, hipMemcpyDeviceToHost, s));
+    }
+  } else {
+    DVC_HIP(hipMemcpyAsync(j.spec.recons, j.fd.recons, FS, hipMemcpyDeviceToHost, s));
+  }
+  const uint8_t *rec8 = e->deviceBuffers ? hostRec.data() : j.spec.recons;
   std::vector<PlaneState> ps(j.nbp);
   std::vector<int4> rec((size_t)65 * j.nbp);
   DVC_HIP(hipMemcpyAsync(ps.data(), j.d_state, sizeof(PlaneState) * j.nbp, hipMemcpyDeviceToHost, s));
@@ -488,9 +503,8 @@ static int finish_job(dvc_engine *e, Job &j, hipStream_t s) {
   st->recon_host_fixups = j.fixups;
   if (j.spec.orig) {
     double mse = 0.0;
-    const int FS = j.W * j.H;
-    for (int p = 0; p < FS; p++) {
-      const double d = (double)j.spec.recons[p] - (double)j.spec.orig[p];
+    for (size_t p = 0; p < FS; p++) {
+      const double d = (double)rec8[p] - (double)j.spec.orig[p];
       mse += d * d;
     }
     mse /= (double)FS;
@@ -757,7 +771,7 @@ static int engine_submit(dvc_engine *e, const dvc_wz_frame *sp, uint64_t pastKey
   j->bands = band_plan(sp->qindex, j->acRange.data(), &j->bitrate0);
   for (auto &b : j->bands) j->nbp += b.qbits;
   j->crc.assign(sp->crc, sp->crc + j->nbp);
-  j->acc.assign(sp->acc, sp->acc + (size_t)j->nbp * (j->N / 8));
+  if (!e->deviceBuffers) j->acc.assign(sp->acc, sp->acc + (size_t)j->nbp * (j->N / 8));
   j->bytes = job_bytes(*j);
   j->pastKey = pastKey ? pastKey : (uint64_t)(uintptr_t)sp->past;
   j->futKey = futKey ? futKey : (uint64_t)(uintptr_t)sp->fut;
@@ -779,6 +793,7 @@ static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **ou
   e->maxInflight = (p && p->max_inflight > 0) ? p->max_inflight : 64;
   e->sequentialBp = p && p->sequential_bp;
   e->graphs = !(p && p->eager_steps);
+  e->deviceBuffers = p && p->device_buffers;
   size_t freeB = 0, totalB = 0;
   (void)hipMemGetInfo(&freeB, &totalB);
   e->budget = (p && p->hbm_budget_bytes) ? (size_t)p->hbm_budget_bytes : (size_t)(0.8 * (double)freeB);

@@ -19,6 +19,7 @@
  */
 #ifndef DVC_ACCEL_H
 #define DVC_ACCEL_H
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -116,6 +117,14 @@ int dvc_ldpca_decode_batch(dvc_ctx *ctx, int n, dvc_ldpca_item *items);
  * 6: the branch-free phi the BP kernel evaluates for x >= 0 (float in, float out). */
 int dvc_selftest_unary(dvc_ctx *ctx, int func, const void *in, void *out, int n);
 
+/* Device buffers for callers without a HIP runtime of their own (no
+ * counterpart in the reference): allocation on the context's device and
+ * blocking copies on its stream.  kind: 0 host->device, 1 device->host,
+ * 2 device->device. */
+int dvc_buffer_alloc(dvc_ctx *ctx, size_t bytes, void **dptr);
+int dvc_buffer_free(dvc_ctx *ctx, void *dptr);
+int dvc_buffer_copy(dvc_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
+
 /* ---------------------------------------------------------------- WZ frames */
 typedef struct {
   int nbitplanes;
@@ -181,8 +190,8 @@ int dvc_side_info_batch(dvc_ctx *ctx, int n, const dvc_si_request *reqs);
  * pointer) and shared by the frames that name it, until released or evicted
  * (least recently used, only when unreferenced).  The request's host buffers
  * (past, fut, orig, recons, stats) must stay valid until its ticket is done;
- * ac_range/crc/acc are copied at submit.  Each frame's result is the same as
- * when decoded alone. */
+ * ac_range/crc/acc are copied at submit (acc at admission when it is a device
+ * buffer).  Each frame's result is the same as when decoded alone. */
 typedef struct dvc_engine dvc_engine;
 typedef struct {
   uint64_t hbm_budget_bytes;  /* 0: 80% of the free device memory */
@@ -192,6 +201,8 @@ typedef struct {
                                  thread (0: one per 16 frames in flight, at most 4) */
   int eager_steps;            /* 1: launch every step's kernels directly; 0: steps in which no frame
                                  starts or ends a band replay a captured HIP graph of their launches */
+  int device_buffers;         /* 1: past, fut, acc and recons of submitted frames are device (HBM)
+                                 pointers; ac_range, crc, orig and stats stay host pointers */
 } dvc_engine_params;
 int dvc_engine_create(dvc_ctx *ctx, const dvc_engine_params *params, dvc_engine **out);
 int dvc_engine_destroy(dvc_engine *eng);

@@ -161,3 +161,31 @@ def test_engine_graph_replay_equals_eager(ctx, lanes):
         assert np.array_equal(rg, re_)
         assert sg["levels"] == se["levels"] and sg["iterations"] == se["iterations"]
         assert sg["bitrate"] == se["bitrate"] and sg["psnr"] == se["psnr"]
+
+
+def test_engine_device_buffers_equal_host(ctx):
+    """Requests whose key frames, syndromes and outputs live in HBM (device
+    pointers, shared key frames under stream keys) decode to the same frames
+    and statistics as host-buffer requests."""
+    from dvc_amd import _lib
+    vid = W.synthetic_video(352, 288, 5, 6, seed=990)
+    frames = [dict(past=vid[k], fut=vid[k + 2], orig=vid[k + 1], wz=W.encode_wz_frame(vid[k + 1], q), idx=(k, k + 1, k + 2))
+              for k, q in ((0, 8), (2, 5))]
+    host = _lib.Engine(ctx, max_inflight=2).decode_all(frames)
+    keys = {f: _lib.DeviceBuffer.from_array(ctx, vid[f]) for f in (0, 2, 4)}
+    accs = [_lib.DeviceBuffer.from_array(ctx, np.asarray(fr["wz"].acc, np.uint8)) for fr in frames]
+    recs = [_lib.DeviceBuffer(ctx, 288 * 352) for _ in frames]
+    eng = _lib.Engine(ctx, max_inflight=2, device_buffers=True)
+    tickets = [eng.submit_device(352, 288, fr["wz"], keys[fr["idx"][0]].ptr, keys[fr["idx"][2]].ptr,
+                                 acc.ptr, rec.ptr, idx=fr["idx"], past_key=100 + fr["idx"][0],
+                                 fut_key=100 + fr["idx"][2])
+               for fr, acc, rec in zip(frames, accs, recs)]
+    pending = set(tickets)
+    while pending:
+        eng.step()
+        pending = {t for t in pending if not eng.done(t)}
+    for t, rec, (hrec, hst) in zip(tickets, recs, host):
+        _, st = eng.result(t)
+        assert np.array_equal(rec.download((288, 352), np.uint8), hrec)
+        assert st["levels"] == hst["levels"] and st["iterations"] == hst["iterations"] and st["bitrate"] == hst["bitrate"]
+    eng.close()
