@@ -99,6 +99,7 @@ SIGNATURES = {
                                     _c.c_int, _c.c_void_p, _c.POINTER(BpStats)]),
     "dvc_ldpca_decode_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(LdpcaItem)]),
     "dvc_selftest_unary": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_int]),
+    "dvc_selftest_phi_sweep": (_c.c_int, [_c.c_void_p, _c.c_uint32, _c.c_uint32, _c.POINTER(_c.c_ulonglong)]),
     "dvc_decode_wz_frames": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WZFrameSpec), _c.c_int]),
     "dvc_side_info_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(SIRequest)]),
     "dvc_buffer_alloc": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_void_p)]),
@@ -242,6 +243,13 @@ class Context:
         out = np.zeros(x.size, np.float64 if func in (4, 5) else np.float32)
         _check(lib.dvc_selftest_unary(self.h, func, _ptr(x), _ptr(out), x.size), "dvc_selftest_unary")
         return out
+
+    def phi_sweep(self, lo, hi):
+        """(compared, mismatches, lowest mismatching pattern) of the BP
+        kernel's phi against the direct restatement over [lo, hi]."""
+        c = (ctypes.c_ulonglong * 3)()
+        _check(lib.dvc_selftest_phi_sweep(self.h, lo, hi, c), "dvc_selftest_phi_sweep")
+        return int(c[0]), int(c[1]), int(c[2])
 
     def decode_wz_frames(self, frames, batched=True):
         """frames: list of dicts with past, fut (uint8 HxW), orig (optional),

@@ -9,7 +9,7 @@
 //   * phi(x) = -logf(tanhf((x + 1e-9f)/2)) with the glibc-identical routines
 //     of dvc_math.h, in their branch-free form for x >= 0 (every argument
 //     here is a magnitude: a row's sum of |phi| is at least each of its
-//     terms, so |sum| - |phi_e| >= 0);
+//     terms, so |sum| - |phi_e| >= 0), dvc_bp_phi_fast;
 //   * check-node sums accumulate in ascending edge order (= ascending column),
 //     the order of `rowTotal[ir[k]] += ...` (:514-515);
 //   * variable-node sums accumulate LLR + c2v0 + c2v1 + c2v2 (:527-539);
@@ -25,7 +25,9 @@
 // column order), so the variable-node update, the next check-node input and
 // the decision are computed in registers without further exchange.  Two
 // barriers per iteration: after the check (row) pass and after the variable
-// (column) pass.
+// (column) pass.  The syndrome test's decision parities are XORed into
+// per-check bits by the variable pass (LDS atomics, double-buffered across
+// iterations), so the check pass reads no column data besides phi.
 //
 // Batched form.  A persistent grid walks a queue of (bitplane, code offset)
 // items in offset-major order.  An item whose code lies above the lowest code
@@ -108,6 +110,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
   __shared__ int s_item, s_cancel;
   __shared__ uint32_t s_crc[T / 64];
   __shared__ double s_logtab[32];  // logf's (invc, logc) table, read with per-lane indices
+  __shared__ uint32_t parBits[2][S::SYN_WORDS];  // decision parity per check, written by alternate iterations
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const LadderDev L = a.lad;
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         float v = pd.LLR[k];
         uint32_t s = f2u(v) >> 31;
         float mag = (s ? -1.0f : 1.0f) * v;
-        uint32_t f = (f2u(dvc_bp_phi_nonneg(mag, s_logtab)) & 0x7fffffffu) | (s << 31);
+        uint32_t f = (f2u(dvc_bp_phi_fast(mag, s_logtab)) & 0x7fffffffu) | (s << 31);
         phi[3 * k] = u2f(f); phi[3 * k + 1] = u2f(f); phi[3 * k + 2] = u2f(f);
       }
     }
@@ -176,23 +179,25 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
     for (int t = 0;; t++) {
       // ---- check-node pass: |phi| sums and sign products; parity of the
       // previous decisions (:495-515, :551-568)
+      // The previous variable pass XORed each column's decision into its
+      // three checks' bits of parBits[(t - 1) & 1]; this pass reads them and
+      // clears parBits[t & 1] for the coming variable pass.
       int fail = 0;
+      const uint32_t *parPrev = parBits[(t + 1) & 1];
       for (int r = tid; r < m; r += T) {
         const int e0 = rowPtr[r], e1 = rowPtr[r + 1];
-        uint32_t syn = (synBits[r >> 5] >> (r & 31)) & 1u;
-        uint32_t sgn = syn, par = syn;
+        const uint32_t syn = (synBits[r >> 5] >> (r & 31)) & 1u;
+        uint32_t sgn = syn;
         float sum = 0.0f;
         for (int j = e0; j < e1; j++) {
-          const int e = rowEdge[j];
-          const uint32_t f = f2u(phi[e]);
+          const uint32_t f = f2u(phi[rowEdge[j]]);
           sum += u2f(f & 0x7fffffffu);
           sgn ^= f >> 31;
-          const int c = e / 3;
-          par ^= (decBits[c >> 5] >> (c & 31)) & 1u;
         }
         rowTot[r] = u2f(f2u(sum) | (sgn << 31));
-        if (t > 0 && par) fail = 1;
+        if (t > 0 && (syn ^ ((parPrev[r >> 5] >> (r & 31)) & 1u))) fail = 1;
       }
+      for (int wd = tid; wd < S::SYN_WORDS; wd += T) parBits[t & 1][wd] = 0u;
       if (tid == 0)
         s_cancel = __hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < level;
       const int anyFail = __syncthreads_or(fail);
@@ -207,13 +212,15 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         if (k < N) {
           float c2v[3];
           uint32_t ph[3];
+          int row[3];
 #pragma unroll
           for (int q = 0; q < 3; q++) {
             const int e = 3 * k + q;
-            const uint32_t R = f2u(rowTot[gir[e]]);
+            row[q] = gir[e];
+            const uint32_t R = f2u(rowTot[row[q]]);
             ph[q] = f2u(phi[e]);
             const float x = u2f(R & 0x7fffffffu) - u2f(ph[q] & 0x7fffffffu);
-            const float v = dvc_bp_phi_nonneg(x, s_logtab);
+            const float v = dvc_bp_phi_fast(x, s_logtab);
             c2v[q] = ((R ^ ph[q]) >> 31) ? v * -1.0f : v * 1.0f;
           }
           float tmp = pd.LLR[k];
@@ -221,12 +228,16 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
           tmp += c2v[1];
           tmp += c2v[2];
           dec = f2u(tmp) >> 31;
+          if (dec) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) atomicXor(&parBits[t & 1][row[q] >> 5], 1u << (row[q] & 31));
+          }
 #pragma unroll
           for (int q = 0; q < 3; q++) {
             const float v2c = tmp - c2v[q];
             const uint32_t s = f2u(v2c) >> 31;
             const float mag = (s ? -1.0f : 1.0f) * v2c;
-            phi[3 * k + q] = u2f((f2u(dvc_bp_phi_nonneg(mag, s_logtab)) & 0x7fffffffu) | (s << 31));
+            phi[3 * k + q] = u2f((f2u(dvc_bp_phi_fast(mag, s_logtab)) & 0x7fffffffu) | (s << 31));
           }
         }
         const uint64_t bal = __ballot(dec);

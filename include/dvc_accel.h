@@ -114,8 +114,15 @@ int dvc_ldpca_decode_batch(dvc_ctx *ctx, int n, dvc_ldpca_item *items);
  * device's scalar routines on n inputs.  func 0..3: float logf, expf, tanhf,
  * the BP kernel phi(x) = -logf(tanhf((x+1e-9f)/2)) (float in, float out);
  * 4: double sqrt of int32 inputs; 5: double exp (double in, double out);
- * 6: the branch-free phi the BP kernel evaluates for x >= 0 (float in, float out). */
+ * 6: the branch-free phi for x >= 0 (float in, float out); 7: its shortened
+ * form, which the BP kernel evaluates (float in, float out). */
 int dvc_selftest_unary(dvc_ctx *ctx, int func, const void *in, void *out, int n);
+/* Verification entry (no counterpart in the reference): evaluates the BP
+ * kernel's phi and the direct restatement of the reference's
+ * -logf(tanhf((x+1e-9f)/2)) on every float bit pattern in [lo, hi] (NaNs
+ * skipped).  counts[0] = patterns compared, counts[1] = mismatches,
+ * counts[2] = lowest mismatching pattern (0xffffffff when none). */
+int dvc_selftest_phi_sweep(dvc_ctx *ctx, uint32_t lo, uint32_t hi, unsigned long long *counts);
 
 /* Device buffers for callers without a HIP runtime of their own (no
  * counterpart in the reference): allocation on the context's device and

@@ -1,11 +1,13 @@
 /* Compares the single-precision routines of dvc_amd/csrc/dvc_math.h with the
  * host C library over a range of float bit patterns.
  *   math_check <func> <first_bits_hex> <last_bits_hex> <stride>
- * func: logf | expf | tanhf | expm1f | phi | phi_nonneg
+ * func: logf | expf | tanhf | expm1f | phi | phi_nonneg | phi_fast
  * Prints "<func> checked=<n> mismatches=<m>" and the first few mismatches.
  * phi is -logf(tanhf((x + (float)1e-9)/2)), the check-node kernel of
  * DvcDec/src/LdpcaDecoder.cpp:513.  phi_nonneg is the branch-free form the BP
- * kernel uses for x >= 0, compared with the same C library value. */
+ * kernel uses for x >= 0, compared with the same C library value; phi_fast is
+ * its shortened form (on the host its quotients are plain IEEE divisions; the
+ * device form is compared with phi_nonneg on the GPU). */
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -20,7 +22,7 @@ int main(int argc, char **argv) {
   unsigned long long stride = strtoull(argv[4], 0, 10);
   if (stride == 0) stride = 1;
   int which = !strcmp(fn, "logf") ? 0 : !strcmp(fn, "expf") ? 1 : !strcmp(fn, "tanhf") ? 2
-            : !strcmp(fn, "expm1f") ? 3 : !strcmp(fn, "phi") ? 4 : !strcmp(fn, "phi_nonneg") ? 5 : -1;
+            : !strcmp(fn, "expm1f") ? 3 : !strcmp(fn, "phi") ? 4 : !strcmp(fn, "phi_nonneg") ? 5 : !strcmp(fn, "phi_fast") ? 6 : -1;
   if (which < 0) { fprintf(stderr, "unknown func %s\n", fn); return 2; }
   double logtab[32];
   for (int i = 0; i < 16; i++) { logtab[i] = dvcm_logf_invc[i]; logtab[16 + i] = dvcm_logf_logc[i]; }
@@ -36,7 +38,8 @@ int main(int argc, char **argv) {
       case 2: g = tanhf(x); m = dvc_tanhf(x); break;
       case 3: g = expm1f(x); m = dvc_expm1f(x); break;
       case 4: g = ref_phi(x); m = dvc_bp_phi(x); break;
-      default: g = ref_phi(x); m = dvc_bp_phi_nonneg(x, logtab); break;
+      case 5: g = ref_phi(x); m = dvc_bp_phi_nonneg(x, logtab); break;
+      default: g = ref_phi(x); m = dvc_bp_phi_fast(x, logtab); break;
     }
     checked++;
     if (dvcm_asu(g) != dvcm_asu(m) && !(g != g && m != m)) {

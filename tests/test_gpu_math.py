@@ -34,10 +34,10 @@ def test_float_routines_match_host(ctx, func, name, lo, hi):
     assert np.array_equal(dev.view(np.uint32), ref.view(np.uint32))
 
 
-@pytest.mark.parametrize("func", [3, 6])
+@pytest.mark.parametrize("func", [3, 6, 7])
 def test_bp_phi_full_grid_of_messages(ctx, func):
     # every float in [0, 64) on a stride, plus the small magnitudes bit by bit;
-    # func 3 is the branching form, 6 the branch-free one the BP kernel runs
+    # func 3 is the branching form, 6 the branch-free one, 7 its shortened form (the BP kernel's)
     b = np.concatenate([np.arange(0, 0x42800000, 4099, dtype=np.uint32), np.arange(0, 1 << 20, dtype=np.uint32),
                         np.arange(0x42800000, 0x7f800001, 65537, dtype=np.uint32)])
     x = b.view(np.float32)
@@ -65,3 +65,13 @@ def test_double_exp_within_margin(ctx):
     ref = np.exp(x)
     rel = np.abs(dev - ref) / ref
     assert rel.max() < 1e-15
+
+
+def test_bp_phi_kernel_form_equals_restatement_on_every_argument(ctx):
+    """The BP kernel's phi (reciprocal-based quotients) against the direct
+    restatement of the reference's expression on the device, for every
+    non-negative float and -0: zero mismatches."""
+    n, bad, first = ctx.phi_sweep(0x00000000, 0x7f800000)
+    assert n == 0x7f800001 and bad == 0, hex(first)
+    n, bad, first = ctx.phi_sweep(0x80000000, 0x80000000)
+    assert n == 1 and bad == 0
