@@ -8,6 +8,7 @@
 // are bit-identical to the CPU path.  Work is laid out per frame of a batch:
 // blockIdx.y (or .z) selects the frame, the other grid dimensions its blocks.
 #include <float.h>
+#include <algorithm>
 #include "dvc_math.h"
 #include "frame_kernels.h"
 #include "ldpca_kernels.h"
@@ -495,22 +496,21 @@ __global__ void k_smf_init(const FrameDev *F, const int *list) {
 // values, bit for bit) and the mixture sums then read the table.  One
 // workgroup per (4x4 block, frame).
 #define ND_TAB 6144  // table entries (24 KB of LDS); larger distances call expf directly
-__global__ __launch_bounds__(256) void k_noise_dist(const FrameDev *F, const int *list) {
+__global__ __launch_bounds__(512) void k_noise_dist(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int b = blockIdx.x;
   if (b >= f.N) return;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, T = blockDim.x;
   const int ow = f.W - 3, index = f.index, range = f.range;
   const SmfWin s = smf_window(f, b);
   const float a = f.alpha[(s.top + (index >> 2)) * f.W + s.left + (index & 3)];
   float *out = f.dist + (size_t)b * f.distStride;
   if (index == 0) {
     const int side = f.oct[(size_t)16 * (s.top * ow + s.left)];
-    for (int k = tid; k < range; k += 256) out[k] = dvc_expf(-a * (float)abs(k - side));
+    for (int k = tid; k < range; k += T) out[k] = dvc_expf(-a * (float)abs(k - side));
     return;
   }
-  __shared__ float s_p[81];
-  __shared__ int s_side[81];
+  __shared__ int2 s_ps[81];  // (probability bits, SI coefficient) of the kept window terms
   __shared__ int s_n, s_maxabs;
   __shared__ float s_tab[ND_TAB];
   if (tid < 64) {
@@ -530,8 +530,7 @@ __global__ __launch_bounds__(256) void k_noise_dist(const FrameDev *F, const int
         const int pos = n + __popcll(mask & ((1ull << tid) - 1ull));
         const int m = s.sT + p / s.w, q = s.sL + p % s.w;
         const int sd = f.oct[(size_t)16 * (m * ow + q) + index];
-        s_p[pos] = pr;
-        s_side[pos] = sd;
+        s_ps[pos] = make_int2(__float_as_int(pr), sd);
         mx = imax(mx, abs(sd));
       }
       n += __popcll(mask);
@@ -542,57 +541,110 @@ __global__ __launch_bounds__(256) void k_noise_dist(const FrameDev *F, const int
   __syncthreads();
   const int n = s_n;
   // |k - range - SI_m| <= range + max|SI_m| for k in [0, 2*range]
-  const int tl = imin(range + s_maxabs + 1, ND_TAB);
-  for (int d = tid; d < tl; d += 256) s_tab[d] = dvc_expf(-a * (float)d);
+  const int need = range + s_maxabs + 1;
+  const int tl = imin(need, ND_TAB);
+  for (int d = tid; d < tl; d += T) s_tab[d] = dvc_expf(-a * (float)d);
   __syncthreads();
   const int rmax = 2 * range + 1;
-  for (int k = tid; k < rmax; k += 256) {
-    const int pc = k - range;
-    float sum = 0.0f;
-    for (int m = 0; m < n; m++) {
-      const int d = abs(pc - s_side[m]);
-      const float e = d < tl ? s_tab[d] : dvc_expf(-a * (float)d);
-      sum += s_p[m] * e;
+  if (need <= ND_TAB) {
+    for (int k = tid; k < rmax; k += T) {
+      const int pc = k - range;
+      float sum = 0.0f;
+      for (int m = 0; m < n; m++) {
+        const int2 ps = s_ps[m];
+        sum += __int_as_float(ps.x) * s_tab[abs(pc - ps.y)];
+      }
+      out[k] = sum;
     }
-    out[k] = sum;
+  } else {
+    for (int k = tid; k < rmax; k += T) {
+      const int pc = k - range;
+      float sum = 0.0f;
+      for (int m = 0; m < n; m++) {
+        const int2 ps = s_ps[m];
+        const int d = abs(pc - ps.y);
+        const float e = d < tl ? s_tab[d] : dvc_expf(-a * (float)d);
+        sum += __int_as_float(ps.x) * e;
+      }
+      out[k] = sum;
+    }
   }
 }
 
 // ------------------------------------------------------------ conditional bit probability
 // condBitProb (noiseModel.cpp:191-310): interval sums in ascending k, p0/(p0+p1),
-// clipped likelihood ratio.  One thread per block.
-__global__ void k_condbit(const FrameDev *F, const int *list) {
+// clipped likelihood ratio.  One wave per 64 blocks: the wave stages the span
+// of each block's two intervals through LDS in chunks of 32 values (each load
+// a coalesced 128-byte piece of one block's distribution row), and each lane
+// then adds its own block's values in ascending k, as the reference's loops do.
+__global__ __launch_bounds__(64) void k_condbit(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= f.N) return;
-  const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4;
-  const int index = f.index, bit = f.bit, step = f.step, range = f.range;
-  const int iq = (1 << (bit - 1)) - 1;
-  const int rc = f.reconsTrans[(top + (index >> 2)) * f.W + left + (index & 3)];
-  const float *prob = f.dist + (size_t)b * f.distStride;
-  int v0l, v0u, v1l, v1u, lim;
-  if (index == 0) {
-    const int q0l = rc << bit, q1l = ((rc << 1) + 1) << (bit - 1);
-    const int q0u = q0l + iq, q1u = q1l + iq;
-    v0l = q0l * step; v1l = q1l * step; v0u = (q0u + 1) * step; v1u = (q1u + 1) * step;
-    lim = range;
-  } else {
-    const int ls = f.levelShift;
-    const int q0l = (rc << bit) - ls, q1l = (((rc << 1) + 1) << (bit - 1)) - ls;
-    const int q0u = q0l + iq;
-    int q1u = q1l + iq;
-    if (q1u > ls) q1u = ls;
-    v0l = (q0l <= 0) ? (q0l - 1) * step + 1 : q0l * step;
-    v1l = (q1l <= 0) ? (q1l - 1) * step + 1 : q1l * step;
-    v0u = (q0u < 0) ? q0u * step + 1 : (q0u + 1) * step;
-    v1u = (q1u < 0) ? q1u * step + 1 : (q1u + 1) * step;
-    v0l += range; v0u += range; v1l += range; v1u += range;
-    lim = 2 * range + 1;
+  const int lane = threadIdx.x;
+  const int b0 = blockIdx.x * 64;
+  if (b0 >= f.N) return;  // uniform over the wave
+  __shared__ float s_v[64 * 33];
+  __shared__ int s_lo[64], s_hi[64];
+  const int b = b0 + lane;
+  const bool valid = b < f.N;
+  int v0l = 0, v0u = 0, v1l = 0, v1u = 0;
+  if (valid) {
+    const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4;
+    const int index = f.index, bit = f.bit, step = f.step, range = f.range;
+    const int iq = (1 << (bit - 1)) - 1;
+    const int rc = f.reconsTrans[(top + (index >> 2)) * f.W + left + (index & 3)];
+    int lim;
+    if (index == 0) {
+      const int q0l = rc << bit, q1l = ((rc << 1) + 1) << (bit - 1);
+      const int q0u = q0l + iq, q1u = q1l + iq;
+      v0l = q0l * step; v1l = q1l * step; v0u = (q0u + 1) * step; v1u = (q1u + 1) * step;
+      lim = range;
+    } else {
+      const int ls = f.levelShift;
+      const int q0l = (rc << bit) - ls, q1l = (((rc << 1) + 1) << (bit - 1)) - ls;
+      const int q0u = q0l + iq;
+      int q1u = q1l + iq;
+      if (q1u > ls) q1u = ls;
+      v0l = (q0l <= 0) ? (q0l - 1) * step + 1 : q0l * step;
+      v1l = (q1l <= 0) ? (q1l - 1) * step + 1 : q1l * step;
+      v0u = (q0u < 0) ? q0u * step + 1 : (q0u + 1) * step;
+      v1u = (q1u < 0) ? q1u * step + 1 : (q1u + 1) * step;
+      v0l += range; v0u += range; v1l += range; v1u += range;
+      lim = 2 * range + 1;
+    }
+    v0l = imax(v0l, 0); v1l = imax(v1l, 0); v0u = imin(v0u, lim); v1u = imin(v1u, lim);
   }
-  v0l = imax(v0l, 0); v1l = imax(v1l, 0); v0u = imin(v0u, lim); v1u = imin(v1u, lim);
+  // the span read for this block: [lo, hi) covers both (non-empty) intervals
+  int lo = 0, hi = 0;
+  if (v0u > v0l) { lo = v0l; hi = v0u; }
+  if (v1u > v1l) {
+    lo = hi > lo ? imin(lo, v1l) : v1l;
+    hi = imax(hi, v1u);
+  }
+  s_lo[lane] = lo;
+  s_hi[lane] = hi;
+  int span = hi - lo;
+  for (int o = 32; o > 0; o >>= 1) span = imax(span, __shfl_xor(span, o));
+  __syncthreads();
+  const size_t stride = f.distStride;
   float p0 = 0.0f, p1 = 0.0f;
-  for (int k = v0l; k < v0u; k++) p0 += prob[k];
-  for (int k = v1l; k < v1u; k++) p1 += prob[k];
+  for (int c0 = 0; c0 < span; c0 += 32) {
+    for (int jj = 0; jj < 64; jj += 2) {
+      const int j = jj + (lane >> 5), i = lane & 31;
+      const int k = s_lo[j] + c0 + i;
+      float v = 0.0f;
+      if (b0 + j < f.N && k < s_hi[j]) v = f.dist[(size_t)(b0 + j) * stride + k];
+      s_v[j * 33 + i] = v;
+    }
+    __syncthreads();
+    for (int i = 0; i < 32; i++) {
+      const int k = lo + c0 + i;
+      const float v = s_v[lane * 33 + i];
+      if (k >= v0l && k < v0u) p0 += v;
+      if (k >= v1l && k < v1u) p1 += v;
+    }
+    __syncthreads();
+  }
+  if (!valid) return;
   p0 = p0 / (p0 + p1);
   p1 = 1 - p0;
   float lr;
@@ -908,11 +960,13 @@ hipError_t launch_oct(const FrameDev *F, const int *list, int n, int maxPos, int
   return hipGetLastError();
 }
 hipError_t launch_noise_dist(const FrameDev *F, const int *list, int n, int maxN, int maxStride, hipStream_t s) {
-  hipLaunchKernelGGL(k_noise_dist, dim3(maxN, n), dim3(256), 0, s, F, list);
+  // a thread per distribution value where that fits (64..512 threads)
+  const int threads = std::min(512, std::max(64, (maxStride + 63) / 64 * 64));
+  hipLaunchKernelGGL(k_noise_dist, dim3(maxN, n), dim3(threads), 0, s, F, list);
   return hipGetLastError();
 }
 hipError_t launch_condbit(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s) {
-  hipLaunchKernelGGL(k_condbit, dim3((maxN + 127) / 128, n), dim3(128), 0, s, F, list);
+  hipLaunchKernelGGL(k_condbit, dim3((maxN + 63) / 64, n), dim3(64), 0, s, F, list);
   return hipGetLastError();
 }
 hipError_t launch_put_bitplane(const FrameDev *F, const int *list, const BpFrameLink *links, int n, int maxN,
