@@ -179,6 +179,8 @@ def bench_wz(args):
     jobs = make_wz_inputs(args.streams)
     log(f"[bench] encoded {len(jobs)} CIF Q8 WZ frames in {time.time() - t0:.1f}s")
     ctx = _lib.Context(device=0, max_iter=args.max_iter, bp_threads=args.bp_threads)
+    tuning = ctx.autotune() if args.autotune else ctx.tuning()
+    log(f"[bench] BP tuning {tuning}")
     dev = DeviceInputs(ctx, jobs)
     for i in range(args.warmup):
         eng = _lib.Engine(ctx, max_inflight=len(jobs), lanes=args.lanes, device_buffers=True)
@@ -241,7 +243,7 @@ def bench_wz(args):
         "psnr_mean": float(np.mean(psnrs)),
         "plain_vs_batched_mismatches": plain_mismatch,
         "inputs": "resident in HBM before the timed region (key frames, syndromes, output frames)",
-        "kernels_ms_per_step": kern, "dominant_kernel": dom[0],
+        "kernels_ms_per_step": kern, "dominant_kernel": dom[0], "bp_tuning": tuning,
     }
     res["roofline"] = roofline_wz(dom, kern, args, len(jobs), outs)
     if not args.no_cpu:
@@ -460,6 +462,8 @@ def main():
     ap.add_argument("--max-iter", type=int, default=100)
     ap.add_argument("--bp-threads", type=int, default=0, help="threads per CIF BP workgroup (0: library default)")
     ap.add_argument("--lanes", type=int, default=2, help="engine lanes (0: library default)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the library's default BP launch configuration instead of dvc_autotune's choice")
     ap.add_argument("--rates", default="50,200,800", help="serve: Poisson arrival rates (requests/s)")
     ap.add_argument("--serve-frames", type=int, default=17, help="serve: frames per stream")
     ap.add_argument("--serve-inflight", type=int, default=128, help="serve: engine frames in flight")

@@ -72,6 +72,14 @@ class WZFrameSpec(ctypes.Structure):
                 ("recons", ctypes.c_void_p), ("stats", ctypes.POINTER(FrameStats))]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("cif_threads", ctypes.c_int), ("qcif_threads", ctypes.c_int), ("qcif_wg_per_cu", ctypes.c_int),
+                ("poll", ctypes.c_int), ("cif_ms", ctypes.c_double), ("qcif_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class SIRequest(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("qindex", ctypes.c_int),
                 ("past_idx", ctypes.c_int), ("cur_idx", ctypes.c_int), ("fut_idx", ctypes.c_int),
@@ -99,6 +107,9 @@ SIGNATURES = {
                                     _c.c_int, _c.c_void_p, _c.POINTER(BpStats)]),
     "dvc_ldpca_decode_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(LdpcaItem)]),
     "dvc_selftest_unary": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_int]),
+    "dvc_autotune": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(Tuning)]),
+    "dvc_get_tuning": (_c.c_int, [_c.c_void_p, _c.POINTER(Tuning)]),
+    "dvc_set_tuning": (_c.c_int, [_c.c_void_p, _c.POINTER(Tuning)]),
     "dvc_selftest_phi_sweep": (_c.c_int, [_c.c_void_p, _c.c_uint32, _c.c_uint32, _c.POINTER(_c.c_ulonglong)]),
     "dvc_decode_wz_frames": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WZFrameSpec), _c.c_int]),
     "dvc_side_info_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(SIRequest)]),
@@ -243,6 +254,24 @@ class Context:
         out = np.zeros(x.size, np.float64 if func in (4, 5) else np.float32)
         _check(lib.dvc_selftest_unary(self.h, func, _ptr(x), _ptr(out), x.size), "dvc_selftest_unary")
         return out
+
+    def autotune(self, planes=0):
+        """Times the BP launch variants and keeps the fastest (dvc_autotune)."""
+        t = Tuning()
+        _check(lib.dvc_autotune(self.h, int(planes), ctypes.byref(t)), "dvc_autotune")
+        return t.as_dict()
+
+    def tuning(self):
+        t = Tuning()
+        _check(lib.dvc_get_tuning(self.h, ctypes.byref(t)), "dvc_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        _check(lib.dvc_get_tuning(self.h, ctypes.byref(t)), "dvc_get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(lib.dvc_set_tuning(self.h, ctypes.byref(t)), "dvc_set_tuning")
 
     def phi_sweep(self, lo, hi):
         """(compared, mismatches, lowest mismatching pattern) of the BP

@@ -37,19 +37,12 @@ struct DevBuf {
   }
 };
 
-static int bp_threads_for(const dvc_ctx *ctx, int N) {
-  if (N == 1584) return 256;
-  return ctx->bp_threads_cif > 0 ? ctx->bp_threads_cif : 1024;
-}
+int bp_threads(const dvc_ctx *c, int N) { return N == 1584 ? c->bp_threads_qcif : c->bp_threads_cif; }
 
-static int persistent_grid(int N, int threads) {
-  hipDeviceProp_t prop;
-  int cus = 256;
-  int dev = 0;
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) cus = prop.multiProcessorCount;
-  // workgroups resident per CU: limited by LDS (CIF codewords use ~150 KB)
-  int per_cu = N == 6336 ? 1 : (threads <= 256 ? 4 : 2);
-  return cus * per_cu;
+int bp_grid(const dvc_ctx *c, int N, int items) {
+  // a CIF codeword fills a CU's LDS (~155 KB); QCIF codewords (~39 KB) share one
+  const int perCU = N == 6336 ? 1 : c->bp_wg_per_cu_qcif;
+  return std::max(1, std::min(c->sm_count * perCU, items));
 }
 
 }  // namespace dvc
@@ -125,7 +118,7 @@ int dvc_ctx_create(const dvc_params *params, dvc_ctx **out) {
   dvc_ctx *c = new dvc_ctx();
   c->device = p.device;
   c->max_iter = p.max_iter > 0 ? p.max_iter : 50;
-  c->bp_threads_cif = p.bp_threads;
+  if (p.bp_threads > 0) c->bp_threads_cif = p.bp_threads;
   c->sm_count = prop.multiProcessorCount;
   std::string dir = p.data_dir ? std::string(p.data_dir) : default_data_dir();
   int rc;
@@ -241,7 +234,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
   DVC_HIP(hipMemcpyAsync(state.data(), dState.p, sizeof(PlaneState) * n, hipMemcpyDeviceToHost, s));
   DVC_HIP(hipStreamSynchronize(s));
 
-  const int threads = bp_threads_for(c, N);
+  const int threads = bp_threads(c, N);
   LadderDev ld = ladder_dev(*L);
   std::vector<int4> rec((size_t)n * C);
   if (sequential) {
@@ -253,7 +246,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
         int zero[2] = {0, 0};
         DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
         DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
-        BpArgs a{ld, dPlanes.p + i, 1, dCounters.p, 1, 1, c->max_iter, dSlots.p, dCounters.p + 1, slotCap};
+        BpArgs a{ld, dPlanes.p + i, 1, dCounters.p, 1, 1, c->max_iter, dSlots.p, dCounters.p + 1, slotCap, c->bp_poll};
         TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, 1, s));
         int4 r;
         DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + level, sizeof(r), hipMemcpyDeviceToHost, s));
@@ -280,8 +273,9 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
     int zero[2] = {0, 0};
     DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
     const int maxOff = C - 1;
-    BpArgs a{ld, dPlanes.p, n, dCounters.p, n * maxOff, maxOff, c->max_iter, dSlots.p, dCounters.p + 1, slotCap};
-    int grid = std::min(persistent_grid(N, threads), std::max(1, n * maxOff));
+    BpArgs a{ld, dPlanes.p, n, dCounters.p, n * maxOff, maxOff, c->max_iter, dSlots.p, dCounters.p + 1, slotCap,
+             c->bp_poll};
+    const int grid = bp_grid(c, N, n * maxOff);
     TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
     TIMED(c, "invh", s, launch_invh(dPlanes.p, n, ld, dSlots.p, dCounters.p + 1, slotCap, s));
     int counters[2];

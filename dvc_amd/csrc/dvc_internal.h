@@ -51,11 +51,16 @@ struct Ladder {
 };
 
 int load_ladder(const std::string &dir, int N, Ladder &L);
+// BP launch shape of the context's tuning: threads per workgroup and the
+// persistent grid for `items` queued (bitplane, code) items.
+int bp_threads(const dvc_ctx *c, int N);
+int bp_grid(const dvc_ctx *c, int N, int items);
 int upload_ladder(Ladder &L);
 void free_ladder(Ladder &L);
 
 }  // namespace dvc
 
+struct dvc_ctx;
 // Per-kernel timing with HIP events recorded on the launching stream.
 struct KernelTimer {
   std::string name;
@@ -77,7 +82,13 @@ struct dvc_ctx {
   int sm_count = 256;
   double *d_costFme = nullptr, *d_costUsi = nullptr;  // motion-cost factor tables
   int max_iter = 50;
-  int bp_threads_cif = 0;
+  // BP launch configuration (dvc_autotune / dvc_set_tuning): threads per
+  // workgroup for each code length, QCIF workgroups per CU, and the interval
+  // in iterations at which a running speculative item polls for cancellation
+  int bp_threads_cif = 1024;
+  int bp_threads_qcif = 256;
+  int bp_wg_per_cu_qcif = 4;
+  int bp_poll = 1;
   dvc::Ladder lad1584, lad6336;
   hipStream_t stream = nullptr;
   void *si_scratch = nullptr;  // dvc_side_info_batch's device buffers, kept between calls

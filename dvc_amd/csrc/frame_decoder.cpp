@@ -248,12 +248,12 @@ int dvc::bp_step_launch(dvc_ctx *c, int N, int n, uint8_t *d_slots, int slotCap,
   Ladder *L = c->ladder(N);
   const int C = L->numCodes;
   LadderDev ld = ladder_dev(*L);
-  const int threads = N == 1584 ? 256 : (c->bp_threads_cif > 0 ? c->bp_threads_cif : 1024);
+  const int threads = bp_threads(c, N);
   const int maxOff = C - 1;
   TIMED_LAUNCH(c, "ce_llr", s, launch_ce_llr(d_planes, n, 1, c->lad1584, c->lad6336, s));
   DVC_HIP(launch_zero_ints(d_counters, 2, s));
-  BpArgs a{ld, d_planes, n, d_counters, n * maxOff, maxOff, c->max_iter, d_slots, d_counters + 1, slotCap};
-  const int grid = std::min(c->sm_count * (N == 6336 ? 1 : 4), n * maxOff);
+  BpArgs a{ld, d_planes, n, d_counters, n * maxOff, maxOff, c->max_iter, d_slots, d_counters + 1, slotCap, c->bp_poll};
+  const int grid = bp_grid(c, N, n * maxOff);
   TIMED_LAUNCH(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
   TIMED_LAUNCH(c, "invh", s, launch_invh(d_planes, n, ld, d_slots, d_counters + 1, slotCap, s));
   return DVC_OK;
@@ -265,7 +265,7 @@ int dvc::bp_step(dvc_ctx *c, int N, const std::vector<PlaneDesc> &planes, bool s
   const int n = (int)planes.size(), C = L->numCodes;
   if (n == 0) return DVC_OK;
   LadderDev ld = ladder_dev(*L);
-  const int threads = N == 1584 ? 256 : (c->bp_threads_cif > 0 ? c->bp_threads_cif : 1024);
+  const int threads = bp_threads(c, N);
   DVC_HIP(hipMemcpyAsync(d_planes, planes.data(), sizeof(PlaneDesc) * n, hipMemcpyHostToDevice, s));
   if (!sequential) return bp_step_launch(c, N, n, d_slots, slotCap, d_counters, d_planes, s);
   TIMED_LAUNCH(c, "ce_llr", s, launch_ce_llr(d_planes, n, 1, c->lad1584, c->lad6336, s));
@@ -286,7 +286,7 @@ int dvc::bp_step(dvc_ctx *c, int N, const std::vector<PlaneDesc> &planes, bool s
       ps = PlaneState{level, C - 1, st[i].entropy, 0};
       DVC_HIP(hipMemcpyAsync(planes[i].state, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
       DVC_HIP(hipMemsetAsync(d_counters, 0, sizeof(int), s));
-      BpArgs a{ld, d_planes + i, 1, d_counters, 1, 1, c->max_iter, d_slots, d_counters + 1, slotCap};
+      BpArgs a{ld, d_planes + i, 1, d_counters, 1, 1, c->max_iter, d_slots, d_counters + 1, slotCap, c->bp_poll};
       TIMED_LAUNCH(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, 1, s));
       DVC_HIP(hipMemcpyAsync(&r, planes[i].rec + level, sizeof(r), hipMemcpyDeviceToHost, s));
       DVC_HIP(hipStreamSynchronize(s));

@@ -199,7 +199,8 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
       }
       for (int wd = tid; wd < S::SYN_WORDS; wd += T) parBits[t & 1][wd] = 0u;
       if (tid == 0)
-        s_cancel = __hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < level;
+        s_cancel = (t % a.poll == 0) &&
+                   __hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < level;
       const int anyFail = __syncthreads_or(fail);
       if (t > 0 && !anyFail) { status = 1; iters = t; break; }
       if (s_cancel) { status = 3; iters = t; break; }

@@ -52,6 +52,7 @@ struct BpArgs {
   uint8_t *slots;     // accepted decisions, N bytes per slot
   int *slot_count;
   int slot_capacity;
+  int poll;           // iterations between polls of the bitplane's best code (>= 1)
 };
 
 // record.y status values

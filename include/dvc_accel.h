@@ -40,7 +40,7 @@ typedef struct {
                             (LdpcaDecoder.h:4); <= 0 selects 50 */
   const char *data_dir;  /* directory with reg1584.ladder / reg6336.ladder; NULL = the
                             library's own data directory */
-  int bp_threads;        /* threads per BP workgroup for N=6336 (0 = autotuned choice) */
+  int bp_threads;        /* threads per BP workgroup for N=6336 (0 = default 1024; see dvc_autotune) */
 } dvc_params;
 
 /* InitCUDA + LDPCA_CUDA_Init_Variable (LDPCA_cuda.cu:36-89, 546-763): selects the
@@ -109,6 +109,22 @@ typedef struct {
  * whose BP converges and passes the CRC is kept, which is the code the
  * sequential search selects.  Host pointers. */
 int dvc_ldpca_decode_batch(dvc_ctx *ctx, int n, dvc_ldpca_item *items);
+
+/* BP launch configuration (no counterpart in the reference; the project's
+ * autotuner).  dvc_autotune decodes a synthetic calibration batch of
+ * `planes` bitplanes per code length (0: 128) with each variant, checks that
+ * all variants decode the same bits, keeps the fastest in the context and
+ * returns it.  The choice changes time only, never results. */
+typedef struct {
+  int cif_threads;      /* threads per workgroup for N = 6336: 512, 704 or 1024 */
+  int qcif_threads;     /* threads per workgroup for N = 1584: 256 or 512 */
+  int qcif_wg_per_cu;   /* QCIF codewords (workgroups) per CU in the persistent grid: 1..4 */
+  int poll;             /* BP iterations between a speculative item's cancellation polls: 1..8 */
+  double cif_ms, qcif_ms;  /* dvc_autotune: calibration time of the chosen variant */
+} dvc_tuning;
+int dvc_autotune(dvc_ctx *ctx, int planes, dvc_tuning *out);
+int dvc_get_tuning(dvc_ctx *ctx, dvc_tuning *out);
+int dvc_set_tuning(dvc_ctx *ctx, const dvc_tuning *tuning);
 
 /* Verification entry (no counterpart in the reference): evaluates one of the
  * device's scalar routines on n inputs.  func 0..3: float logf, expf, tanhf,

@@ -68,3 +68,26 @@ def test_fixed_start_level(ctx):
     dec, st = ctx.ldpca_decode(pl["LR"], pl["acc"], pl["crc"], start_level=0)
     assert st["start_level"] == 0
     assert np.array_equal(dec, pl["bits"])
+
+
+def test_autotune_and_every_tuning_decode_alike():
+    """The autotuner picks valid launch variants; decoding under other
+    variants (threads per workgroup, QCIF codewords per CU, poll interval)
+    gives the same bits, codes and iteration counts as the default."""
+    from dvc_amd import _lib
+    planes = W.ldpca_bitplane_set(n_cif=24, n_qcif=24, seed=41)
+    args = ([p["LR"] for p in planes], [p["acc"] for p in planes], [p["crc"] for p in planes])
+    base = _lib.Context(max_iter=50)
+    ref_dec, ref_st = base.ldpca_decode_batch(*args)
+    t = base.autotune(64)
+    print("tuning", t, flush=True)
+    assert t["cif_threads"] in (512, 704, 1024) and t["qcif_threads"] in (256, 512)
+    assert 1 <= t["qcif_wg_per_cu"] <= 4 and 1 <= t["poll"] <= 8 and t["cif_ms"] > 0
+    for var in [dict(cif_threads=704, qcif_threads=512, qcif_wg_per_cu=2, poll=4),
+                dict(cif_threads=512, qcif_threads=256, qcif_wg_per_cu=1, poll=2)]:
+        base.set_tuning(**var)
+        dec, st = base.ldpca_decode_batch(*args)
+        for d0, d1, s0, s1, p in zip(ref_dec, dec, ref_st, st, planes):
+            assert np.array_equal(d0, d1) and np.array_equal(d1, p["bits"])
+            assert (s0["level"], s0["iterations"]) == (s1["level"], s1["iterations"])
+    base.close()
