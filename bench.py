@@ -462,8 +462,8 @@ def main():
     ap.add_argument("--max-iter", type=int, default=100)
     ap.add_argument("--bp-threads", type=int, default=0, help="threads per CIF BP workgroup (0: library default)")
     ap.add_argument("--lanes", type=int, default=2, help="engine lanes (0: library default)")
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="keep the library's default BP launch configuration instead of dvc_autotune's choice")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run dvc_autotune before timing (the library defaults are its choices on MI355X)")
     ap.add_argument("--rates", default="50,200,800", help="serve: Poisson arrival rates (requests/s)")
     ap.add_argument("--serve-frames", type=int, default=17, help="serve: frames per stream")
     ap.add_argument("--serve-inflight", type=int, default=128, help="serve: engine frames in flight")

@@ -85,8 +85,9 @@ struct dvc_ctx {
   // BP launch configuration (dvc_autotune / dvc_set_tuning): threads per
   // workgroup for each code length, QCIF workgroups per CU, and the interval
   // in iterations at which a running speculative item polls for cancellation
+  // (defaults: dvc_autotune's choices on MI355X)
   int bp_threads_cif = 1024;
-  int bp_threads_qcif = 256;
+  int bp_threads_qcif = 512;
   int bp_wg_per_cu_qcif = 4;
   int bp_poll = 1;
   dvc::Ladder lad1584, lad6336;
