@@ -32,7 +32,7 @@ class DvcError(RuntimeError):
 
 class Params(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("max_iter", ctypes.c_int),
-                ("data_dir", ctypes.c_char_p), ("bp_threads", ctypes.c_int)]
+                ("data_dir", ctypes.c_char_p), ("bp_threads", ctypes.c_int), ("parallel_mode", ctypes.c_int)]
 
 
 class BpStats(ctypes.Structure):
@@ -191,8 +191,8 @@ class DeviceBuffer:
 class Context:
     """Owns a dvc_ctx (device, both ladders and their inverses resident)."""
 
-    def __init__(self, device=0, max_iter=50, bp_threads=0, data_dir=None):
-        p = Params(device, max_iter, str(data_dir).encode() if data_dir else None, bp_threads)
+    def __init__(self, device=0, max_iter=50, bp_threads=0, data_dir=None, parallel_mode=0):
+        p = Params(device, max_iter, str(data_dir).encode() if data_dir else None, bp_threads, parallel_mode)
         h = ctypes.c_void_p()
         _check(lib.dvc_ctx_create(ctypes.byref(p), ctypes.byref(h)), "dvc_ctx_create")
         self.h = h
@@ -236,13 +236,14 @@ class Context:
         _check(lib.dvc_ladder_export(self.h, n, code, _ptr(ir), _ptr(inv)), "dvc_ladder_export")
         return ir, inv
 
-    def ldpca_decode(self, lr, acc, crc, start_level=-1):
+    def ldpca_decode(self, lr, acc, crc, start_level=-1, carry=None):
         """Sequential rate search for one bitplane (host arrays).  Returns
-        (decoded uint8[N], stats dict)."""
+        (decoded uint8[N], stats dict).  carry: with parallel_mode 1, the
+        decisions carried into the first code (default: none match)."""
         lr = np.ascontiguousarray(lr, np.float32)
         acc = np.ascontiguousarray(acc, np.uint8)
         n = lr.size
-        out = np.zeros(n, np.uint8)
+        out = np.full(n, 0xAA, np.uint8) if carry is None else np.array(carry, np.uint8)
         st = BpStats()
         _check(lib.dvc_ldpca_decode(self.h, n, _ptr(lr), _ptr(acc), int(crc), int(start_level), _ptr(out),
                                     ctypes.byref(st)), "dvc_ldpca_decode")
@@ -330,15 +331,17 @@ class Context:
         _check(lib.dvc_side_info_batch(self.h, k, reqs), "dvc_side_info_batch")
         return outs
 
-    def ldpca_decode_batch(self, lrs, accs, crcs, start_levels=None):
-        """Speculative batched decode of many bitplanes (mixed N allowed)."""
+    def ldpca_decode_batch(self, lrs, accs, crcs, start_levels=None, carries=None):
+        """Speculative batched decode of many bitplanes (mixed N allowed).
+        carries: with parallel_mode 1, per bitplane the decisions carried into
+        its first code (default: none match)."""
         k = len(lrs)
         items = (LdpcaItem * max(k, 1))()
         keep, outs = [], []
         for i in range(k):
             lr = np.ascontiguousarray(lrs[i], np.float32)
             acc = np.ascontiguousarray(accs[i], np.uint8)
-            out = np.zeros(lr.size, np.uint8)
+            out = np.full(lr.size, 0xAA, np.uint8) if carries is None else np.array(carries[i], np.uint8)
             keep += [lr, acc]
             outs.append(out)
             sl = -1 if start_levels is None else int(start_levels[i])

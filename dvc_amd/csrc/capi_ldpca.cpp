@@ -107,6 +107,7 @@ int dvc_ctx_create(const dvc_params *params, dvc_ctx **out) {
   *out = nullptr;
   dvc_params p{};
   if (params) p = *params;
+  DVC_CHECK(p.parallel_mode == 0 || p.parallel_mode == 1, DVC_EINVAL, "dvc_ctx_create: parallel_mode must be 0 or 1");
   int ndev = 0;
   DVC_HIP(hipGetDeviceCount(&ndev));
   DVC_CHECK(ndev > 0 && p.device >= 0 && p.device < ndev, DVC_EDEVICE, "dvc_ctx_create: no such HIP device");
@@ -119,6 +120,7 @@ int dvc_ctx_create(const dvc_params *params, dvc_ctx **out) {
   c->device = p.device;
   c->max_iter = p.max_iter > 0 ? p.max_iter : 50;
   if (p.bp_threads > 0) c->bp_threads_cif = p.bp_threads;
+  c->bp_mode = p.parallel_mode;
   c->sm_count = prop.multiProcessorCount;
   std::string dir = p.data_dir ? std::string(p.data_dir) : default_data_dir();
   int rc;
@@ -196,16 +198,21 @@ int dvc_ladder_export(dvc_ctx *c, int N, int code, int *ir_out, uint8_t *inv_out
 // Shared driver for one group of bitplanes with the same code length.
 // sequential != 0: one code per launch in increasing order (the plain path,
 // which mirrors the reference's per-code beliefPropagation_CUDA call).
+// ParallelMode 1: items[i].decoded holds the decisions carried into the
+// bitplane's first code on entry (the reference's in/out decoded buffer).
 static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int sequential, hipStream_t s) {
   Ladder *L = c->ladder(N);
   const int C = L->numCodes, nb = N / 8;
+  const bool mode1 = c->bp_mode == 1;
   DevBuf<float> dLR, dLLR;
-  DevBuf<uint8_t> dAcc, dSlots;
+  DevBuf<uint8_t> dAcc, dSlots, dCarry;
   DevBuf<PlaneDesc> dPlanes;
   DevBuf<PlaneState> dState;
   DevBuf<int4> dRec;
-  DevBuf<int> dCounters;  // [0] queue head, [1] slot count
-  const int slotCap = sequential ? 2 : 65 * n + 1;  // each (bitplane, code) accepts at most once
+  DevBuf<int> dCounters, dBcase;  // counters: [0] queue head, [1] slot count, [2] bitplanes to resume
+  // slots: each (bitplane, code) accepts at most once; ParallelMode 1 keeps
+  // every run code's decisions and its repeat-case decisions as well
+  const int slotCap = sequential ? (mode1 ? 2 * C + 2 : 2) : (mode1 ? 2 * C * n + 1 : C * n + 1);
   DVC_HIP(dLR.alloc((size_t)n * N));
   DVC_HIP(dLLR.alloc((size_t)n * N));
   DVC_HIP(dAcc.alloc((size_t)n * nb));
@@ -213,15 +220,22 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
   DVC_HIP(dPlanes.alloc(n));
   DVC_HIP(dState.alloc(n));
   DVC_HIP(dRec.alloc((size_t)n * C));
-  DVC_HIP(dCounters.alloc(2));
+  DVC_HIP(dCounters.alloc(3));
+  if (mode1) {
+    DVC_HIP(dCarry.alloc((size_t)n * N));
+    DVC_HIP(dBcase.alloc((size_t)n * C));
+  }
   std::vector<PlaneDesc> planes(n);
   std::vector<PlaneState> state(n);
   for (int i = 0; i < n; i++) {
     DVC_CHECK(items[i].LR && items[i].acc && items[i].decoded, DVC_EINVAL, "dvc_ldpca: NULL item pointer");
     DVC_HIP(hipMemcpyAsync(dLR.p + (size_t)i * N, items[i].LR, sizeof(float) * N, hipMemcpyHostToDevice, s));
     DVC_HIP(hipMemcpyAsync(dAcc.p + (size_t)i * nb, items[i].acc, nb, hipMemcpyHostToDevice, s));
+    if (mode1)
+      DVC_HIP(hipMemcpyAsync(dCarry.p + (size_t)i * N, items[i].decoded, N, hipMemcpyHostToDevice, s));
     planes[i] = PlaneDesc{N, dLR.p + (size_t)i * N, dLLR.p + (size_t)i * N, dAcc.p + (size_t)i * nb,
-                          (uint32_t)items[i].crc, dState.p + i, dRec.p + (size_t)i * C};
+                          (uint32_t)items[i].crc, dState.p + i, dRec.p + (size_t)i * C,
+                          mode1 ? dCarry.p + (size_t)i * N : nullptr, mode1 ? dBcase.p + (size_t)i * C : nullptr};
     int st0 = items[i].start_level;
     DVC_CHECK(st0 < C, DVC_EINVAL, "dvc_ldpca: start_level beyond the ladder");
     state[i] = PlaneState{st0 >= 0 ? st0 : -1, C - 1, 0.0f, 0};
@@ -238,31 +252,36 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
   LadderDev ld = ladder_dev(*L);
   std::vector<int4> rec((size_t)n * C);
   if (sequential) {
-    // Plain path: bitplane by bitplane, one code per launch.
+    // Plain path: bitplane by bitplane, one code per launch, each with the
+    // decisions the previous code left (ParallelMode 1).
     for (int i = 0; i < n; i++) {
       int level = state[i].start;
+      int4 r;
       for (; level < C - 1; level++) {
-        PlaneState ps{level, C - 1, state[i].entropy, 0};
-        int zero[2] = {0, 0};
+        PlaneState ps{state[i].start, C - 1, state[i].entropy, level};
+        int zero[3] = {0, 0, 0};
         DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
         DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
-        BpArgs a{ld, dPlanes.p + i, 1, dCounters.p, 1, 1, c->max_iter, dSlots.p, dCounters.p + 1, slotCap, c->bp_poll};
+        BpArgs a{ld, dPlanes.p + i, 1, dCounters.p, 1, 1, c->max_iter, dSlots.p, dCounters.p + 1, slotCap, c->bp_poll,
+                 c->bp_mode};
         TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, 1, s));
-        int4 r;
         DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + level, sizeof(r), hipMemcpyDeviceToHost, s));
         DVC_HIP(hipStreamSynchronize(s));
         if (r.y == REC_ACCEPTED) {
           DVC_HIP(hipMemcpy(items[i].decoded, dSlots.p + (size_t)r.z * N, N, hipMemcpyDeviceToHost));
           break;
         }
+        if (mode1) {
+          DVC_CHECK(r.z >= 0 && r.z < slotCap, DVC_ESTATE, "dvc_ldpca: missing decisions of a failed code");
+          DVC_HIP(hipMemcpyAsync(dCarry.p + (size_t)i * N, dSlots.p + (size_t)r.z * N, N, hipMemcpyDeviceToDevice, s));
+        }
       }
       if (level == C - 1) {
-        PlaneState ps{state[i].start, C - 1, state[i].entropy, 0};
-        int zero[2] = {0, 0};
+        PlaneState ps{state[i].start, C - 1, state[i].entropy, C};
+        int zero[3] = {0, 0, 0};
         DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
         DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
         TIMED(c, "invh", s, launch_invh(dPlanes.p + i, 1, ld, dSlots.p, dCounters.p + 1, slotCap, s));
-        int4 r;
         DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + (C - 1), sizeof(r), hipMemcpyDeviceToHost, s));
         DVC_HIP(hipStreamSynchronize(s));
         DVC_HIP(hipMemcpy(items[i].decoded, dSlots.p + (size_t)r.z * N, N, hipMemcpyDeviceToHost));
@@ -270,13 +289,27 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
     }
     DVC_HIP(hipMemcpy(rec.data(), dRec.p, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost));
   } else {
-    int zero[2] = {0, 0};
+    int zero[3] = {0, 0, 0};
     DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
     const int maxOff = C - 1;
     BpArgs a{ld, dPlanes.p, n, dCounters.p, n * maxOff, maxOff, c->max_iter, dSlots.p, dCounters.p + 1, slotCap,
-             c->bp_poll};
+             c->bp_poll, c->bp_mode};
     const int grid = bp_grid(c, N, n * maxOff);
     TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
+    for (int round = 0; mode1; round++) {
+      // ParallelMode 1: resolve with the carried decisions; resume the
+      // bitplanes that reach a code this launch did not finish
+      DVC_CHECK(round <= C, DVC_ESTATE, "dvc_ldpca: ParallelMode 1 resolution does not settle");
+      DVC_HIP(launch_zero_ints(dCounters.p + 2, 1, s));
+      DVC_HIP(launch_mode1_resolve(dPlanes.p, n, C, N, dSlots.p, slotCap, dCounters.p + 2, s));
+      int resume = 0;
+      DVC_HIP(hipMemcpyAsync(&resume, dCounters.p + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+      DVC_HIP(hipStreamSynchronize(s));
+      DVC_CHECK(resume < (1 << 20), DVC_ENOMEM, "ParallelMode 1: decision slots exhausted");
+    if (!resume) break;
+      DVC_HIP(launch_zero_ints(dCounters.p, 1, s));
+      TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
+    }
     TIMED(c, "invh", s, launch_invh(dPlanes.p, n, ld, dSlots.p, dCounters.p + 1, slotCap, s));
     int counters[2];
     DVC_HIP(hipMemcpyAsync(rec.data(), dRec.p, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost, s));

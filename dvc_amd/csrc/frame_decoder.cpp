@@ -109,6 +109,8 @@ struct Job {
   PlaneState *d_state = nullptr;
   int4 *d_rec = nullptr;
   float *d_LLR = nullptr;
+  uint8_t *d_carry = nullptr;  // ParallelMode 1: decisions carried from code to code and bitplane to bitplane
+  int *d_bcase = nullptr;      // ParallelMode 1: [nbp][65] repeat-case decision slots
   // cursor
   int bandPos = 0, bit = 0, bp = 0;
   int fixups = 0;                       // reconstruction entries recomputed on the host
@@ -143,7 +145,8 @@ static size_t job_bytes(const Job &j) {
                    4 * (size_t)N, (size_t)N, (size_t)FS, sizeof(MVd) * (size_t)(FS / 256), sizeof(MVd) * (size_t)(FS / 64),
                    sizeof(MVd) * (size_t)(FS / 64), sizeof(MVd) * (size_t)(FS / 256), 16 * (size_t)(FS / 256),
                    8 * (size_t)(FS / 64), (size_t)4, 4 * (size_t)N, (size_t)j.nbp * (N / 8),
-                   sizeof(PlaneState) * (size_t)j.nbp, sizeof(int4) * 65 * (size_t)j.nbp, 4 * (size_t)N})
+                   sizeof(PlaneState) * (size_t)j.nbp, sizeof(int4) * 65 * (size_t)j.nbp, 4 * (size_t)N, (size_t)N,
+                   4 * 65 * (size_t)j.nbp})
     t = align_up(t + b);
   return t;
 }
@@ -162,7 +165,8 @@ static int alloc_job(Job &j, Arena &arena) {
                o_bmvt = take(sizeof(MVd) * (size_t)(FS / 64)), o_fwd = take(sizeof(MVd) * (size_t)(FS / 256)),
                o_holes = take(16 * (size_t)(FS / 256)), o_mse = take(8 * (size_t)(FS / 64)), o_tc = take(4),
                o_tl = take(4 * (size_t)N), o_acc = take((size_t)j.nbp * (N / 8)), o_state = take(sizeof(PlaneState) * (size_t)j.nbp),
-               o_recd = take(sizeof(int4) * 65 * (size_t)j.nbp), o_llr = take(4 * (size_t)N);
+               o_recd = take(sizeof(int4) * 65 * (size_t)j.nbp), o_llr = take(4 * (size_t)N), o_carry = take(N),
+               o_bcase = take(4 * 65 * (size_t)j.nbp);
   uint8_t *base = nullptr;
   (void)arena;
   DVC_HIP(hipMalloc((void **)&base, off));
@@ -177,6 +181,8 @@ static int alloc_job(Job &j, Arena &arena) {
   f.mse = (double *)(base + o_mse); f.tieCount = (int *)(base + o_tc); f.tieList = (int *)(base + o_tl);
   j.d_acc = base + o_acc; j.d_state = (PlaneState *)(base + o_state); j.d_rec = (int4 *)(base + o_recd);
   j.d_LLR = (float *)(base + o_llr);
+  j.d_carry = base + o_carry;
+  j.d_bcase = (int *)(base + o_bcase);
   return DVC_OK;
 }
 
@@ -251,10 +257,25 @@ int dvc::bp_step_launch(dvc_ctx *c, int N, int n, uint8_t *d_slots, int slotCap,
   const int threads = bp_threads(c, N);
   const int maxOff = C - 1;
   TIMED_LAUNCH(c, "ce_llr", s, launch_ce_llr(d_planes, n, 1, c->lad1584, c->lad6336, s));
-  DVC_HIP(launch_zero_ints(d_counters, 2, s));
-  BpArgs a{ld, d_planes, n, d_counters, n * maxOff, maxOff, c->max_iter, d_slots, d_counters + 1, slotCap, c->bp_poll};
+  DVC_HIP(launch_zero_ints(d_counters, 3, s));
+  BpArgs a{ld, d_planes, n, d_counters, n * maxOff, maxOff, c->max_iter, d_slots, d_counters + 1, slotCap, c->bp_poll,
+           c->bp_mode};
   const int grid = bp_grid(c, N, n * maxOff);
   TIMED_LAUNCH(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
+  for (int round = 0; c->bp_mode == 1; round++) {
+    // ParallelMode 1: resolve with the carried decisions and resume the
+    // bitplanes that reach a code the launch did not finish (host-synchronous)
+    DVC_CHECK(round <= C, DVC_ESTATE, "ParallelMode 1 resolution does not settle");
+    DVC_HIP(launch_zero_ints(d_counters + 2, 1, s));
+    DVC_HIP(launch_mode1_resolve(d_planes, n, C, N, d_slots, slotCap, d_counters + 2, s));
+    int resume = 0;
+    DVC_HIP(hipMemcpyAsync(&resume, d_counters + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+    DVC_HIP(hipStreamSynchronize(s));
+    DVC_CHECK(resume < (1 << 20), DVC_ENOMEM, "ParallelMode 1: decision slots exhausted");
+    if (!resume) break;
+    DVC_HIP(launch_zero_ints(d_counters, 1, s));
+    TIMED_LAUNCH(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
+  }
   TIMED_LAUNCH(c, "invh", s, launch_invh(d_planes, n, ld, d_slots, d_counters + 1, slotCap, s));
   return DVC_OK;
 }
@@ -283,16 +304,22 @@ int dvc::bp_step(dvc_ctx *c, int N, const std::vector<PlaneDesc> &planes, bool s
     PlaneState ps;
     int4 r;
     for (; level < C - 1; level++) {
-      ps = PlaneState{level, C - 1, st[i].entropy, 0};
+      ps = PlaneState{st[i].start, C - 1, st[i].entropy, level};
       DVC_HIP(hipMemcpyAsync(planes[i].state, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
-      DVC_HIP(hipMemsetAsync(d_counters, 0, sizeof(int), s));
-      BpArgs a{ld, d_planes + i, 1, d_counters, 1, 1, c->max_iter, d_slots, d_counters + 1, slotCap, c->bp_poll};
+      DVC_HIP(hipMemsetAsync(d_counters, 0, sizeof(int), s));  // queue head only: the step's slots accumulate
+      BpArgs a{ld, d_planes + i, 1, d_counters, 1, 1, c->max_iter, d_slots, d_counters + 1, slotCap, c->bp_poll,
+               c->bp_mode};
       TIMED_LAUNCH(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, 1, s));
       DVC_HIP(hipMemcpyAsync(&r, planes[i].rec + level, sizeof(r), hipMemcpyDeviceToHost, s));
       DVC_HIP(hipStreamSynchronize(s));
       if (r.y == REC_ACCEPTED) { ok = true; break; }
+      if (c->bp_mode == 1) {
+        // the failed code's decisions are carried into the next code
+        DVC_CHECK(r.z >= 0 && r.z < slotCap, DVC_ESTATE, "missing decisions of a failed code");
+        DVC_HIP(hipMemcpyAsync(planes[i].carry, d_slots + (size_t)r.z * N, N, hipMemcpyDeviceToDevice, s));
+      }
     }
-    ps = PlaneState{st[i].start, ok ? level : C - 1, st[i].entropy, 0};
+    ps = PlaneState{st[i].start, ok ? level : C - 1, st[i].entropy, C};
     DVC_HIP(hipMemcpyAsync(planes[i].state, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
     if (!ok) TIMED_LAUNCH(c, "invh", s, launch_invh(d_planes + i, 1, ld, d_slots, d_counters + 1, slotCap, s));
     DVC_HIP(hipStreamSynchronize(s));
@@ -454,6 +481,10 @@ static int admit(dvc_engine *e, Job *j, hipStream_t s) {
   j->initRec.assign((size_t)65 * j->nbp, make_int4(0, REC_NOT_RUN, -1, 0));
   DVC_HIP(hipMemcpyAsync(j->d_state, j->initState.data(), sizeof(PlaneState) * j->nbp, hipMemcpyHostToDevice, s));
   DVC_HIP(hipMemcpyAsync(j->d_rec, j->initRec.data(), j->initRec.size() * sizeof(int4), hipMemcpyHostToDevice, s));
+  // ParallelMode 1: the reference reads its per-frame decoded buffer before
+  // anything is written to it (DVC_Decoder.cpp:124); bytes that are neither
+  // 0 nor 1 (as in its golden runs, MALLOC_PERTURB_ fill 0xAA) never match
+  DVC_HIP(hipMemsetAsync(j->d_carry, 0xAA, j->N, s));
   j->bandPos = 0;
   j->bit = j->bands.empty() ? 0 : j->bands[0].qbits;
   j->bp = 0;
@@ -580,11 +611,14 @@ static int lane_step(dvc_engine *e, Lane &ln) {
     ln.links.resize(active.size());
     for (size_t a = 0; a < active.size(); a++) {
       Job &j = *ln.inflight[active[a]];
+      const bool m1 = c->bp_mode == 1;
       PlaneDesc pd{j.N, j.fd.LR, j.d_LLR, j.d_acc + (size_t)j.bp * (j.N / 8), (uint32_t)j.crc[j.bp],
-                   j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp};
+                   j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp, m1 ? j.d_carry : nullptr,
+                   m1 ? j.d_bcase + (size_t)65 * j.bp : nullptr};
       ln.groups[j.N == 6336].push_back(pd);
       ln.links[a] = BpFrameLink{j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp,
-                                ln.d_slots + (size_t)(j.N == 6336 ? ln.groupCap : 0) * 6336, ln.groupCap};
+                                ln.d_slots + (size_t)(j.N == 6336 ? ln.groupCap : 0) * 6336, ln.groupCap,
+                                m1 ? j.d_carry : nullptr};
     }
     const int na = (int)active.size(), n0 = (int)ln.groups[0].size(), n1 = (int)ln.groups[1].size();
     DVC_HIP(hipMemcpyAsync(ln.d_links, ln.links.data(), sizeof(BpFrameLink) * na, hipMemcpyHostToDevice, s));
@@ -603,9 +637,9 @@ static int lane_step(dvc_engine *e, Lane &ln) {
         if (!ng) continue;
         uint8_t *slots = ln.d_slots + (size_t)(g ? ln.groupCap : 0) * 6336;
         const int r = e->sequentialBp
-                          ? bp_step(c, g ? 6336 : 1584, ln.groups[g], true, slots, ln.groupCap, ln.d_counters + 2 * g,
+                          ? bp_step(c, g ? 6336 : 1584, ln.groups[g], true, slots, ln.groupCap, ln.d_counters + 3 * g,
                                     ln.d_planes + off, s)
-                          : bp_step_launch(c, g ? 6336 : 1584, ng, slots, ln.groupCap, ln.d_counters + 2 * g,
+                          : bp_step_launch(c, g ? 6336 : 1584, ng, slots, ln.groupCap, ln.d_counters + 3 * g,
                                            ln.d_planes + off, s);
         if (r) return r;
         off += ng;
@@ -613,8 +647,9 @@ static int lane_step(dvc_engine *e, Lane &ln) {
       TIMED_LAUNCH(c, "put_bitplane", s, launch_put_bitplane(ln.d_frames, L0, ln.d_links, na, maxN, s));
       return DVC_OK;
     };
-    const bool steady = e->graphs && !e->sequentialBp && !c->profiling && hl3.empty() && starting.empty() &&
-                        ending.empty();
+    // (ParallelMode 1 steps read a resolution count back, so they are not captured)
+    const bool steady = e->graphs && !e->sequentialBp && !c->profiling && c->bp_mode == 0 && hl3.empty() &&
+                        starting.empty() && ending.empty();
     if (steady) {
       const auto key = std::make_tuple(na, n0, n1, maxN);
       auto it = ln.graphs.find(key);
@@ -805,7 +840,7 @@ static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **ou
   for (int k = 0; k < nl && err == hipSuccess; k++) {
     Lane &l = e->lanes[k];
     l.cap = (e->maxInflight + nl - 1) / nl;
-    l.groupCap = 65 * l.cap + 1;
+    l.groupCap = (c->bp_mode == 1 ? 130 : 65) * l.cap + 1;  // ParallelMode 1 keeps two decision sets per code
     const int M = l.cap;
     err = hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking);
     if (err == hipSuccess) err = hipMalloc((void **)&l.d_frames, sizeof(FrameDev) * M);
@@ -813,7 +848,7 @@ static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **ou
     if (err == hipSuccess) err = hipMalloc((void **)&l.d_planes, sizeof(PlaneDesc) * M);
     if (err == hipSuccess) err = hipMalloc((void **)&l.d_links, sizeof(BpFrameLink) * M);
     if (err == hipSuccess) err = hipMalloc((void **)&l.d_slots, (size_t)2 * l.groupCap * 6336);
-    if (err == hipSuccess) err = hipMalloc((void **)&l.d_counters, 4 * sizeof(int));
+    if (err == hipSuccess) err = hipMalloc((void **)&l.d_counters, 6 * sizeof(int));
     if (err == hipSuccess) err = hipMalloc((void **)&l.d_ties, sizeof(int) * M);
   }
   if (err != hipSuccess) {

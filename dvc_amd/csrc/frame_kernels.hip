@@ -668,6 +668,7 @@ __global__ void k_put_bitplane(const FrameDev *F, const int *list, const BpFrame
   if (slot < 0 || slot >= lk.slot_capacity) return;  // the host reports the missing record
   const uint8_t bitv = lk.slots[(size_t)slot * f.N + b];
   f.decoded[b] = bitv;
+  if (lk.carry) lk.carry[b] = bitv;  // ParallelMode 1: the next bitplane starts from these decisions
   const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4, index = f.index;
   int *p = &f.reconsTrans[(top + (index >> 2)) * f.W + left + (index & 3)];
   *p = (*p << 1) + (int)bitv;

@@ -86,6 +86,7 @@ __global__ __launch_bounds__(256) void ce_llr_kernel(const PlaneDesc *planes, in
     PlaneState *st = d.state;
     st->entropy = rlb;
     if (st->start < 0 || !fixed_start) st->start = s;
+    st->run = st->start;
   }
 }
 // ------------------------------------------------------------------ BP
@@ -111,6 +112,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
   __shared__ uint32_t s_crc[T / 64];
   __shared__ double s_logtab[32];  // logf's (invc, logc) table, read with per-lane indices
   __shared__ uint32_t parBits[2][S::SYN_WORDS];  // decision parity per check, written by alternate iterations
+  __shared__ int s_changed[2];  // ParallelMode 1: some decision changed in the variable pass (alternate iterations)
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const LadderDev L = a.lad;
@@ -128,7 +130,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         if (w >= a.total) break;
         int b = w % a.nplanes, off = w / a.nplanes;
         PlaneState *st = a.planes[b].state;
-        int level = st->start + off;
+        int level = st->run + off;
         if (off >= a.max_offset || level >= L.numCodes - 1) continue;
         if (__hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < level) continue;
         item = w;
@@ -142,7 +144,12 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
     const int b = w % a.nplanes;
     const PlaneDesc pd = a.planes[b];
     PlaneState *st = pd.state;
-    const int level = st->start + w / a.nplanes;
+    const int run = st->run;
+    const int level = run + w / a.nplanes;
+    // ParallelMode 1: the decisions carried into this code are known only for
+    // the queue's first code; a later code runs as if its first decisions
+    // differ from them and records the one case where that matters
+    const bool mode1 = a.mode == 1, exact = level == run;
     const int m = L.syndLength[level];
     const uint16_t *gir = L.ir16 + (size_t)level * E;
 
@@ -176,6 +183,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
     __syncthreads();
 
     int status = 0, iters = 0;
+    int sameCount = 0, bcaseSlot = 0;
     for (int t = 0;; t++) {
       // ---- check-node pass: |phi| sums and sign products; parity of the
       // previous decisions (:495-515, :551-568)
@@ -198,15 +206,41 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         if (t > 0 && (syn ^ ((parPrev[r >> 5] >> (r & 31)) & 1u))) fail = 1;
       }
       for (int wd = tid; wd < S::SYN_WORDS; wd += T) parBits[t & 1][wd] = 0u;
-      if (tid == 0)
+      // ParallelMode 1 (beliefPropagation_OpenMP, LdpcaDecoder.cpp:254-264):
+      // count consecutive iterations whose decisions all repeat the previous
+      // ones; the fourth such iteration ends the code as a failure, before
+      // the parity test of the same iteration
+      if (mode1 && t > 0) sameCount = s_changed[(t - 1) & 1] ? 0 : sameCount + 1;
+      if (tid == 0) {
+        s_changed[t & 1] = 0;
         s_cancel = (t % a.poll == 0) &&
                    __hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < level;
+      }
       const int anyFail = __syncthreads_or(fail);
+      if (mode1 && !exact && t == 4 && sameCount == 3) {
+        // Decisions of iterations 0-3 all equal: had the carried decisions
+        // equalled them too, the code would have failed here after 4
+        // iterations.  Keep these decisions for the resolution.
+        if (tid == 0) s_item = atomicAdd(a.slot_count, 1);
+        __syncthreads();
+        const int bs = s_item;
+        if (bs < a.slot_capacity) {
+          uint8_t *dst = a.slots + (size_t)bs * N;
+          for (int j = 0; j < S::CPT; j++) {
+            const int k = tid + j * T;
+            if (k < N) dst[k] = (uint8_t)((decBits[k >> 5] >> (k & 31)) & 1u);
+          }
+        }
+        bcaseSlot = bs + 1;
+        __syncthreads();
+      }
+      if (mode1 && sameCount == 4) { status = 0; iters = t; break; }
       if (t > 0 && !anyFail) { status = 1; iters = t; break; }
       if (s_cancel) { status = 3; iters = t; break; }
       if (t == a.max_iter) { status = 0; iters = t; break; }
 
       // ---- variable-node pass: c2v, overall LLR, decision, next v2c (:512-548)
+      bool changed = false;
       for (int j = 0; j < S::CPT; j++) {
         const int k = tid + j * T;
         uint32_t dec = 0;
@@ -228,7 +262,13 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
           tmp += c2v[0];
           tmp += c2v[1];
           tmp += c2v[2];
-          dec = f2u(tmp) >> 31;
+          if (!mode1) {
+            dec = f2u(tmp) >> 31;
+          } else {
+            dec = tmp < 0.0f;  // LLR_overall < 0 (LdpcaDecoder.cpp:257-260): -0 decides 0
+            if (t == 0) changed |= !exact || pd.carry[k] != (uint8_t)dec;
+            else changed |= ((decBits[k >> 5] >> (k & 31)) & 1u) != dec;
+          }
           if (dec) {
 #pragma unroll
             for (int q = 0; q < 3; q++) atomicXor(&parBits[t & 1][row[q] >> 5], 1u << (row[q] & 31));
@@ -248,6 +288,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
           decBits[word + 1] = (uint32_t)(bal >> 32);
         }
       }
+      if (changed) s_changed[t & 1] = 1;
       __syncthreads();
     }
 
@@ -286,9 +327,84 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         }
       }
     }
-    if (tid == 0 && !ok)
+    if (!ok && mode1 && status != 3) {
+      // ParallelMode 1 keeps every run code's final decisions: they are the
+      // decisions carried into the next code
+      if (tid == 0) s_item = atomicAdd(a.slot_count, 1);
+      __syncthreads();
+      const int slot = s_item;
+      if (slot < a.slot_capacity) {
+        uint8_t *dst = a.slots + (size_t)slot * N;
+        for (int j = 0; j < S::CPT; j++) {
+          const int k = tid + j * T;
+          if (k < N) dst[k] = (uint8_t)((decBits[k >> 5] >> (k & 31)) & 1u);
+        }
+      }
+      if (tid == 0) pd.rec[level] = make_int4(iters, status == 1 ? 1 : 0, slot, 0);
+    } else if (tid == 0 && !ok) {
       pd.rec[level] = make_int4(iters, status == 1 ? 1 : (status == 3 ? 3 : 0), -1, 0);
+    }
+    if (tid == 0 && mode1) pd.bcase[level] = bcaseSlot;
     __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ ParallelMode 1 resolution
+// After a speculative ParallelMode 1 launch, walks each bitplane's codes from
+// the queue's first code with the decisions actually carried from code to
+// code (LdpcaDecoder.cpp:646-771 with beliefPropagation_OpenMP).  A code that
+// recorded the repeat case fails after 4 iterations when the carried
+// decisions equal its recorded ones; otherwise its speculative outcome
+// stands.  The first accepted code becomes `best`.  When the walk reaches a
+// code the launch cancelled or never ran, the bitplane is queued again from
+// that code with the carried decisions now known (rerun_count counts them).
+// One wave per bitplane.
+// A slot index at or beyond slot_capacity (its decisions were not stored)
+// adds 1 << 20 to rerun_count, which the host reports as exhausted slots.
+__global__ __launch_bounds__(64) void mode1_resolve_kernel(const PlaneDesc *planes, int nplanes, int numCodes, int N,
+                                                           const uint8_t *slots, int slot_capacity, int *rerun_count) {
+  const int p = blockIdx.x;
+  if (p >= nplanes) return;
+  const PlaneDesc d = planes[p];
+  PlaneState *st = d.state;
+  const int run0 = st->run, lane = threadIdx.x;
+  for (int L = run0; L < numCodes - 1; L++) {
+    const int4 r = d.rec[L];
+    if (r.y == REC_NOT_RUN || r.y == REC_CANCELLED) {
+      if (lane == 0) {
+        st->run = L;
+        st->best = numCodes - 1;
+        atomicAdd(rerun_count, 1);
+      }
+      return;
+    }
+    const int bs = L != run0 ? d.bcase[L] : 0;
+    if (bs - 1 >= slot_capacity || ((r.y == REC_FAILED || r.y == REC_CRC_MISMATCH) && (r.z < 0 || r.z >= slot_capacity))) {
+      if (lane == 0) atomicAdd(rerun_count, 1 << 20);
+      return;
+    }
+    bool repeat = false;
+    if (bs > 0) {
+      const uint8_t *b = slots + (size_t)(bs - 1) * N;
+      int diff = 0;
+      for (int k = lane; k < N; k += 64) diff |= b[k] != d.carry[k];
+      repeat = __ballot(diff) == 0;
+    }
+    if (!repeat && r.y == REC_ACCEPTED) {
+      if (lane == 0) {
+        st->best = L;
+        st->run = numCodes;
+      }
+      return;
+    }
+    const int fin = repeat ? bs - 1 : r.z;
+    if (repeat && lane == 0) d.rec[L] = make_int4(4, REC_FAILED, bs - 1, 0);
+    const uint8_t *src = slots + (size_t)fin * N;
+    for (int k = lane; k < N; k += 64) d.carry[k] = src[k];
+  }
+  if (lane == 0) {
+    st->best = numCodes - 1;  // every BP code failed: full-rate inverse
+    st->run = numCodes;
   }
 }
 
@@ -342,6 +458,13 @@ __global__ void zero_ints_kernel(int *p, int n) {
 }
 
 // ------------------------------------------------------------------ host side
+hipError_t launch_mode1_resolve(const PlaneDesc *planes, int n, int numCodes, int N, const uint8_t *slots,
+                                int slot_capacity, int *rerun_count, hipStream_t s) {
+  hipLaunchKernelGGL(mode1_resolve_kernel, dim3(n), dim3(64), 0, s, planes, n, numCodes, N, slots, slot_capacity,
+                     rerun_count);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {
   hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
   return hipGetLastError();

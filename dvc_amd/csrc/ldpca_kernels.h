@@ -13,7 +13,8 @@ struct PlaneState {
   int start;      // nCodeStart (-1: computed by the CE kernel)
   int best;       // lowest code that converged with a matching CRC (numCodes-1 = none)
   float entropy;  // H(X|Y) from ComputeCE
-  int pad;
+  int run;        // first code of the current launch's queue (start, or the code a
+                  // ParallelMode 1 resolution resumes at; numCodes = nothing left)
 };
 
 struct PlaneDesc {
@@ -24,6 +25,10 @@ struct PlaneDesc {
   uint32_t crc;        // encoder CRC-8
   PlaneState *state;
   int4 *rec;           // [numCodes] per code: iterations, status, decision slot, used_inverse
+  // ParallelMode 1 only (NULL otherwise):
+  uint8_t *carry;      // N decision bytes carried into code `run` (any byte other than 0/1 never matches)
+  int *bcase;          // [numCodes] decision slot + 1 of a code's iteration-3 decisions when they
+                       // repeat its first three (see bp_persistent_kernel), else 0
 };
 
 // What k_put_bitplane needs to find the accepted decisions of a bitplane.
@@ -32,6 +37,7 @@ struct BpFrameLink {
   const int4 *records;
   const uint8_t *slots;
   int slot_capacity;
+  uint8_t *carry;  // ParallelMode 1: receives the accepted decisions (the next bitplane's carry)
 };
 
 struct LadderDev {
@@ -53,6 +59,7 @@ struct BpArgs {
   int *slot_count;
   int slot_capacity;
   int poll;           // iterations between polls of the bitplane's best code (>= 1)
+  int mode;           // 0: beliefPropagation, 1: beliefPropagation_OpenMP (ParallelMode 1)
 };
 
 // record.y status values
@@ -61,6 +68,8 @@ enum { REC_FAILED = 0, REC_CRC_MISMATCH = 1, REC_ACCEPTED = 2, REC_CANCELLED = 3
 LadderDev ladder_dev(const Ladder &L);
 hipError_t launch_bp_any(int N, int threads, const BpArgs &a, int grid, hipStream_t s);
 hipError_t launch_zero_ints(int *p, int n, hipStream_t s);
+hipError_t launch_mode1_resolve(const PlaneDesc *planes, int n, int numCodes, int N, const uint8_t *slots,
+                                int slot_capacity, int *rerun_count, hipStream_t s);
 hipError_t launch_ce_llr(const PlaneDesc *d_planes, int n, int fixed_start, const Ladder &l1584,
                          const Ladder &l6336, hipStream_t s);
 hipError_t launch_invh(const PlaneDesc *d_planes, int n, const LadderDev &L, uint8_t *slots, int *slot_count,

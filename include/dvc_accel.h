@@ -41,6 +41,10 @@ typedef struct {
   const char *data_dir;  /* directory with reg1584.ladder / reg6336.ladder; NULL = the
                             library's own data directory */
   int bp_threads;        /* threads per BP workgroup for N=6336 (0 = default 1024; see dvc_autotune) */
+  int parallel_mode;     /* BP semantics: 0 = beliefPropagation (ParallelMode 0/2, LdpcaDecoder.cpp:462-580);
+                            1 = beliefPropagation_OpenMP (ParallelMode 1, :164-306): decisions LLR < 0 and a
+                            failure after 4 consecutive iterations with unchanged decisions, compared first
+                            against the decisions the previous call left (see dvc_ldpca_decode) */
 } dvc_params;
 
 /* InitCUDA + LDPCA_CUDA_Init_Variable (LDPCA_cuda.cu:36-89, 546-763): selects the
@@ -88,8 +92,11 @@ typedef struct {
  *   crc         the encoder's CRC-8 of the bitplane
  *   start_level -1: computed on the device from ComputeCE (LdpcaDecoder.cpp:127-143);
  *               >= 0: used as given
- *   decoded     N output bytes (0/1)
- * BP follows ParallelMode 0 semantics (beliefPropagation, LdpcaDecoder.cpp:462-580). */
+ *   decoded     N output bytes (0/1).  With parallel_mode 1 it is also read on
+ *               entry: the decisions carried into the first code, as the
+ *               reference's decoded buffer carries them from call to call
+ *               (bytes other than 0/1 never match a decision)
+ * BP follows the context's parallel_mode semantics (0 by default). */
 int dvc_ldpca_decode(dvc_ctx *ctx, int N, const float *LR, const uint8_t *acc, uint8_t crc,
                      int start_level, uint8_t *decoded, dvc_bp_stats *stats);
 
