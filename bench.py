@@ -197,6 +197,7 @@ def bench_wz(args):
         times.append(time.perf_counter() - t)
         log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms for {len(jobs)} frames")
     outs = [(dev.recons(i), dict(st, psnr=psnr(dev.recons(i), j["orig"]))) for i, (j, st) in enumerate(zip(jobs, stats))]
+    mode1 = bench_wz_mode1(args, jobs, dev) if args.mode1_steps else None
     names = ["create_si", "keyframe_filters", "noise_param", "oct", "smf_init", "noise_dist", "condbit", "ce_llr",
              "bp_6336", "invh", "put_bitplane", "reconstruct", "inverse_transform", "update_si", "update_smf",
              "update_si_ml"]
@@ -244,11 +245,41 @@ def bench_wz(args):
         "plain_vs_batched_mismatches": plain_mismatch,
         "inputs": "resident in HBM before the timed region (key frames, syndromes, output frames)",
         "kernels_ms_per_step": kern, "dominant_kernel": dom[0], "bp_tuning": tuning,
+        "parallel_mode_1": mode1,
     }
     res["roofline"] = roofline_wz(dom, kern, args, len(jobs), outs)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_wz(jobs, args)
     return res
+
+
+def bench_wz_mode1(args, jobs, dev):
+    """The same workload decoded with ParallelMode 1 BP semantics (the
+    reference's OpenMP decoder: early jump on repeated decisions), which is
+    also the semantics of the CPU baseline leg."""
+    from dvc_amd import _lib
+    ctx = _lib.Context(device=0, max_iter=args.max_iter, bp_threads=args.bp_threads, parallel_mode=1)
+    eng = _lib.Engine(ctx, max_inflight=len(jobs), lanes=args.lanes, device_buffers=True)
+    run_engine(eng, dev.submit_all(eng, jobs[:16]))
+    eng.close()
+    times = []
+    for i in range(args.mode1_steps):
+        t = time.perf_counter()
+        eng = _lib.Engine(ctx, max_inflight=len(jobs), lanes=args.lanes, device_buffers=True)
+        stats = run_engine(eng, dev.submit_all(eng, jobs))
+        eng.close()
+        times.append(time.perf_counter() - t)
+        log(f"[bench] ParallelMode 1 step {i}: {times[-1] * 1e3:.1f} ms for {len(jobs)} frames")
+    plain_t = None
+    if args.plain_frames:
+        t = time.perf_counter()
+        ctx.decode_wz_frames([jobs[0]], batched=False)
+        plain_t = time.perf_counter() - t
+    t_step = float(np.mean(times))
+    ctx.close()
+    return {"value": len(jobs) / t_step, "unit": "WZ frames/s", "ms_per_step": t_step * 1e3,
+            "plain_fps": (1.0 / plain_t) if plain_t else None,
+            "bp_iterations_per_frame": float(np.mean([sum(st["iterations"]) for st in stats]))}
 
 
 def profiled_traffic(kernel, workload):
@@ -467,6 +498,8 @@ def main():
     ap.add_argument("--rates", default="50,200,800", help="serve: Poisson arrival rates (requests/s)")
     ap.add_argument("--serve-frames", type=int, default=17, help="serve: frames per stream")
     ap.add_argument("--serve-inflight", type=int, default=128, help="serve: engine frames in flight")
+    ap.add_argument("--mode1-steps", type=int, default=2,
+                    help="wz_cif_q8: also time this many steps with ParallelMode 1 BP semantics (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-planes", type=int, default=150)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)

@@ -22,7 +22,8 @@ def checker():
 
 @pytest.mark.parametrize("func,first,last,stride", [
     ("phi", "0", "7f800000", "1"),          # every non-negative finite float
-    ("phi_nonneg", "0", "80000000", "1"),   # the BP kernel's branch-free form, same domain plus -0
+    ("phi_nonneg", "0", "80000000", "1"),   # the branch-free form, same domain plus -0
+    ("phi_fast", "0", "80000000", "1"),     # its shortened form (the BP kernel's; IEEE quotients on the host)
     ("logf", "1", "7f7fffff", "7"),
     ("expf", "0", "ffffffff", "13"),
     ("tanhf", "0", "ffffffff", "13"),
