@@ -40,11 +40,15 @@ struct Ladder {
   std::vector<uint8_t> crcBasis;   // N: CRC-8 of the message with only bit k set
 
   // Device image, per BP code c (0..numCodes-2):
-  //   d_ir16[c*E + e]          decoder row of edge e (edges in column order)
-  //   d_rowEdge[c*E + j]       edges grouped by row, ascending edge within a row
-  //   d_rowPtr[c*(N+1) + r]    start of row r in d_rowEdge (r <= m)
+  //   d_colRows[c*N + k]       column k's edges 3k..3k+2 as 21-bit fields
+  //                            f = row | rank << 13 (rank: position of the
+  //                            edge among its row's edges in ascending edge
+  //                            order): x = f0 | f1 << 21, y = f1 >> 11 | f2 << 10
+  //   d_rowPtr[c*(N+1) + r]    number of edges in rows below r (r <= m): row
+  //                            r's messages sit at rowPtr[r] + rank
   //   d_tx[c*N + k]            accumulated-syndrome position of syndrome bit k
-  uint16_t *d_ir16 = nullptr, *d_rowEdge = nullptr, *d_rowPtr = nullptr, *d_tx = nullptr;
+  uint2 *d_colRows = nullptr;
+  uint16_t *d_rowPtr = nullptr, *d_tx = nullptr;
   int *d_syndLength = nullptr;
   float *d_codeRate = nullptr;
   uint8_t *d_invH = nullptr, *d_crcBasis = nullptr;

@@ -354,6 +354,15 @@ DVC_MATH_FN float dvcm_div_normal(float a, float b) {
 //     arguments 2y >= 2 and -2y > -2 and is dropped; the remaining result
 //     cases share one exponent-adjusted form;
 //   * both quotients use dvcm_div_normal (operands are normal and bounded).
+// Pins a value in a register at this point, so the compiler cannot sink its
+// computation into a branch (device builds; a no-op on the host).
+DVC_MATH_FN float dvcm_pin(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __asm__("" : "+v"(v));
+#endif
+  return v;
+}
+
 DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *logtab) {
   DVCM_NOCONTRACT
   const float ln2_hi = 6.9313812256e-01f, ln2_lo = 9.0580006145e-06f, invln2 = 1.4426950216e+00f,
@@ -388,13 +397,18 @@ DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *logtab) {
   const float r_km1 = 0.5f * (xx - e2) - 0.5f;
   const int kbig = (k <= -2 || k > 56);
   const int kc = k < 0 ? 0 : (k > 31 ? 31 : k);
+  // every result case is computed, then one is selected: no branches, so
+  // independent calls can be interleaved
   const float u = e2 - xx;
-  float Y;
-  if (kbig) Y = 1.0f - u;
-  else if (k < 23) Y = dvcm_asf(0x3f800000u - (0x1000000u >> kc)) - u;
-  else Y = (xx - (e2 + dvcm_asf((uint32_t)(0x7f - k) << 23))) + 1.0f;
+  const float Ybig = 1.0f - u;
+  const float Ymid = dvcm_pin(dvcm_asf(0x3f800000u - (0x1000000u >> kc)) - u);
+  const float Yhi = dvcm_pin((xx - (e2 + dvcm_asf((uint32_t)(0x7f - k) << 23))) + 1.0f);
+  const float Y = kbig ? Ybig : (k < 23 ? Ymid : Yhi);
   const float R = dvcm_asf(dvcm_asu(Y) + ((uint32_t)k << 23));
-  float em1 = (k == 0) ? r_k0 : (k == -1) ? r_km1 : (kbig ? R - 1.0f : R);
+  const float Rm1 = R - 1.0f;
+  float em1 = kbig ? Rm1 : R;
+  em1 = (k == -1) ? r_km1 : em1;
+  em1 = (k == 0) ? r_k0 : em1;
   em1 = (!red && hx < 0x33000000u) ? a : em1;
   // ---- tanhf(y)
   const float q = dvcm_div_normal(big ? 2.0f : -em1, em1 + 2.0f);

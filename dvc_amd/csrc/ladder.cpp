@@ -143,29 +143,40 @@ int load_ladder(const std::string &dir, int N, Ladder &L) {
 
 int upload_ladder(Ladder &L) {
   const int N = L.N, E = L.E, C = L.numCodes;
-  std::vector<uint16_t> ir16((size_t)C * E), rowEdge((size_t)C * E), rowPtr((size_t)C * (N + 1)), tx((size_t)C * N);
+  std::vector<uint32_t> colRows((size_t)C * N * 2);
+  std::vector<uint16_t> rowPtr((size_t)C * (N + 1)), tx((size_t)C * N);
+  DVC_CHECK(E <= 0xffff && N <= 0x2000, DVC_EIO, "ladder: more than 65535 edges or 8192 checks");
   for (int c = 0; c < C; c++) {
     int m = L.syndLength[c], b = L.bitsPerPeriod[c];
     const int *dir = &L.decIr[(size_t)c * E];
     std::vector<int> cnt(m + 1, 0);
-    for (int e = 0; e < E; e++) { ir16[(size_t)c * E + e] = (uint16_t)dir[e]; cnt[dir[e] + 1]++; }
+    for (int e = 0; e < E; e++) cnt[dir[e] + 1]++;
     for (int r = 0; r < m; r++) cnt[r + 1] += cnt[r];
     for (int r = 0; r <= m; r++) rowPtr[(size_t)c * (N + 1) + r] = (uint16_t)cnt[r];
+    // rank of each edge within its row, rows listing edges in ascending order
     std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-    for (int e = 0; e < E; e++) rowEdge[(size_t)c * E + fill[dir[e]]++] = (uint16_t)e;  // ascending e
+    std::vector<uint32_t> field(E);
+    for (int e = 0; e < E; e++) {
+      const int rank = fill[dir[e]]++ - cnt[dir[e]];
+      DVC_CHECK(rank <= 0xff, DVC_EIO, "ladder: check degree above 256");
+      field[e] = (uint32_t)dir[e] | ((uint32_t)rank << 13);
+    }
+    for (int k = 0; k < N; k++) {  // every column has degree 3: edges 3k..3k+2
+      const uint32_t f0 = field[3 * k], f1 = field[3 * k + 1], f2 = field[3 * k + 2];
+      colRows[((size_t)c * N + k) * 2] = f0 | (f1 << 21);
+      colRows[((size_t)c * N + k) * 2 + 1] = (f1 >> 11) | (f2 << 10);
+    }
     const int *ix = &L.idx[(size_t)c * L.period];
     for (int k = 0; k < m; k++) tx[(size_t)c * N + k] = (uint16_t)(ix[k % b] + (k / b) * L.period);
   }
-  DVC_HIP(hipMalloc(&L.d_ir16, ir16.size() * 2));
-  DVC_HIP(hipMalloc(&L.d_rowEdge, rowEdge.size() * 2));
+  DVC_HIP(hipMalloc(&L.d_colRows, colRows.size() * 4));
   DVC_HIP(hipMalloc(&L.d_rowPtr, rowPtr.size() * 2));
   DVC_HIP(hipMalloc(&L.d_tx, tx.size() * 2));
   DVC_HIP(hipMalloc(&L.d_syndLength, C * sizeof(int)));
   DVC_HIP(hipMalloc(&L.d_codeRate, C * sizeof(float)));
   DVC_HIP(hipMalloc(&L.d_invH, L.invH.size()));
   DVC_HIP(hipMalloc(&L.d_crcBasis, N));
-  DVC_HIP(hipMemcpy(L.d_ir16, ir16.data(), ir16.size() * 2, hipMemcpyHostToDevice));
-  DVC_HIP(hipMemcpy(L.d_rowEdge, rowEdge.data(), rowEdge.size() * 2, hipMemcpyHostToDevice));
+  DVC_HIP(hipMemcpy(L.d_colRows, colRows.data(), colRows.size() * 4, hipMemcpyHostToDevice));
   DVC_HIP(hipMemcpy(L.d_rowPtr, rowPtr.data(), rowPtr.size() * 2, hipMemcpyHostToDevice));
   DVC_HIP(hipMemcpy(L.d_tx, tx.data(), tx.size() * 2, hipMemcpyHostToDevice));
   DVC_HIP(hipMemcpy(L.d_syndLength, L.syndLength.data(), C * sizeof(int), hipMemcpyHostToDevice));
@@ -176,10 +187,11 @@ int upload_ladder(Ladder &L) {
 }
 
 void free_ladder(Ladder &L) {
-  void *ptrs[] = {L.d_ir16, L.d_rowEdge, L.d_rowPtr, L.d_tx, L.d_syndLength, L.d_codeRate, L.d_invH, L.d_crcBasis};
+  void *ptrs[] = {L.d_colRows, L.d_rowPtr, L.d_tx, L.d_syndLength, L.d_codeRate, L.d_invH, L.d_crcBasis};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  L.d_ir16 = L.d_rowEdge = L.d_rowPtr = L.d_tx = nullptr;
+  L.d_colRows = nullptr;
+  L.d_rowPtr = L.d_tx = nullptr;
   L.d_syndLength = nullptr; L.d_codeRate = nullptr; L.d_invH = L.d_crcBasis = nullptr;
 }
 

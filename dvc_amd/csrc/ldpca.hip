@@ -18,11 +18,12 @@
 //
 // Layout.  One workgroup owns one (bitplane, code) pair.  Messages live in LDS:
 // phi[E] holds |phi| of every variable-to-check message with the message's
-// sign in the sign bit; rowTot[m] holds each check's sum of |phi| with the
-// check's sign product (including the syndrome bit) in the sign bit; the
-// code's row->edge lists are staged into LDS once per item.  Thread t owns
-// columns t, t+T, ... (their three edges are contiguous, edges are stored in
-// column order), so the variable-node update, the next check-node input and
+// sign in the sign bit, stored by check row (row r's edges at rowPtr[r] ..
+// rowPtr[r+1]-1, ascending edge order), so a check sums a contiguous run;
+// rowTot[m] holds each check's sum of |phi| with the check's sign product
+// (including the syndrome bit) in the sign bit.  Thread t owns columns t,
+// t+T, ...; a column's three edges are found from its packed (row, rank in
+// row) fields, so the variable-node update, the next check-node input and
 // the decision are computed in registers without further exchange.  Two
 // barriers per iteration: after the check (row) pass and after the variable
 // (column) pass.  The syndrome test's decision parities are XORed into
@@ -48,6 +49,17 @@
 namespace dvc {
 
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+
+// A column's three edges, packed by the ladder upload as 21-bit fields
+// (row | rank-in-row << 13): x = f0 | f1 << 21, y = f1 >> 11 | f2 << 10.
+__device__ __forceinline__ void col_edges(uint2 c, int row[3], int rank[3]) {
+  const uint32_t f[3] = {c.x & 0x1fffffu, (c.x >> 21) | ((c.y & 0x3ffu) << 11), c.y >> 10};
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    row[q] = (int)(f[q] & 0x1fffu);
+    rank[q] = (int)(f[q] >> 13);
+  }
+}
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 
 // Accumulated-syndrome bit i (MSB-first bytes, ByteToBit LdpcaDecoder.cpp:81-107).
@@ -102,9 +114,8 @@ template <int N, int T>
 __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
   using S = BpShape<N, T>;
   constexpr int E = S::E;
-  __shared__ float phi[E];
+  __shared__ float phi[E];  // by check row
   __shared__ float rowTot[N];
-  __shared__ uint16_t rowEdge[E];
   __shared__ uint16_t rowPtr[N + 2];
   __shared__ uint32_t decBits[S::DEC_WORDS];
   __shared__ uint32_t synBits[S::SYN_WORDS];
@@ -151,11 +162,11 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
     // differ from them and records the one case where that matters
     const bool mode1 = a.mode == 1, exact = level == run;
     const int m = L.syndLength[level];
-    const uint16_t *gir = L.ir16 + (size_t)level * E;
+    const uint2 *gcol = L.colRows + (size_t)level * N;
 
-    // ---- stage the code's row lists and the syndrome (LdpcaDecoder.cpp:684-693)
-    for (int i = tid; i < E; i += T) rowEdge[i] = L.rowEdge[(size_t)level * E + i];
-    for (int i = tid; i <= m; i += T) rowPtr[i] = L.rowPtr[(size_t)level * (N + 1) + i];
+    // ---- stage the code's row starts and the syndrome (LdpcaDecoder.cpp:684-693)
+    const uint16_t *growPtr = L.rowPtr + (size_t)level * (N + 1);
+    for (int i = tid; i <= m; i += T) rowPtr[i] = growPtr[i];
     for (int wd = tid; wd < S::SYN_WORDS; wd += T) {
       uint32_t bits = 0;
       for (int q = 0; q < 32; q++) {
@@ -177,7 +188,10 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         uint32_t s = f2u(v) >> 31;
         float mag = (s ? -1.0f : 1.0f) * v;
         uint32_t f = (f2u(dvc_bp_phi_fast(mag, s_logtab)) & 0x7fffffffu) | (s << 31);
-        phi[3 * k] = u2f(f); phi[3 * k + 1] = u2f(f); phi[3 * k + 2] = u2f(f);
+        int row[3], rank[3];
+        col_edges(gcol[k], row, rank);
+#pragma unroll
+        for (int q = 0; q < 3; q++) phi[growPtr[row[q]] + rank[q]] = u2f(f);
       }
     }
     __syncthreads();
@@ -198,7 +212,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         uint32_t sgn = syn;
         float sum = 0.0f;
         for (int j = e0; j < e1; j++) {
-          const uint32_t f = f2u(phi[rowEdge[j]]);
+          const uint32_t f = f2u(phi[j]);
           sum += u2f(f & 0x7fffffffu);
           sgn ^= f >> 31;
         }
@@ -240,25 +254,36 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
       if (t == a.max_iter) { status = 0; iters = t; break; }
 
       // ---- variable-node pass: c2v, overall LLR, decision, next v2c (:512-548)
+      // The next column's rows and channel LLR are fetched while this
+      // column's six phi evaluations run.
       bool changed = false;
+      uint2 rowsNext = tid < N ? gcol[tid] : make_uint2(0u, 0u);
+      float llrNext = tid < N ? pd.LLR[tid] : 0.0f;
       for (int j = 0; j < S::CPT; j++) {
         const int k = tid + j * T;
+        const uint2 rows = rowsNext;
+        const float llr = llrNext;
+        if (j + 1 < S::CPT && k + T < N) {
+          rowsNext = gcol[k + T];
+          llrNext = pd.LLR[k + T];
+        }
         uint32_t dec = 0;
         if (k < N) {
           float c2v[3];
           uint32_t ph[3];
-          int row[3];
+          int row[3], pos[3];
+          col_edges(rows, row, pos);
+#pragma unroll
+          for (int q = 0; q < 3; q++) pos[q] += rowPtr[row[q]];
 #pragma unroll
           for (int q = 0; q < 3; q++) {
-            const int e = 3 * k + q;
-            row[q] = gir[e];
             const uint32_t R = f2u(rowTot[row[q]]);
-            ph[q] = f2u(phi[e]);
+            ph[q] = f2u(phi[pos[q]]);
             const float x = u2f(R & 0x7fffffffu) - u2f(ph[q] & 0x7fffffffu);
             const float v = dvc_bp_phi_fast(x, s_logtab);
             c2v[q] = ((R ^ ph[q]) >> 31) ? v * -1.0f : v * 1.0f;
           }
-          float tmp = pd.LLR[k];
+          float tmp = llr;
           tmp += c2v[0];
           tmp += c2v[1];
           tmp += c2v[2];
@@ -278,7 +303,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
             const float v2c = tmp - c2v[q];
             const uint32_t s = f2u(v2c) >> 31;
             const float mag = (s ? -1.0f : 1.0f) * v2c;
-            phi[3 * k + q] = u2f((f2u(dvc_bp_phi_fast(mag, s_logtab)) & 0x7fffffffu) | (s << 31));
+            phi[pos[q]] = u2f((f2u(dvc_bp_phi_fast(mag, s_logtab)) & 0x7fffffffu) | (s << 31));
           }
         }
         const uint64_t bal = __ballot(dec);
@@ -505,7 +530,7 @@ hipError_t launch_invh(const PlaneDesc *d_planes, int n, const LadderDev &L, uin
 LadderDev ladder_dev(const Ladder &L) {
   LadderDev d;
   d.N = L.N; d.E = L.E; d.numCodes = L.numCodes;
-  d.ir16 = L.d_ir16; d.rowEdge = L.d_rowEdge; d.rowPtr = L.d_rowPtr; d.tx = L.d_tx;
+  d.colRows = L.d_colRows; d.rowPtr = L.d_rowPtr; d.tx = L.d_tx;
   d.syndLength = L.d_syndLength; d.invH = L.d_invH; d.crcBasis = L.d_crcBasis;
   return d;
 }

@@ -42,7 +42,8 @@ struct BpFrameLink {
 
 struct LadderDev {
   int N, E, numCodes;
-  const uint16_t *ir16, *rowEdge, *rowPtr, *tx;
+  const uint2 *colRows;  // per code and column: its three edges as (row, rank in row), see col_edges
+  const uint16_t *rowPtr, *tx;
   const int *syndLength;
   const uint8_t *invH, *crcBasis;
 };
