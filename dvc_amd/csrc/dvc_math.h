@@ -328,17 +328,16 @@ DVC_MATH_FN float dvc_bp_phi_nonneg(float x, const double *logtab) {
   return -dvc_logf_normal(z, logtab);
 }
 
-// Float quotient a/b as a reciprocal refined by two fused Newton-Raphson
-// corrections (Markstein's sequence) on the device, without the range
-// scaling of the general IEEE sequence.  Used only where both operands are
-// normal and the quotient cannot overflow; dvc_bp_phi_fast's result equals
-// dvc_bp_phi_nonneg's on every argument, which the device self-test checks
-// over all 2^31 non-negative floats.  Host builds divide.
-DVC_MATH_FN float dvcm_div_normal(float a, float b) {
+// Float quotient a/b for dvc_bp_phi_fast: on the device the hardware
+// reciprocal and one fused correction of the quotient (q = a*y, then
+// q + (a - b*q)*y), without the range scaling and reciprocal refinement of
+// the general IEEE sequence.  This is not a correctly rounded division in
+// general; it is used only for phi's two quotients, and phi's result with it
+// equals the reference expression's on every non-negative float (the device
+// self-test sweeps all 2^31 of them).  Host builds divide.
+DVC_MATH_FN float dvcm_div_phi(float a, float b) {
 #if defined(__HIP_DEVICE_COMPILE__)
-  float y = __builtin_amdgcn_rcpf(b);
-  const float e = fmaf(-b, y, 1.0f);
-  y = fmaf(e, y, y);
+  const float y = __builtin_amdgcn_rcpf(b);
   const float q = a * y;
   const float r = fmaf(-b, q, a);
   return fmaf(r, y, q);
@@ -348,12 +347,20 @@ DVC_MATH_FN float dvcm_div_normal(float a, float b) {
 }
 
 // dvc_bp_phi_nonneg with fewer instructions, same value for every x >= 0:
-//   * y is clamped to 22, where tanhf is 1 anyway, so every intermediate is
-//     finite and the reduction's integer conversion needs no clamp;
-//   * expm1f's k = 1 case (a in (0.35, 1.04)) is unreachable for the
-//     arguments 2y >= 2 and -2y > -2 and is dropped; the remaining result
-//     cases share one exponent-adjusted form;
-//   * both quotients use dvcm_div_normal (operands are normal and bounded).
+//   * y is clamped to 9.75: tanhf(y) is 1.0f from y = 9.02 on (2/(e^2y+1)
+//     is below half an ulp of 1), so every larger argument gives the same
+//     result, every intermediate is finite, and expm1f's reduction index k
+//     stays in [-3, 28];
+//   * with k = (int)(a/ln2 +- 0.5) taken on every argument, the reduction
+//     a - k*ln2_hi - k*ln2_lo reproduces expm1f's special cases by itself:
+//     k = 0 leaves a unchanged (the unreduced case, c = 0) and k = -1 is
+//     exactly the hx < 1.5 ln2 case's a + ln2_hi, -ln2_lo;
+//   * k takes the values -3..0 and 3..28; the result cases k = 0, k = -1,
+//     k <= -2 and k >= 3 are all computed and one selected (no branches);
+//     for k = 23..28 (y >= 7.8) the k <= 22 form stands in for expm1f's
+//     k > 22 form, which gives the same tanhf there; |a| < 2^-25 needs no
+//     case of its own (x - (x*e - hxs) is x there);
+//   * both quotients use dvcm_div_phi.
 // Pins a value in a register at this point, so the compiler cannot sink its
 // computation into a branch (device builds; a no-op on the host).
 DVC_MATH_FN float dvcm_pin(float v) {
@@ -369,50 +376,36 @@ DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *logtab) {
               Q1 = -3.3333335072e-02f, Q2 = 1.5873016091e-03f, Q3 = -7.9365076090e-05f,
               Q4 = 4.0082177293e-06f, Q5 = -2.0109921195e-07f;
   const float y0 = (x + (float)0.000000001) / 2.0f;
-  const uint32_t iy = dvcm_asu(y0);
-  const float y = fminf(y0, 22.0f);
-  const int big = iy >= 0x3f800000u;
+  const float y = fminf(y0, 9.75f);
+  const int big = dvcm_asu(y0) >= 0x3f800000u;
   const float a = big ? 2.0f * y : -2.0f * y;
-  // ---- expm1f(a), a in (-2, -1e-9] or [2, 44]
-  const uint32_t hx = dvcm_asu(a) & 0x7fffffffu;
-  const int neg = !big;
-  const int red = hx > 0x3eb17218u;
-  const int near1 = hx < 0x3F851592u;  // only reached with a < 0 here (k = -1)
-  const int kg = (int)(invln2 * a + (neg ? -0.5f : 0.5f));
-  const float tg = (float)kg;
-  const float hi = near1 ? a + ln2_hi : a - tg * ln2_hi;
-  const float lo = near1 ? -ln2_lo : tg * ln2_lo;
-  const int k = red ? (near1 ? -1 : kg) : 0;
-  const float xr = hi - lo;
-  const float c = (hi - xr) - lo;
-  const float xx = red ? xr : a;
+  // ---- expm1f(a), a in (-2, -1e-9] or [2, 19.5]
+  const int k = (int)(invln2 * a + (big ? 0.5f : -0.5f));
+  const float tk = (float)k;
+  const float hi = a - tk * ln2_hi;
+  const float lo = tk * ln2_lo;
+  const float xx = hi - lo;
+  const float c = (hi - xx) - lo;
   const float hfx = 0.5f * xx;
   const float hxs = xx * hfx;
   const float r1 = 1.0f + hxs * (Q1 + hxs * (Q2 + hxs * (Q3 + hxs * (Q4 + hxs * Q5))));
   const float t3 = 3.0f - r1 * hfx;
-  const float e = hxs * dvcm_div_normal(r1 - t3, 6.0f - xx * t3);
+  const float e = hxs * dvcm_div_phi(r1 - t3, 6.0f - xx * t3);
   const float r_k0 = xx - (xx * e - hxs);
   float e2 = (xx * (e - c) - c);
   e2 -= hxs;
   const float r_km1 = 0.5f * (xx - e2) - 0.5f;
-  const int kbig = (k <= -2 || k > 56);
-  const int kc = k < 0 ? 0 : (k > 31 ? 31 : k);
-  // every result case is computed, then one is selected: no branches, so
-  // independent calls can be interleaved
+  const int kneg = k <= -2;
   const float u = e2 - xx;
-  const float Ybig = 1.0f - u;
-  const float Ymid = dvcm_pin(dvcm_asf(0x3f800000u - (0x1000000u >> kc)) - u);
-  const float Yhi = dvcm_pin((xx - (e2 + dvcm_asf((uint32_t)(0x7f - k) << 23))) + 1.0f);
-  const float Y = kbig ? Ybig : (k < 23 ? Ymid : Yhi);
+  const float Yneg = 1.0f - u;
+  const float Ypos = dvcm_pin(dvcm_asf(0x3f800000u - (0x1000000u >> (k & 31))) - u);
+  const float Y = kneg ? Yneg : Ypos;
   const float R = dvcm_asf(dvcm_asu(Y) + ((uint32_t)k << 23));
-  const float Rm1 = R - 1.0f;
-  float em1 = kbig ? Rm1 : R;
+  float em1 = kneg ? R - 1.0f : R;
   em1 = (k == -1) ? r_km1 : em1;
   em1 = (k == 0) ? r_k0 : em1;
-  em1 = (!red && hx < 0x33000000u) ? a : em1;
   // ---- tanhf(y)
-  const float q = dvcm_div_normal(big ? 2.0f : -em1, em1 + 2.0f);
-  float z = big ? 1.0f - q : q;
-  z = iy >= 0x41b00000u ? 1.0f : z;
+  const float q = dvcm_div_phi(big ? 2.0f : -em1, em1 + 2.0f);
+  const float z = big ? 1.0f - q : q;
   return -dvc_logf_normal(z, logtab);
 }

@@ -211,10 +211,15 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         const uint32_t syn = (synBits[r >> 5] >> (r & 31)) & 1u;
         uint32_t sgn = syn;
         float sum = 0.0f;
-        for (int j = e0; j < e1; j++) {
-          const uint32_t f = f2u(phi[j]);
-          sum += u2f(f & 0x7fffffffu);
-          sgn ^= f >> 31;
+        // four loads in flight per step; a slot past the row's end reads as
+        // +0, and sum + 0.0f is sum bit for bit (sum is never -0)
+        for (int j = e0; j < e1; j += 4) {
+          uint32_t f[4];
+#pragma unroll
+          for (int i = 0; i < 4; i++) f[i] = j + i < e1 ? f2u(phi[j + i]) : 0u;
+#pragma unroll
+          for (int i = 0; i < 4; i++) sum += u2f(f[i] & 0x7fffffffu);
+          sgn ^= (f[0] ^ f[1] ^ f[2] ^ f[3]) >> 31;
         }
         rowTot[r] = u2f(f2u(sum) | (sgn << 31));
         if (t > 0 && (syn ^ ((parPrev[r >> 5] >> (r & 31)) & 1u))) fail = 1;
@@ -273,16 +278,21 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
           uint32_t ph[3];
           int row[3], pos[3];
           col_edges(rows, row, pos);
+          uint32_t R[3];
+          float x[3], v[3];
 #pragma unroll
           for (int q = 0; q < 3; q++) pos[q] += rowPtr[row[q]];
 #pragma unroll
           for (int q = 0; q < 3; q++) {
-            const uint32_t R = f2u(rowTot[row[q]]);
+            R[q] = f2u(rowTot[row[q]]);
             ph[q] = f2u(phi[pos[q]]);
-            const float x = u2f(R & 0x7fffffffu) - u2f(ph[q] & 0x7fffffffu);
-            const float v = dvc_bp_phi_fast(x, s_logtab);
-            c2v[q] = ((R ^ ph[q]) >> 31) ? v * -1.0f : v * 1.0f;
           }
+#pragma unroll
+          for (int q = 0; q < 3; q++) x[q] = u2f(R[q] & 0x7fffffffu) - u2f(ph[q] & 0x7fffffffu);
+#pragma unroll
+          for (int q = 0; q < 3; q++) v[q] = dvc_bp_phi_fast(x[q], s_logtab);
+#pragma unroll
+          for (int q = 0; q < 3; q++) c2v[q] = ((R[q] ^ ph[q]) >> 31) ? v[q] * -1.0f : v[q] * 1.0f;
           float tmp = llr;
           tmp += c2v[0];
           tmp += c2v[1];
@@ -298,13 +308,17 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
 #pragma unroll
             for (int q = 0; q < 3; q++) atomicXor(&parBits[t & 1][row[q] >> 5], 1u << (row[q] & 31));
           }
+          uint32_t sg[3];
 #pragma unroll
           for (int q = 0; q < 3; q++) {
             const float v2c = tmp - c2v[q];
-            const uint32_t s = f2u(v2c) >> 31;
-            const float mag = (s ? -1.0f : 1.0f) * v2c;
-            phi[pos[q]] = u2f((f2u(dvc_bp_phi_fast(mag, s_logtab)) & 0x7fffffffu) | (s << 31));
+            sg[q] = f2u(v2c) >> 31;
+            x[q] = (sg[q] ? -1.0f : 1.0f) * v2c;
           }
+#pragma unroll
+          for (int q = 0; q < 3; q++) v[q] = dvc_bp_phi_fast(x[q], s_logtab);
+#pragma unroll
+          for (int q = 0; q < 3; q++) phi[pos[q]] = u2f((f2u(v[q]) & 0x7fffffffu) | (sg[q] << 31));
         }
         const uint64_t bal = __ballot(dec);
         if (lane == 0) {
