@@ -27,6 +27,7 @@
 #include "dvc_internal.h"
 #include "frame_kernels.h"
 #include "ldpca_kernels.h"
+#include "engine_kernels.h"
 
 namespace dvc {
 
@@ -114,22 +115,7 @@ struct Job {
   // cursor
   int bandPos = 0, bit = 0, bp = 0;
   int fixups = 0;                       // reconstruction entries recomputed on the host
-  std::vector<PlaneState> initState;    // upload sources, alive until the frame retires
-  std::vector<int4> initRec;
   bool done() const { return bandPos >= (int)bands.size(); }
-};
-
-struct Arena {
-  std::vector<void *> ptrs;
-  ~Arena() {
-    for (void *p : ptrs) (void)hipFree(p);
-  }
-  template <typename T>
-  hipError_t alloc(T **p, size_t count) {
-    hipError_t e = hipMalloc((void **)p, std::max<size_t>(count, 1) * sizeof(T));
-    if (e == hipSuccess) ptrs.push_back(*p);
-    return e;
-  }
 };
 
 static size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -151,8 +137,8 @@ static size_t job_bytes(const Job &j) {
   return t;
 }
 
-// Carves the per-frame buffers out of one allocation.
-static int alloc_job(Job &j, Arena &arena) {
+// Carves the per-frame buffers out of one block of j.bytes (base).
+static void carve_job(Job &j, uint8_t *base) {
   const int W = j.W, H = j.H, N = j.N, FS = W * H;
   int maxStride = 0;
   for (auto &b : j.bands) maxStride = std::max(maxStride, b.stride);
@@ -167,9 +153,6 @@ static int alloc_job(Job &j, Arena &arena) {
                o_tl = take(4 * (size_t)N), o_acc = take((size_t)j.nbp * (N / 8)), o_state = take(sizeof(PlaneState) * (size_t)j.nbp),
                o_recd = take(sizeof(int4) * 65 * (size_t)j.nbp), o_llr = take(4 * (size_t)N), o_carry = take(N),
                o_bcase = take(4 * 65 * (size_t)j.nbp);
-  uint8_t *base = nullptr;
-  (void)arena;
-  DVC_HIP(hipMalloc((void **)&base, off));
   j.mem = base;
   j.bytes = off;
   FrameDev &f = j.fd;
@@ -183,7 +166,6 @@ static int alloc_job(Job &j, Arena &arena) {
   j.d_LLR = (float *)(base + o_llr);
   j.d_carry = base + o_carry;
   j.d_bcase = (int *)(base + o_bcase);
-  return DVC_OK;
 }
 
 // reconstruction (reconstruction.cpp:184-289) of one coefficient with the C
@@ -348,19 +330,29 @@ struct Lane {
   int cap = 0, groupCap = 0;
   std::vector<Job *> inflight;
   std::vector<Job *> finished;  // retired by the engine after the step
-  FrameDev *d_frames = nullptr;
-  int *d_lists = nullptr;
-  PlaneDesc *d_planes = nullptr;
-  BpFrameLink *d_links = nullptr;
+  // A step's descriptors, written to one pinned host block and uploaded to
+  // its device mirror with one copy: frame descriptors, five frame lists of
+  // cap entries (stepped, starting a band, ending a band, set up, finishing),
+  // BP frame links, plane descriptors, result copy segments (3 per frame).
+  uint8_t *h_stage = nullptr, *d_stage = nullptr;
+  size_t stageBytes = 0;
+  FrameDev *h_frames = nullptr, *d_frames = nullptr;
+  int *h_lists = nullptr, *d_lists = nullptr;
+  BpFrameLink *h_links = nullptr, *d_links = nullptr;
+  PlaneDesc *h_planes = nullptr, *d_planes = nullptr;
+  CopySeg *h_segs = nullptr, *d_segs = nullptr;
+  // results of the step's finishing frames (rate-search states, run records,
+  // reconstructions the host needs), gathered on the device and read back
+  // with one copy; grown on demand
+  uint8_t *h_out = nullptr, *d_out = nullptr;
+  size_t outBytes = 0;
   uint8_t *d_slots = nullptr;
   int *d_counters = nullptr;
   int *d_ties = nullptr;
   int rc = 0;
   std::string err;
-  // host staging of one step, alive until the step's last synchronisation
-  std::vector<FrameDev> hf;
-  std::vector<int> l0, l1, l2, l3, ties;
-  std::vector<BpFrameLink> links;
+  // host-side lists of one step
+  std::vector<int> l0, l1, l2, l3, l4, ties;
   std::vector<PlaneDesc> groups[2];
   // Captured launch sequences of steady steps (no frame starts or ends a
   // band), keyed by their launch shape: frames stepped, planes of each code
@@ -383,6 +375,14 @@ struct dvc_engine {
   std::vector<dvc::Lane> lanes;
   std::map<uint64_t, int> doneTickets;  // ticket -> 1
   std::map<uint64_t, dvc::KeyEntry> keys;
+  // device blocks of retired frames and dropped key frames, kept for reuse
+  // by size (counted in `pooled`, not in `used`)
+  std::multimap<size_t, void *> pool;
+  size_t pooled = 0;
+  // admission descriptors: frame initialisations then key frames, one
+  // pinned block and its device mirror
+  uint8_t *h_admit = nullptr, *d_admit = nullptr;
+  size_t admitBytes = 0, admitKeysOff = 0;
   int inflight() const {
     int n = 0;
     for (auto &l : lanes) n += (int)l.inflight.size();
@@ -392,8 +392,40 @@ struct dvc_engine {
 
 namespace dvc {
 
+// Device block of `bytes` from the pool (a retired block of that size) or a
+// new allocation.
+static int pool_alloc(dvc_engine *e, size_t bytes, void **p) {
+  auto it = e->pool.find(bytes);
+  if (it != e->pool.end()) {
+    *p = it->second;
+    e->pool.erase(it);
+    e->pooled -= bytes;
+    return DVC_OK;
+  }
+  DVC_HIP(hipMalloc(p, bytes));
+  return DVC_OK;
+}
+
+// Keeps a block whose last user has finished (its stream was synchronised).
+static void pool_free(dvc_engine *e, void *p, size_t bytes) {
+  if (!p) return;
+  e->pool.emplace(bytes, p);
+  e->pooled += bytes;
+}
+
+// Frees pooled blocks while live, pooled and needed bytes exceed the budget.
+static void pool_trim(dvc_engine *e, size_t need) {
+  while (!e->pool.empty() && e->used + e->pooled + need > e->budget) {
+    auto it = std::prev(e->pool.end());
+    (void)hipFree(it->second);
+    e->pooled -= it->first;
+    e->pool.erase(it);
+  }
+}
+
 static void free_job(dvc_engine *e, Job *j) {
-  if (j->mem) (void)hipFree(j->mem);
+  pool_free(e, j->mem, j->bytes);
+  j->mem = nullptr;
   e->used -= j->bytes;
   for (uint64_t k : {j->pastKey, j->futKey}) {
     auto it = e->keys.find(k);
@@ -409,7 +441,7 @@ static void trim_keys(dvc_engine *e, size_t need) {
   for (auto it = e->keys.begin(); it != e->keys.end();) {
     if (it->second.refs == 0 && !it->second.named) {
       KeyFrameDev &k = it->second.k;
-      (void)hipFree(k.full);
+      pool_free(e, k.full, it->second.bytes);
       e->used -= it->second.bytes;
       it = e->keys.erase(it);
     } else {
@@ -421,14 +453,17 @@ static void trim_keys(dvc_engine *e, size_t need) {
     for (auto it = e->keys.begin(); it != e->keys.end(); ++it)
       if (it->second.refs == 0 && (victim == e->keys.end() || it->second.lastUse < victim->second.lastUse)) victim = it;
     if (victim == e->keys.end()) break;
-    (void)hipFree(victim->second.k.full);
+    pool_free(e, victim->second.k.full, victim->second.bytes);
     e->used -= victim->second.bytes;
     e->keys.erase(victim);
   }
 }
 
+// A key frame for a frame being admitted: the resident one under `key`, or
+// a new one whose filtering is queued in `jobs` (device requests are read in
+// place; host requests are copied to the device first).
 static int key_frame(dvc_engine *e, uint64_t key, bool named, const uint8_t *src, int W, int H, hipStream_t s,
-                     KeyFrameDev *out) {
+                     KeyFrameDev *out, std::vector<KeyFrameJob> &jobs) {
   auto it = e->keys.find(key);
   if (it != e->keys.end()) {
     DVC_CHECK(it->second.W == W && it->second.H == H, DVC_EINVAL, "key frame reused with another size");
@@ -441,23 +476,27 @@ static int key_frame(dvc_engine *e, uint64_t key, bool named, const uint8_t *src
   const size_t FS = (size_t)W * H;
   ke.bytes = 6 * FS;
   uint8_t *p = nullptr;
-  DVC_HIP(hipMalloc((void **)&p, ke.bytes));
+  int rc = pool_alloc(e, ke.bytes, (void **)&p);
+  if (rc) return rc;
   ke.k.full = p; ke.k.lp = p + FS; ke.k.half = p + 2 * FS;
   ke.W = W; ke.H = H; ke.refs = 1; ke.named = named; ke.lastUse = ++e->clock;
-  DVC_HIP(hipMemcpyAsync(ke.k.full, src, FS, e->deviceBuffers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
-  TIMED_LAUNCH(e->c, "keyframe_filters", s, launch_keyframe_filters(ke.k.full, ke.k.lp, ke.k.half, W, H, s));
+  if (!e->deviceBuffers) DVC_HIP(hipMemcpyAsync(ke.k.full, src, FS, hipMemcpyHostToDevice, s));
+  jobs.push_back(KeyFrameJob{e->deviceBuffers ? src : ke.k.full, ke.k.full, ke.k.lp, ke.k.half, W, H});
   e->used += ke.bytes;
   e->keys[key] = ke;
   *out = ke.k;
   return DVC_OK;
 }
 
-// Allocates a frame's device state and uploads its inputs on stream s (the
-// caller synchronises s before any lane uses the frame).
-static int admit(dvc_engine *e, Job *j, hipStream_t s) {
-  Arena unused;
-  int rc = alloc_job(*j, unused);
+// Gives a frame its device state and queues its initialisation (`inits`)
+// and new key frames (`keyJobs`) for the admission's batched launches on
+// stream s; the caller synchronises s before any lane uses the frame.
+static int admit(dvc_engine *e, Job *j, hipStream_t s, std::vector<JobInitDev> &inits,
+                 std::vector<KeyFrameJob> &keyJobs) {
+  void *mem = nullptr;
+  int rc = pool_alloc(e, j->bytes, &mem);
   if (rc) return rc;
+  carve_job(*j, (uint8_t *)mem);
   e->used += j->bytes;
   FrameDev &f = j->fd;
   const dvc_wz_frame &sp = j->spec;
@@ -466,50 +505,47 @@ static int admit(dvc_engine *e, Job *j, hipStream_t s) {
   f.iFC = (double)(sp.fut_idx - sp.cur_idx) / (double)(sp.fut_idx - sp.past_idx);
   KeyFrameDev kp, kf;
   const bool pn = j->pastKey != (uint64_t)(uintptr_t)sp.past, fn = j->futKey != (uint64_t)(uintptr_t)sp.fut;
-  if ((rc = key_frame(e, j->pastKey, pn, sp.past, j->W, j->H, s, &kp))) return rc;
-  if ((rc = key_frame(e, j->futKey, fn, sp.fut, j->W, j->H, s, &kf))) return rc;
+  if ((rc = key_frame(e, j->pastKey, pn, sp.past, j->W, j->H, s, &kp, keyJobs))) return rc;
+  if ((rc = key_frame(e, j->futKey, fn, sp.fut, j->W, j->H, s, &kf, keyJobs))) return rc;
   f.past = kp.full; f.past_lp = kp.lp; f.past_half = kp.half;
   f.fut = kf.full; f.fut_lp = kf.lp; f.fut_half = kf.half;
-  if (e->deviceBuffers)
-    DVC_HIP(hipMemcpyAsync(j->d_acc, j->spec.acc, (size_t)j->nbp * (j->N / 8), hipMemcpyDeviceToDevice, s));
-  else
-    DVC_HIP(hipMemcpyAsync(j->d_acc, j->acc.data(), j->acc.size(), hipMemcpyHostToDevice, s));
-  DVC_HIP(hipMemsetAsync(f.reconsTrans, 0, sizeof(int) * j->W * j->H, s));
-  // per-bitplane rate-search state: start level from the entropy (computed on
-  // the device), no accepted code yet, and no record of any code run
-  j->initState.assign(j->nbp, PlaneState{-1, 64, 0.0f, 0});
-  j->initRec.assign((size_t)65 * j->nbp, make_int4(0, REC_NOT_RUN, -1, 0));
-  DVC_HIP(hipMemcpyAsync(j->d_state, j->initState.data(), sizeof(PlaneState) * j->nbp, hipMemcpyHostToDevice, s));
-  DVC_HIP(hipMemcpyAsync(j->d_rec, j->initRec.data(), j->initRec.size() * sizeof(int4), hipMemcpyHostToDevice, s));
-  // ParallelMode 1: the reference reads its per-frame decoded buffer before
-  // anything is written to it (DVC_Decoder.cpp:124); bytes that are neither
-  // 0 nor 1 (as in its golden runs, MALLOC_PERTURB_ fill 0xAA) never match
-  DVC_HIP(hipMemsetAsync(j->d_carry, 0xAA, j->N, s));
+  if (!e->deviceBuffers) DVC_HIP(hipMemcpyAsync(j->d_acc, j->acc.data(), j->acc.size(), hipMemcpyHostToDevice, s));
+  // k_job_init: no decoded bits; per-bitplane rate-search state with the
+  // start level from the entropy (computed on the device), no accepted code
+  // and no record of any code run; ParallelMode 1's carried decisions 0xAA
+  // (the reference reads its per-frame decoded buffer before anything is
+  // written to it, DVC_Decoder.cpp:124, and in its golden runs that buffer
+  // holds the MALLOC_PERTURB_ fill 0xAA, which never matches a decision)
+  inits.push_back(JobInitDev{f.reconsTrans, j->W * j->H, j->d_carry, j->N, j->d_state, j->d_rec, j->nbp,
+                             e->deviceBuffers ? j->spec.acc : nullptr, j->d_acc, j->nbp * (j->N / 8)});
   j->bandPos = 0;
   j->bit = j->bands.empty() ? 0 : j->bands[0].qbits;
   j->bp = 0;
   return DVC_OK;
 }
 
-static int finish_job(dvc_engine *e, Job &j, hipStream_t s) {
+// Bytes of a finishing frame's gathered results: rate-search states, run
+// records and, when the host needs it, the reconstruction.
+static bool job_needs_host_recons(const dvc_engine *e, const Job &j) {
+  return !e->deviceBuffers || (j.spec.orig && j.spec.stats);
+}
+static size_t job_out_bytes(const dvc_engine *e, const Job &j) {
+  return align_up(sizeof(PlaneState) * j.nbp) + align_up(sizeof(int4) * 65 * (size_t)j.nbp) +
+         (job_needs_host_recons(e, j) ? align_up((size_t)j.W * j.H) : 0);
+}
+
+// A finished frame's outputs from its gathered results `out` (host copy, in
+// the layout of job_out_bytes; a device request's reconstruction was copied
+// to the caller's buffer on the device).
+static int finish_job(dvc_engine *e, Job &j, const uint8_t *out) {
   dvc_ctx *c = e->c;
   const size_t FS = (size_t)j.W * j.H;
-  std::vector<uint8_t> hostRec;  // device mode: the frame on the host for the PSNR
-  if (e->deviceBuffers) {
-    DVC_HIP(hipMemcpyAsync(j.spec.recons, j.fd.recons, FS, hipMemcpyDeviceToDevice, s));
-    if (j.spec.orig && j.spec.stats) {
-      hostRec.resize(FS);
-      DVC_HIP(hipMemcpyAsync(hostRec.data(), j.fd.recons, FS, hipMemcpyDeviceToHost, s));
-    }
-  } else {
-    DVC_HIP(hipMemcpyAsync(j.spec.recons, j.fd.recons, FS, hipMemcpyDeviceToHost, s));
-  }
-  const uint8_t *rec8 = e->deviceBuffers ? hostRec.data() : j.spec.recons;
-  std::vector<PlaneState> ps(j.nbp);
-  std::vector<int4> rec((size_t)65 * j.nbp);
-  DVC_HIP(hipMemcpyAsync(ps.data(), j.d_state, sizeof(PlaneState) * j.nbp, hipMemcpyDeviceToHost, s));
-  DVC_HIP(hipMemcpyAsync(rec.data(), j.d_rec, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost, s));
-  DVC_HIP(hipStreamSynchronize(s));
+  const PlaneState *ps = (const PlaneState *)out;
+  const int4 *rec = (const int4 *)(out + align_up(sizeof(PlaneState) * j.nbp));
+  const uint8_t *rec8 = job_needs_host_recons(e, j)
+                            ? out + align_up(sizeof(PlaneState) * j.nbp) + align_up(sizeof(int4) * 65 * (size_t)j.nbp)
+                            : nullptr;
+  if (!e->deviceBuffers) memcpy(j.spec.recons, rec8, FS);
   dvc_frame_stats *st = j.spec.stats;
   if (!st) return DVC_OK;
   memset(st, 0, sizeof(*st));
@@ -553,8 +589,6 @@ static int lane_step(dvc_engine *e, Lane &ln) {
   int rc;
   const int n = (int)ln.inflight.size();
   if (n == 0) return DVC_OK;
-  ln.hf.resize(n);
-  ln.l0.clear(); ln.l1.clear(); ln.l2.clear(); ln.l3.clear();
   int maxN = 0, maxMB = 0, maxPos = 0;
   for (int i = 0; i < n; i++) {
     Job *j = ln.inflight[i];
@@ -564,16 +598,11 @@ static int lane_step(dvc_engine *e, Lane &ln) {
     j->fd.tieCount = ln.d_ties + i;
   }
   CostTables ct{c->d_costFme, c->d_costUsi};
-  const int *L0 = ln.d_lists, *L1 = ln.d_lists + ln.cap, *L2 = ln.d_lists + 2 * ln.cap, *L3 = ln.d_lists + 3 * ln.cap;
-  auto up_list = [&](const int *dst, const std::vector<int> &l) -> int {
-    DVC_HIP(hipMemcpyAsync((void *)dst, l.data(), sizeof(int) * l.size(), hipMemcpyHostToDevice, s));
-    return DVC_OK;
-  };
-  // ---- setup of newly admitted frames: SI, noise parameters, OCT, SMF
-  std::vector<int> &hl3 = ln.l3, &active = ln.l0, &starting = ln.l1, &ending = ln.l2;
+  // ---- the step's frame lists (all known before any kernel runs)
+  std::vector<int> &active = ln.l0, &starting = ln.l1, &ending = ln.l2, &setup = ln.l3, &fin = ln.l4;
+  for (auto *l : {&active, &starting, &ending, &setup, &fin}) l->clear();
   for (int i = 0; i < n; i++)
-    if (!ln.inflight[i]->setup) hl3.push_back(i);
-  // band parameters of this step
+    if (!ln.inflight[i]->setup) setup.push_back(i);
   for (int i = 0; i < n; i++) {
     Job &j = *ln.inflight[i];
     if (j.done()) continue;
@@ -587,45 +616,84 @@ static int lane_step(dvc_engine *e, Lane &ln) {
     active.push_back(i);
     if (j.bit == 1) ending.push_back(i);
   }
-  for (int i = 0; i < n; i++) ln.hf[i] = ln.inflight[i]->fd;
-  DVC_HIP(hipMemcpyAsync(ln.d_frames, ln.hf.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, s));
-  if (!hl3.empty()) {
-    const int nf = (int)hl3.size();
-    if ((rc = up_list(L3, hl3))) return rc;
+  // frames whose last band ends in this step (or that have none)
+  for (int i = 0; i < n; i++) {
+    const Job &j = *ln.inflight[i];
+    if (j.done() || (j.bit == 1 && j.bandPos + 1 == (int)j.bands.size())) fin.push_back(i);
+  }
+  const int M = ln.cap;
+  const int *L0 = ln.d_lists, *L1 = ln.d_lists + M, *L2 = ln.d_lists + 2 * M, *L3 = ln.d_lists + 3 * M,
+            *L4 = ln.d_lists + 4 * M;
+  {
+    int k = 0;
+    for (auto *l : {&active, &starting, &ending, &setup, &fin}) {
+      if (!l->empty()) memcpy(ln.h_lists + (size_t)k * M, l->data(), sizeof(int) * l->size());
+      k++;
+    }
+  }
+  for (int i = 0; i < n; i++) ln.h_frames[i] = ln.inflight[i]->fd;
+  // ---- LDPCA descriptors of the stepped frames
+  ln.groups[0].clear();
+  ln.groups[1].clear();
+  const bool m1 = c->bp_mode == 1;
+  for (size_t a = 0; a < active.size(); a++) {
+    Job &j = *ln.inflight[active[a]];
+    PlaneDesc pd{j.N, j.fd.LR, j.d_LLR, j.d_acc + (size_t)j.bp * (j.N / 8), (uint32_t)j.crc[j.bp],
+                 j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp, m1 ? j.d_carry : nullptr,
+                 m1 ? j.d_bcase + (size_t)65 * j.bp : nullptr};
+    ln.groups[j.N == 6336].push_back(pd);
+    ln.h_links[a] = BpFrameLink{j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp,
+                                ln.d_slots + (size_t)(j.N == 6336 ? ln.groupCap : 0) * 6336, ln.groupCap,
+                                m1 ? j.d_carry : nullptr};
+  }
+  const int na = (int)active.size(), n0 = (int)ln.groups[0].size(), n1 = (int)ln.groups[1].size();
+  if (n0) memcpy(ln.h_planes, ln.groups[0].data(), sizeof(PlaneDesc) * n0);
+  if (n1) memcpy(ln.h_planes + n0, ln.groups[1].data(), sizeof(PlaneDesc) * n1);
+  // ---- result gathering of the finishing frames: copy segments into the
+  // lane's output block (and, for device requests, to the caller's buffer)
+  size_t outUsed = 0;
+  std::vector<size_t> outOff(fin.size());
+  for (size_t k = 0; k < fin.size(); k++) {
+    outOff[k] = outUsed;
+    outUsed += job_out_bytes(e, *ln.inflight[fin[k]]);
+  }
+  if (outUsed > ln.outBytes) {
+    // the lane's stream is idle here (synchronised at the end of every step)
+    if (ln.d_out) (void)hipFree(ln.d_out);
+    if (ln.h_out) (void)hipHostFree(ln.h_out);
+    ln.d_out = ln.h_out = nullptr;
+    ln.outBytes = 0;
+    const size_t want = outUsed + outUsed / 4;
+    DVC_HIP(hipMalloc((void **)&ln.d_out, want));
+    DVC_HIP(hipHostMalloc((void **)&ln.h_out, want, hipHostMallocDefault));
+    ln.outBytes = want;
+  }
+  int nseg = 0;
+  for (size_t k = 0; k < fin.size(); k++) {
+    const Job &j = *ln.inflight[fin[k]];
+    uint8_t *o = ln.d_out + outOff[k];
+    const size_t bs = sizeof(PlaneState) * j.nbp, br = sizeof(int4) * 65 * (size_t)j.nbp, FS = (size_t)j.W * j.H;
+    ln.h_segs[nseg++] = CopySeg{(const uint8_t *)j.d_state, o, bs};
+    ln.h_segs[nseg++] = CopySeg{(const uint8_t *)j.d_rec, o + align_up(bs), br};
+    if (job_needs_host_recons(e, j)) ln.h_segs[nseg++] = CopySeg{j.fd.recons, o + align_up(bs) + align_up(br), FS};
+    if (e->deviceBuffers) ln.h_segs[nseg++] = CopySeg{j.fd.recons, j.spec.recons, FS};
+  }
+  // ---- one upload of the step's descriptors
+  DVC_HIP(hipMemcpyAsync(ln.d_stage, ln.h_stage, ln.stageBytes, hipMemcpyHostToDevice, s));
+  // ---- setup of newly admitted frames: SI, noise parameters, OCT, SMF
+  if (!setup.empty()) {
+    const int nf = (int)setup.size();
     TIMED_LAUNCH(c, "create_si", s, launch_create_si(ln.d_frames, L3, nf, maxMB, ct, s));
     TIMED_LAUNCH(c, "noise_param", s, launch_noise_param(ln.d_frames, L3, nf, maxN, s));
     TIMED_LAUNCH(c, "oct", s, launch_oct(ln.d_frames, L3, nf, maxPos, 1, s));
     TIMED_LAUNCH(c, "smf_init", s, launch_smf_init(ln.d_frames, L3, nf, maxN, s));
-    for (int i : hl3) ln.inflight[i]->setup = true;
+    for (int i : setup) ln.inflight[i]->setup = true;
   }
   if (!active.empty()) {
     if (!starting.empty()) {
       int maxStride = 0;
       for (int i : starting) maxStride = std::max(maxStride, ln.inflight[i]->fd.distStride);
-      if ((rc = up_list(L1, starting))) return rc;
       TIMED_LAUNCH(c, "noise_dist", s, launch_noise_dist(ln.d_frames, L1, (int)starting.size(), maxN, maxStride, s));
-    }
-    if ((rc = up_list(L0, active))) return rc;
-    ln.groups[0].clear();
-    ln.groups[1].clear();
-    ln.links.resize(active.size());
-    for (size_t a = 0; a < active.size(); a++) {
-      Job &j = *ln.inflight[active[a]];
-      const bool m1 = c->bp_mode == 1;
-      PlaneDesc pd{j.N, j.fd.LR, j.d_LLR, j.d_acc + (size_t)j.bp * (j.N / 8), (uint32_t)j.crc[j.bp],
-                   j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp, m1 ? j.d_carry : nullptr,
-                   m1 ? j.d_bcase + (size_t)65 * j.bp : nullptr};
-      ln.groups[j.N == 6336].push_back(pd);
-      ln.links[a] = BpFrameLink{j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp,
-                                ln.d_slots + (size_t)(j.N == 6336 ? ln.groupCap : 0) * 6336, ln.groupCap,
-                                m1 ? j.d_carry : nullptr};
-    }
-    const int na = (int)active.size(), n0 = (int)ln.groups[0].size(), n1 = (int)ln.groups[1].size();
-    DVC_HIP(hipMemcpyAsync(ln.d_links, ln.links.data(), sizeof(BpFrameLink) * na, hipMemcpyHostToDevice, s));
-    if (!e->sequentialBp) {
-      if (n0) DVC_HIP(hipMemcpyAsync(ln.d_planes, ln.groups[0].data(), sizeof(PlaneDesc) * n0, hipMemcpyHostToDevice, s));
-      if (n1)
-        DVC_HIP(hipMemcpyAsync(ln.d_planes + n0, ln.groups[1].data(), sizeof(PlaneDesc) * n1, hipMemcpyHostToDevice, s));
     }
     // the step's kernels: conditional bit probabilities, LDPCA per code
     // length, bitplane insertion
@@ -648,7 +716,7 @@ static int lane_step(dvc_engine *e, Lane &ln) {
       return DVC_OK;
     };
     // (ParallelMode 1 steps read a resolution count back, so they are not captured)
-    const bool steady = e->graphs && !e->sequentialBp && !c->profiling && c->bp_mode == 0 && hl3.empty() &&
+    const bool steady = e->graphs && !e->sequentialBp && !c->profiling && c->bp_mode == 0 && setup.empty() &&
                         starting.empty() && ending.empty();
     if (steady) {
       const auto key = std::make_tuple(na, n0, n1, maxN);
@@ -679,7 +747,6 @@ static int lane_step(dvc_engine *e, Lane &ln) {
     }
     if (!ending.empty()) {
       const int ne = (int)ending.size();
-      if ((rc = up_list(L2, ending))) return rc;
       DVC_HIP(hipMemsetAsync(ln.d_ties, 0, sizeof(int) * n, s));
       TIMED_LAUNCH(c, "reconstruct", s, launch_reconstruct(ln.d_frames, L2, ne, maxN, s));
       ln.ties.assign(n, 0);
@@ -718,16 +785,18 @@ static int lane_step(dvc_engine *e, Lane &ln) {
       }
     }
   }
-  // ---- retire: final inverse transform (DVC_Decoder.cpp:411-423) and outputs
-  std::vector<int> fin;
-  for (int i = 0; i < n; i++)
-    if (ln.inflight[i]->setup && ln.inflight[i]->done()) fin.push_back(i);
+  // ---- retire: final inverse transform (DVC_Decoder.cpp:411-423), results
+  // gathered on the device and read back with one copy, then outputs
   if (!fin.empty()) {
-    ln.l3 = fin;
-    if ((rc = up_list(L3, ln.l3))) return rc;
-    TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(ln.d_frames, L3, (int)fin.size(), maxN, s));
-    for (int i : fin)
-      if ((rc = finish_job(e, *ln.inflight[i], s))) return rc;
+    TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(ln.d_frames, L4, (int)fin.size(), maxN, s));
+    DVC_HIP(launch_copy_segments(ln.d_segs, nseg, s));
+    DVC_HIP(hipMemcpyAsync(ln.h_out, ln.d_out, outUsed, hipMemcpyDeviceToHost, s));
+    DVC_HIP(hipStreamSynchronize(s));
+    for (size_t k = 0; k < fin.size(); k++) {
+      Job &j = *ln.inflight[fin[k]];
+      DVC_CHECK(j.setup && j.done(), DVC_ESTATE, "engine: a finishing frame has bands left");
+      if ((rc = finish_job(e, j, ln.h_out + outOff[k]))) return rc;
+    }
     std::vector<Job *> keep;
     for (int i = 0; i < n; i++) {
       Job *j = ln.inflight[i];
@@ -749,6 +818,8 @@ static int engine_step(dvc_engine *e, int *completed) {
   int rc;
   *completed = 0;
   bool admitted = false;
+  std::vector<JobInitDev> inits;
+  std::vector<KeyFrameJob> keyJobs;
   while (!e->pending.empty() && e->inflight() < e->maxInflight) {
     Job *j = e->pending.front();
     Lane *dst = nullptr;
@@ -757,13 +828,30 @@ static int engine_step(dvc_engine *e, int *completed) {
     if (!dst) break;
     const size_t keyNeed = 12 * (size_t)j->W * j->H;
     trim_keys(e, j->bytes + keyNeed);
+    pool_trim(e, j->bytes + keyNeed);
     if (e->inflight() > 0 && e->used + j->bytes + keyNeed > e->budget) break;
     e->pending.pop_front();
-    if ((rc = admit(e, j, c->stream))) { delete j; return rc; }
+    if ((rc = admit(e, j, c->stream, inits, keyJobs))) { delete j; return rc; }
     dst->inflight.push_back(j);
     admitted = true;
   }
-  if (admitted) DVC_HIP(hipStreamSynchronize(c->stream));
+  if (admitted) {
+    // one upload of the admission's descriptors, then the batched key-frame
+    // filters and frame initialisation
+    const size_t bi = sizeof(JobInitDev) * inits.size(), bk = sizeof(KeyFrameJob) * keyJobs.size();
+    DVC_CHECK(bi <= e->admitKeysOff && e->admitKeysOff + bk <= e->admitBytes, DVC_ESTATE,
+              "engine: admission staging too small");
+    if (bi) memcpy(e->h_admit, inits.data(), bi);
+    if (bk) memcpy(e->h_admit + e->admitKeysOff, keyJobs.data(), bk);
+    DVC_HIP(hipMemcpyAsync(e->d_admit, e->h_admit, e->admitBytes, hipMemcpyHostToDevice, c->stream));
+    int maxW = 0, maxH = 0;
+    for (const KeyFrameJob &k : keyJobs) { maxW = std::max(maxW, k.W); maxH = std::max(maxH, k.H); }
+    TIMED_LAUNCH(c, "keyframe_filters", c->stream,
+                 launch_keyframe_batch((const KeyFrameJob *)(e->d_admit + e->admitKeysOff), (int)keyJobs.size(), maxW,
+                                       maxH, c->stream));
+    DVC_HIP(launch_job_init((const JobInitDev *)e->d_admit, (int)inits.size(), c->stream));
+    DVC_HIP(hipStreamSynchronize(c->stream));
+  }
   std::vector<Lane *> busy;
   for (auto &l : e->lanes)
     if (!l.inflight.empty()) busy.push_back(&l);
@@ -836,17 +924,31 @@ static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **ou
   int nl = (p && p->lanes > 0) ? p->lanes : std::min(4, std::max(1, e->maxInflight / 16));
   nl = std::min(nl, e->maxInflight);
   e->lanes.resize(nl);
-  hipError_t err = hipSuccess;
+  // admission staging: frame initialisations, then up to two key frames per frame
+  e->admitKeysOff = align_up(sizeof(JobInitDev) * (size_t)e->maxInflight);
+  e->admitBytes = e->admitKeysOff + align_up(sizeof(KeyFrameJob) * 2 * (size_t)e->maxInflight);
+  hipError_t err = hipHostMalloc((void **)&e->h_admit, e->admitBytes, hipHostMallocDefault);
+  if (err == hipSuccess) err = hipMalloc((void **)&e->d_admit, e->admitBytes);
   for (int k = 0; k < nl && err == hipSuccess; k++) {
     Lane &l = e->lanes[k];
     l.cap = (e->maxInflight + nl - 1) / nl;
     l.groupCap = (c->bp_mode == 1 ? 130 : 65) * l.cap + 1;  // ParallelMode 1 keeps two decision sets per code
-    const int M = l.cap;
+    const size_t M = l.cap;
+    const size_t oFrames = 0, oLists = align_up(oFrames + sizeof(FrameDev) * M), oLinks = align_up(oLists + sizeof(int) * 5 * M),
+                 oPlanes = align_up(oLinks + sizeof(BpFrameLink) * M), oSegs = align_up(oPlanes + sizeof(PlaneDesc) * M),
+                 total = align_up(oSegs + sizeof(CopySeg) * 4 * M);
     err = hipStreamCreateWithFlags(&l.s, hipStreamNonBlocking);
-    if (err == hipSuccess) err = hipMalloc((void **)&l.d_frames, sizeof(FrameDev) * M);
-    if (err == hipSuccess) err = hipMalloc((void **)&l.d_lists, sizeof(int) * 4 * (size_t)M);
-    if (err == hipSuccess) err = hipMalloc((void **)&l.d_planes, sizeof(PlaneDesc) * M);
-    if (err == hipSuccess) err = hipMalloc((void **)&l.d_links, sizeof(BpFrameLink) * M);
+    if (err == hipSuccess) err = hipHostMalloc((void **)&l.h_stage, total, hipHostMallocDefault);
+    if (err == hipSuccess) err = hipMalloc((void **)&l.d_stage, total);
+    if (err == hipSuccess) {
+      l.stageBytes = total;
+      memset(l.h_stage, 0, total);
+      l.h_frames = (FrameDev *)(l.h_stage + oFrames); l.d_frames = (FrameDev *)(l.d_stage + oFrames);
+      l.h_lists = (int *)(l.h_stage + oLists); l.d_lists = (int *)(l.d_stage + oLists);
+      l.h_links = (BpFrameLink *)(l.h_stage + oLinks); l.d_links = (BpFrameLink *)(l.d_stage + oLinks);
+      l.h_planes = (PlaneDesc *)(l.h_stage + oPlanes); l.d_planes = (PlaneDesc *)(l.d_stage + oPlanes);
+      l.h_segs = (CopySeg *)(l.h_stage + oSegs); l.d_segs = (CopySeg *)(l.d_stage + oSegs);
+    }
     if (err == hipSuccess) err = hipMalloc((void **)&l.d_slots, (size_t)2 * l.groupCap * 6336);
     if (err == hipSuccess) err = hipMalloc((void **)&l.d_counters, 6 * sizeof(int));
     if (err == hipSuccess) err = hipMalloc((void **)&l.d_ties, sizeof(int) * M);
@@ -871,12 +973,17 @@ static void engine_destroy(dvc_engine *e) {
     for (Job *j : l.inflight) { if (j->mem) (void)hipFree(j->mem); delete j; }
     for (Job *j : l.finished) { if (j->mem) (void)hipFree(j->mem); delete j; }
     for (auto &g : l.graphs) (void)hipGraphExecDestroy(g.second);
-    void *ptrs[] = {l.d_frames, l.d_lists, l.d_planes, l.d_links, l.d_slots, l.d_counters, l.d_ties};
+    void *ptrs[] = {l.d_stage, l.d_out, l.d_slots, l.d_counters, l.d_ties};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
+    if (l.h_stage) (void)hipHostFree(l.h_stage);
+    if (l.h_out) (void)hipHostFree(l.h_out);
     if (l.s) (void)hipStreamDestroy(l.s);
   }
   for (auto &kv : e->keys) (void)hipFree(kv.second.k.full);
+  for (auto &kv : e->pool) (void)hipFree(kv.second);
+  if (e->d_admit) (void)hipFree(e->d_admit);
+  if (e->h_admit) (void)hipHostFree(e->h_admit);
   delete e;
 }
 

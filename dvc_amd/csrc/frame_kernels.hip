@@ -12,6 +12,7 @@
 #include "dvc_math.h"
 #include "frame_kernels.h"
 #include "ldpca_kernels.h"
+#include "engine_kernels.h"
 
 #pragma clang fp contract(off)
 
@@ -48,10 +49,14 @@ __device__ __forceinline__ void wave_argmin(double &cost, int &idx) {
 // FIR_filter (filter.cpp:55-117): the 6-tap (1,-5,20,20,-5,1)/32 half-pel
 // filter; its dyadic coefficients make (int)(sum + 0.5) then the [0,255]
 // clip equal to clip((S + 16) >> 5) (1-D) and clip((S + 512) >> 10) (2-D).
-__global__ void k_keyframe_filters(const uint8_t *in, uint8_t *lp, uint8_t *half, int W, int H) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x, i = blockIdx.y;
+// Thread (i, j) of a grid of rows i < rowsY (rowsY >= H) and columns j:
+// low-pass sample (i, j) and the half-pel samples of rows i, i + rowsY, ...
+// of the 2H x 2W image; with copyTo set it also copies in[i][j] there.
+__device__ __forceinline__ void keyframe_filters_at(const uint8_t *in, uint8_t *lp, uint8_t *half, int W, int H,
+                                                    int i, int j, int rowsY, uint8_t *copyTo) {
   const int W2 = 2 * W;
   if (j >= W2) return;
+  if (copyTo && i < H && j < W) copyTo[i * W + j] = in[i * W + j];
   if (i < H && j < W) {
     int sum = 0;
     for (int m = i - 1; m <= i + 1; m++) {
@@ -63,7 +68,7 @@ __global__ void k_keyframe_filters(const uint8_t *in, uint8_t *lp, uint8_t *half
     }
     lp[i * W + j] = (uint8_t)(sum / 9);
   }
-  for (int ii = i; ii < 2 * H; ii += gridDim.y) {
+  for (int ii = i; ii < 2 * H; ii += rowsY) {
     const int sub = (j & 1) + ((ii & 1) << 1), y = ii >> 1, x = j >> 1;
     const int tap[6] = {1, -5, 20, 20, -5, 1};
     int v;
@@ -89,6 +94,18 @@ __global__ void k_keyframe_filters(const uint8_t *in, uint8_t *lp, uint8_t *half
     }
     half[ii * W2 + j] = (uint8_t)imax(0, imin(255, v));
   }
+}
+
+__global__ void k_keyframe_filters(const uint8_t *in, uint8_t *lp, uint8_t *half, int W, int H) {
+  keyframe_filters_at(in, lp, half, W, H, blockIdx.y, blockIdx.x * blockDim.x + threadIdx.x, gridDim.y, nullptr);
+}
+
+// blockIdx.z = key frame; rows and columns span the largest frame.
+__global__ void k_keyframe_batch(const KeyFrameJob *jobs) {
+  const KeyFrameJob kj = jobs[blockIdx.z];
+  const int i = blockIdx.y, j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 2 * kj.H) return;  // rows in [H, 2H) still own half-pel row i
+  keyframe_filters_at(kj.src, kj.lp, kj.half, kj.W, kj.H, i, j, gridDim.y, kj.src != kj.full ? kj.full : nullptr);
 }
 
 // ------------------------------------------------------------ forward ME
@@ -945,6 +962,11 @@ __global__ __launch_bounds__(256) void k_update_si_ml(const FrameDev *F, const i
 // ------------------------------------------------------------ launchers
 hipError_t launch_keyframe_filters(const uint8_t *in, uint8_t *lp, uint8_t *half, int W, int H, hipStream_t s) {
   hipLaunchKernelGGL(k_keyframe_filters, dim3((2 * W + 127) / 128, H), dim3(128), 0, s, in, lp, half, W, H);
+  return hipGetLastError();
+}
+hipError_t launch_keyframe_batch(const KeyFrameJob *jobs, int n, int maxW, int maxH, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_keyframe_batch, dim3((2 * maxW + 127) / 128, maxH, n), dim3(128), 0, s, jobs);
   return hipGetLastError();
 }
 hipError_t launch_create_si(const FrameDev *F, const int *list, int n, int maxMB, CostTables ct, hipStream_t s) {
