@@ -557,23 +557,32 @@ __global__ __launch_bounds__(512) void k_noise_dist(const FrameDev *F, const int
   }
   __syncthreads();
   const int n = s_n;
-  // |k - range - SI_m| <= range + max|SI_m| for k in [0, 2*range]
-  const int need = range + s_maxabs + 1;
-  const int tl = imin(need, ND_TAB);
-  for (int d = tid; d < tl; d += T) s_tab[d] = dvc_expf(-a * (float)d);
-  __syncthreads();
+  // |k - range - SI_m| <= range + max|SI_m| = D for k in [0, 2*range]
+  const int D = range + s_maxabs;
   const int rmax = 2 * range + 1;
-  if (need <= ND_TAB) {
+  if (2 * D + 1 <= ND_TAB) {
+    // two-sided table s_tab[D + t] = expf(-a*|t|), t in [-D, D]: term m of
+    // value k reads s_tab[k + off_m] with off_m = D - range - SI_m
+    for (int d = tid; d <= D; d += T) {
+      const float v = dvc_expf(-a * (float)d);
+      s_tab[D + d] = v;
+      s_tab[D - d] = v;
+    }
+    for (int m = tid; m < n; m += T) s_ps[m].y = D - range - s_ps[m].y;
+    __syncthreads();
     for (int k = tid; k < rmax; k += T) {
-      const int pc = k - range;
       float sum = 0.0f;
       for (int m = 0; m < n; m++) {
         const int2 ps = s_ps[m];
-        sum += __int_as_float(ps.x) * s_tab[abs(pc - ps.y)];
+        sum += __int_as_float(ps.x) * s_tab[k + ps.y];
       }
       out[k] = sum;
     }
   } else {
+    const int need = D + 1;
+    const int tl = imin(need, ND_TAB);
+    for (int d = tid; d < tl; d += T) s_tab[d] = dvc_expf(-a * (float)d);
+    __syncthreads();
     for (int k = tid; k < rmax; k += T) {
       const int pc = k - range;
       float sum = 0.0f;
