@@ -7,11 +7,12 @@ Workloads
                 CIF 352x288, 33 frames, GOP 2, Qindex 8 (16 WZ frames, 63
                 bitplanes each, N = 6336), synthetic sequence as described in
                 dvc_amd.workload.  One step decodes `--streams` such sequences
-                (16 WZ frames each, 16 sequences = 256 frames by default) through
-                the continuous-batching engine (2 lanes) with the speculative
-                persistent LDPCA launch; value = WZ frames/s.  The plain path
-                (one frame at a time, sequential rate search) is timed on one
-                sequence beside it.
+                (16 WZ frames each, 32 sequences = 512 frames by default) through
+                one long-lived continuous-batching engine (2 lanes that step
+                independently) with the speculative persistent LDPCA launch;
+                value = WZ frames/s.  The plain path (one frame at a time,
+                sequential rate search) is timed on one sequence beside it, and
+                the same workload again with ParallelMode 1 BP semantics.
   ldpca_batch   512 seeded bitplanes (384 x N=6336, 128 x N=1584) decoded in
                 one speculative persistent launch per step; one step = the
                 whole set.  value = BP edge-updates/s, counting the
@@ -19,9 +20,12 @@ Workloads
                 one edge's check-to-variable plus variable-to-check update in
                 one iteration; E = 3N per code).
 
+  serve, si_batch, hbm_copy: the serving loop, batched side information and
+                a device copy (the HBM traffic calibration).
+
 Prints one JSON line (see the end of main()).  The CPU baseline leg runs the
-oracle's sequential decoder (ParallelMode 0 semantics, one core) on a bounded
-sample of the same bitplanes.
+oracle (ParallelMode 1 over the host cores, and ParallelMode 0 on one core)
+on a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -189,7 +193,7 @@ def bench_wz(args):
     # one long-lived engine, as a server keeps it: warmup steps leave its
     # device-block pool and captured launch graphs in place; every step
     # brings its key frames in afresh (new keys) and decodes every frame
-    eng = _lib.Engine(ctx, max_inflight=len(jobs), lanes=args.lanes, device_buffers=True)
+    eng = _lib.Engine(ctx, max_inflight=len(jobs), lanes=args.lanes, device_buffers=True, lane_steps=1 << 20)
     gen = 0
     for i in range(args.warmup):
         run_engine(eng, dev.submit_all(eng, jobs, gen))
@@ -267,7 +271,7 @@ def bench_wz_mode1(args, jobs, dev):
     also the semantics of the CPU baseline leg."""
     from dvc_amd import _lib
     ctx = _lib.Context(device=0, max_iter=args.max_iter, bp_threads=args.bp_threads, parallel_mode=1)
-    eng = _lib.Engine(ctx, max_inflight=len(jobs), lanes=args.lanes, device_buffers=True)
+    eng = _lib.Engine(ctx, max_inflight=len(jobs), lanes=args.lanes, device_buffers=True, lane_steps=1 << 20)
     run_engine(eng, dev.submit_all(eng, jobs, 1000))
     times = []
     for i in range(args.mode1_steps):
@@ -494,7 +498,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch", "hbm_copy", "serve", "si_batch"])
-    ap.add_argument("--streams", type=int, default=16)
+    ap.add_argument("--streams", type=int, default=32, help="CIF sequences of 16 WZ frames decoded per step")
     ap.add_argument("--plain-frames", type=int, default=4)
     ap.add_argument("--max-iter", type=int, default=100)
     ap.add_argument("--bp-threads", type=int, default=0, help="threads per CIF BP workgroup (0: library default)")

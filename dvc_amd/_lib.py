@@ -89,7 +89,8 @@ class SIRequest(ctypes.Structure):
 
 class EngineParams(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int), ("sequential_bp", ctypes.c_int),
-                ("lanes", ctypes.c_int), ("eager_steps", ctypes.c_int), ("device_buffers", ctypes.c_int)]
+                ("lanes", ctypes.c_int), ("eager_steps", ctypes.c_int), ("device_buffers", ctypes.c_int),
+                ("lane_steps", ctypes.c_int)]
 
 
 # Every symbol include/dvc_accel.h declares, with its ctypes signature.
@@ -373,9 +374,9 @@ class Engine:
     step, and collect results as their tickets complete."""
 
     def __init__(self, ctx, max_inflight=64, hbm_budget_bytes=0, sequential_bp=False, lanes=0, eager_steps=False,
-                 device_buffers=False):
+                 device_buffers=False, lane_steps=0):
         p = EngineParams(int(hbm_budget_bytes), int(max_inflight), 1 if sequential_bp else 0, int(lanes),
-                         1 if eager_steps else 0, 1 if device_buffers else 0)
+                         1 if eager_steps else 0, 1 if device_buffers else 0, int(lane_steps))
         h = ctypes.c_void_p()
         _check(lib.dvc_engine_create(ctx.h, ctypes.byref(p), ctypes.byref(h)), "dvc_engine_create")
         self.h = h

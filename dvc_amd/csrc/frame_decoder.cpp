@@ -370,6 +370,7 @@ struct dvc_engine {
   bool sequentialBp = false;
   bool graphs = true;  // replay captured steady steps
   bool deviceBuffers = false;  // past, fut, acc and recons of requests are device pointers
+  int laneSteps = 0;           // steps a lane may take per engine step (dvc_engine_params.lane_steps)
   uint64_t nextTicket = 1, clock = 0;
   std::deque<dvc::Job *> pending;
   std::vector<dvc::Lane> lanes;
@@ -856,11 +857,18 @@ static int engine_step(dvc_engine *e, int *completed) {
   for (auto &l : e->lanes)
     if (!l.inflight.empty()) busy.push_back(&l);
   if (busy.empty()) return DVC_OK;
-  auto run = [e, c](Lane *l) {
+  // a lane keeps stepping without waiting for the others until one of its
+  // frames completes or lane_steps steps are done; one step while frames
+  // wait for admission (the pending queue cannot change during the call)
+  const int maxSteps = e->pending.empty() ? std::max(1, e->laneSteps) : 1;
+  auto run = [e, c, maxSteps](Lane *l) {
     l->rc = DVC_OK;
     if (hipSetDevice(c->device) != hipSuccess) { l->rc = DVC_EDEVICE; l->err = "hipSetDevice failed"; return; }
-    l->rc = lane_step(e, *l);
-    if (l->rc) l->err = dvc_last_error();
+    for (int k = 0; k < maxSteps; k++) {
+      l->rc = lane_step(e, *l);
+      if (l->rc) { l->err = dvc_last_error(); return; }
+      if (!l->finished.empty() || l->inflight.empty()) return;
+    }
   };
   std::vector<std::thread> th;
   for (size_t k = 1; k < busy.size(); k++) th.emplace_back(run, busy[k]);
@@ -917,6 +925,7 @@ static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **ou
   e->sequentialBp = p && p->sequential_bp;
   e->graphs = !(p && p->eager_steps);
   e->deviceBuffers = p && p->device_buffers;
+  e->laneSteps = p ? p->lane_steps : 0;
   size_t freeB = 0, totalB = 0;
   (void)hipMemGetInfo(&freeB, &totalB);
   e->budget = (p && p->hbm_budget_bytes) ? (size_t)p->hbm_budget_bytes : (size_t)(0.8 * (double)freeB);
@@ -1129,6 +1138,7 @@ int dvc_decode_wz_frames(dvc_ctx *c, int n, const dvc_wz_frame *frames, int batc
   dvc_engine_params p{};
   p.max_inflight = batched ? n : 1;
   p.sequential_bp = batched ? 0 : 1;
+  p.lane_steps = 1 << 20;  // every frame is submitted up front: lanes never wait for each other
   dvc_engine *e = nullptr;
   int rc = engine_create(c, &p, &e);
   if (rc) return rc;

@@ -214,8 +214,8 @@ int dvc_side_info_batch(dvc_ctx *ctx, int n, const dvc_si_request *reqs);
  * strictly one after another, main.cpp:132-142): requests are queued by
  * dvc_engine_submit and admitted between steps while their device footprint
  * fits the HBM budget; every dvc_engine_step advances each in-flight frame by
- * one bitplane (setting up newcomers first) and retires finished frames, whose
- * recons/stats are then written.  Key frames are kept resident under a
+ * at least one bitplane (setting up newcomers first) and retires finished
+ * frames, whose recons/stats are then written.  Key frames are kept resident under a
  * caller-chosen key (e.g. stream id << 32 | frame number; 0 = the host
  * pointer) and shared by the frames that name it, until released or evicted
  * (least recently used, only when unreferenced).  The request's host buffers
@@ -233,6 +233,10 @@ typedef struct {
                                  starts or ends a band replay a captured HIP graph of their launches */
   int device_buffers;         /* 1: past, fut, acc and recons of submitted frames are device (HBM)
                                  pointers; ac_range, crc, orig and stats stay host pointers */
+  int lane_steps;             /* bitplane steps a lane may take in one dvc_engine_step without
+                                 waiting for the other lanes: it stops early when one of its frames
+                                 completes, and takes one step while frames wait for admission
+                                 (0 or 1: every lane takes one step per call) */
 } dvc_engine_params;
 int dvc_engine_create(dvc_ctx *ctx, const dvc_engine_params *params, dvc_engine **out);
 int dvc_engine_destroy(dvc_engine *eng);
