@@ -42,6 +42,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "WZ frames/sec and BP edge-updates/s (batched vs plain, 1 GPU); p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
+VALU_PEAK_WAVE_INSTR = 256 * 2 * 2.4e9  # 256 CUs x 2 wave64 fp32 instructions/clk x 2.4 GHz
 
 
 def log(*a):
@@ -79,8 +80,8 @@ def bench_ldpca(args):
 
     # Roofline of the dominant kernel (bp_6336): algorithmic HBM bytes per
     # launch = per accepted (bitplane, code) item 4N (LLR) + ceil(m/8)
-    # (syndrome) + ceil(N/8) (decisions) + 2E (row indices of the code),
-    # summed over every code the sequential search runs.
+    # (syndrome) + ceil(N/8) (decisions) + 8N (packed column edges of the
+    # code), summed over every code the sequential search runs.
     ladders = {n: W.ladder(n) for n in (1584, 6336)}
     m_of = {n: (n // 66) * ladders[n].bits_per_period for n in ladders}
     hbm_bytes = 0
@@ -92,7 +93,7 @@ def bench_ldpca(args):
         for lvl in range(st["start_level"], st["level"] + 1):
             if lvl == 64:
                 continue
-            hbm_bytes += 4 * n + (int(m_of[n][lvl]) + 7) // 8 + (n + 7) // 8 + 2 * 3 * n
+            hbm_bytes += 4 * n + (int(m_of[n][lvl]) + 7) // 8 + (n + 7) // 8 + 8 * n
         lds_bytes += 16 * st["iterations"] * 3 * n
     bp_avg_ms = bp_ms / max(bp_n, 1)
     per_step = max(bp_n // max(args.steps, 1), 1)  # bp_6336 launches per step
@@ -305,20 +306,42 @@ def profiled_traffic(kernel, workload):
     return None
 
 
+def profiled_value(kernel, workload, key):
+    """A per-launch figure `key` of `kernel` from the committed rocprofv3
+    summary of this workload, or None."""
+    f = ROOT / "profiles" / f"{kernel}_{workload}.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith(f"{key} {kernel} "):
+            return float(line.split()[-1])
+    return None
+
+
 def bp_roofline(kernel_ms_avg, hbm_bytes_per_launch, edge_updates_per_launch, workload):
     """Roofline of the BP kernel.  HBM: per (bitplane, code) the sequential
-    search runs, 4N (LLR) + ceil(m/8) (syndrome) + ceil(N/8) (decisions) + 2E
-    (16-bit row lists).  LDS: 16 bytes per edge-update (v2c read and write,
-    c2v read and write)."""
+    search runs, 4N (LLR) + ceil(m/8) (syndrome) + ceil(N/8) (decisions) + 8N
+    (packed column edges).  LDS: 16 bytes per edge-update (v2c read and
+    write, c2v read and write).  VALU: the profiled VALU wave-instructions
+    per launch over the live launch time, against the fp32 issue peak (two
+    wave64 instructions per CU-cycle: 157.3 TFLOP/s / (256 CUs x 2.4 GHz x
+    64 lanes x 2)); the exact glibc-identical transcendentals make this the
+    bound."""
     t = kernel_ms_avg * 1e-3
     hbm = hbm_bytes_per_launch / t / 1e9
     lds = 16.0 * edge_updates_per_launch / t / 1e9
-    return {"bound": "hbm", "kernel": "bp_6336", "achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": hbm / HBM_PEAK_GBS, "traffic": profiled_traffic("bp_6336", workload),
-            "kernel_ms_avg": kernel_ms_avg, "algorithmic_bytes_per_launch": hbm_bytes_per_launch,
-            "lds": {"achieved": lds, "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": lds / LDS_PEAK_GBS,
-                    "bytes_per_launch": 16.0 * edge_updates_per_launch},
-            "profile": f"profiles/bp_6336_{workload}.md"}
+    res = {"bound": "hbm", "kernel": "bp_6336", "achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": hbm / HBM_PEAK_GBS, "traffic": profiled_traffic("bp_6336", workload),
+           "kernel_ms_avg": kernel_ms_avg, "algorithmic_bytes_per_launch": hbm_bytes_per_launch,
+           "lds": {"achieved": lds, "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": lds / LDS_PEAK_GBS,
+                   "bytes_per_launch": 16.0 * edge_updates_per_launch},
+           "profile": f"profiles/bp_6336_{workload}.md"}
+    vi = profiled_value("bp_6336", workload, "valu_wave_instr_per_launch")
+    if vi:
+        peak = VALU_PEAK_WAVE_INSTR
+        res["valu"] = {"achieved": vi / t, "peak": peak, "unit": "wave-instr/s", "frac": vi / t / peak,
+                       "wave_instr_per_launch": vi}
+    return res
 
 
 def roofline_wz(dom, kern, args, nframes, outs):
@@ -334,7 +357,7 @@ def roofline_wz(dom, kern, args, nframes, outs):
         for b in range(st["nbitplanes"]):
             for lvl in range(st["start_levels"][b], st["levels"][b] + 1):
                 if lvl < 64:
-                    byts += 4 * 6336 + (96 * (lvl + 2) + 7) // 8 + 792 + 2 * 19008
+                    byts += 4 * 6336 + (96 * (lvl + 2) + 7) // 8 + 792 + 8 * 6336
             eu += st["iterations"][b] * 3 * 6336
     return bp_roofline(d["total_ms"] / launches, byts / launches, eu / launches, "wz_cif_q8")
 
