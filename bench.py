@@ -211,6 +211,7 @@ def bench_wz(args):
     eng.close()
     outs = [(dev.recons(i), dict(st, psnr=psnr(dev.recons(i), j["orig"]))) for i, (j, st) in enumerate(zip(jobs, stats))]
     mode1 = bench_wz_mode1(args, jobs, dev) if args.mode1_steps else None
+    pcie = bench_wz_host_buffers(args, ctx, jobs) if args.pcie_steps else None
     names = ["create_si", "keyframe_filters", "noise_param", "oct", "smf_init", "noise_dist", "condbit", "ce_llr",
              "bp_6336", "invh", "put_bitplane", "reconstruct", "inverse_transform", "update_si", "update_smf",
              "update_si_ml"]
@@ -259,11 +260,38 @@ def bench_wz(args):
         "inputs": "resident in HBM before the timed region (key frames, syndromes, output frames)",
         "kernels_ms_per_step": kern, "dominant_kernel": dom[0], "bp_tuning": tuning,
         "parallel_mode_1": mode1,
+        "pcie_inclusive": pcie,
     }
     res["roofline"] = roofline_wz(dom, kern, args, len(jobs), outs)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_wz(jobs, args)
     return res
+
+
+def bench_wz_host_buffers(args, ctx, jobs):
+    """The same workload with host (pageable) request buffers: key frames,
+    syndromes and output frames cross PCIe inside the timed steps, and the
+    PSNR is computed on the host.  Reported beside `value`, never as it."""
+    from dvc_amd import _lib
+    eng = _lib.Engine(ctx, max_inflight=len(jobs), lanes=args.lanes, device_buffers=False, lane_steps=1 << 20)
+
+    def submit_all(gen):
+        return [eng.submit(j, past_key=(gen << 48) | ((j["seq"] + 1) << 32) | j["idx"][0],
+                           fut_key=(gen << 48) | ((j["seq"] + 1) << 32) | j["idx"][2]) for j in jobs]
+    run_engine(eng, submit_all(2000))
+    times = []
+    for i in range(args.pcie_steps):
+        t = time.perf_counter()
+        run_engine(eng, submit_all(2001 + i))
+        times.append(time.perf_counter() - t)
+        log(f"[bench] host buffers step {i}: {times[-1] * 1e3:.1f} ms for {len(jobs)} frames")
+    eng.close()
+    t_step = float(np.mean(times))
+    keys = {(j["seq"], j["idx"][0]) for j in jobs} | {(j["seq"], j["idx"][2]) for j in jobs}
+    fs = jobs[0]["past"].size
+    pcie = len(keys) * fs + sum(len(j["wz"].acc) + fs for j in jobs)  # key frames and syndromes up, frames down
+    return {"value": len(jobs) / t_step, "unit": "WZ frames/s", "ms_per_step": t_step * 1e3,
+            "bytes_over_pcie_per_frame": int(pcie / len(jobs))}
 
 
 def bench_wz_mode1(args, jobs, dev):
@@ -533,6 +561,8 @@ def main():
     ap.add_argument("--serve-inflight", type=int, default=128, help="serve: engine frames in flight")
     ap.add_argument("--mode1-steps", type=int, default=2,
                     help="wz_cif_q8: also time this many steps with ParallelMode 1 BP semantics (0: skip)")
+    ap.add_argument("--pcie-steps", type=int, default=2,
+                    help="timed steps of the host-buffer leg (PCIe-inclusive rate; 0 skips it)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-planes", type=int, default=150)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
