@@ -210,8 +210,7 @@ def bench_wz(args):
         log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms for {len(jobs)} frames")
     eng.close()
     outs = [(dev.recons(i), dict(st, psnr=psnr(dev.recons(i), j["orig"]))) for i, (j, st) in enumerate(zip(jobs, stats))]
-    mode1 = bench_wz_mode1(args, jobs, dev) if args.mode1_steps else None
-    pcie = bench_wz_host_buffers(args, ctx, jobs) if args.pcie_steps else None
+    # per-kernel HIP-event totals of the timed steps only
     names = ["create_si", "keyframe_filters", "noise_param", "oct", "smf_init", "noise_dist", "condbit", "ce_llr",
              "bp_6336", "invh", "put_bitplane", "reconstruct", "inverse_transform", "update_si", "update_smf",
              "update_si_ml"]
@@ -221,6 +220,8 @@ def bench_wz(args):
         if cnt:
             kern[nm] = {"total_ms": ms / args.steps, "launches": cnt // args.steps}
     ctx.profile(False)
+    mode1 = bench_wz_mode1(args, jobs, dev) if args.mode1_steps else None
+    pcie = bench_wz_host_buffers(args, ctx, jobs) if args.pcie_steps else None
     # correctness of the measured outputs against the encoder's source planes:
     # every decoded bitplane must equal the source (a CRC-accepted or full-rate
     # decode), which the parity tests pin bit-exactly against the reference.

@@ -50,6 +50,14 @@ namespace dvc {
 
 __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 
+// Pointers the kernels read from descriptors in memory are generic, and a
+// generic (flat) load is counted by the LDS wait counter as well: a wait for
+// an LDS read would also wait for it.  These views make such loads global.
+typedef const float __attribute__((address_space(1))) gfloat_t;
+typedef const uint8_t __attribute__((address_space(1))) gbyte_t;
+__device__ __forceinline__ gfloat_t *as_global(const float *p) { return (gfloat_t *)p; }
+__device__ __forceinline__ gbyte_t *as_global(const uint8_t *p) { return (gbyte_t *)p; }
+
 // A column's three edges, packed by the ladder upload as 21-bit fields
 // (row | rank-in-row << 13): x = f0 | f1 << 21, y = f1 >> 11 | f2 << 10.
 __device__ __forceinline__ void col_edges(uint2 c, int row[3], int rank[3]) {
@@ -63,7 +71,7 @@ __device__ __forceinline__ void col_edges(uint2 c, int row[3], int rank[3]) {
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 
 // Accumulated-syndrome bit i (MSB-first bytes, ByteToBit LdpcaDecoder.cpp:81-107).
-__device__ __forceinline__ uint32_t acc_bit(const uint8_t *acc, int i) {
+__device__ __forceinline__ uint32_t acc_bit(gbyte_t *acc, int i) {
   return (acc[i >> 3] >> (7 - (i & 7))) & 1u;
 }
 
@@ -173,8 +181,8 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         int k = wd * 32 + q;
         if (k < m) {
           int cur = L.tx[(size_t)level * N + k];
-          uint32_t s = acc_bit(pd.acc, cur);
-          if (k > 0) s ^= acc_bit(pd.acc, L.tx[(size_t)level * N + k - 1]);
+          uint32_t s = acc_bit(as_global(pd.acc), cur);
+          if (k > 0) s ^= acc_bit(as_global(pd.acc), L.tx[(size_t)level * N + k - 1]);
           bits |= s << q;
         }
       }
@@ -184,7 +192,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
     for (int j = 0; j < S::CPT; j++) {
       int k = tid + j * T;
       if (k < N) {
-        float v = pd.LLR[k];
+        float v = as_global(pd.LLR)[k];
         uint32_t s = f2u(v) >> 31;
         float mag = (s ? -1.0f : 1.0f) * v;
         uint32_t f = (f2u(dvc_bp_phi_fast(mag, s_logtab)) & 0x7fffffffu) | (s << 31);
@@ -263,14 +271,15 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
       // column's six phi evaluations run.
       bool changed = false;
       uint2 rowsNext = tid < N ? gcol[tid] : make_uint2(0u, 0u);
-      float llrNext = tid < N ? pd.LLR[tid] : 0.0f;
+      gfloat_t *gLLR = as_global(pd.LLR);
+      float llrNext = tid < N ? gLLR[tid] : 0.0f;
       for (int j = 0; j < S::CPT; j++) {
         const int k = tid + j * T;
         const uint2 rows = rowsNext;
         const float llr = llrNext;
         if (j + 1 < S::CPT && k + T < N) {
           rowsNext = gcol[k + T];
-          llrNext = pd.LLR[k + T];
+          llrNext = gLLR[k + T];
         }
         uint32_t dec = 0;
         if (k < N) {
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
             dec = f2u(tmp) >> 31;
           } else {
             dec = tmp < 0.0f;  // LLR_overall < 0 (LdpcaDecoder.cpp:257-260): -0 decides 0
-            if (t == 0) changed |= !exact || pd.carry[k] != (uint8_t)dec;
+            if (t == 0) changed |= !exact || as_global(pd.carry)[k] != (uint8_t)dec;
             else changed |= ((decBits[k >> 5] >> (k & 31)) & 1u) != dec;
           }
           if (dec) {
@@ -466,7 +475,7 @@ __global__ void invh_kernel(const PlaneDesc *planes, int nplanes, LadderDev L, u
     for (int q = 0; q < 32; q++) {
       int k = wd * 32 + q;
       uint32_t s = 0;
-      if (k < N) s = acc_bit(d.acc, k) ^ (k > 0 ? acc_bit(d.acc, k - 1) : 0u);
+      if (k < N) s = acc_bit(as_global(d.acc), k) ^ (k > 0 ? acc_bit(as_global(d.acc), k - 1) : 0u);
       v |= s << (31 - (q & 31));  // MSB-first within the 32-bit chunk
     }
     sw[wd] = v;
