@@ -832,6 +832,7 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
     uint32_t p0 = usi_word(P, r, j, sh), p1 = usi_word(P, r + 1, j, sh), p2 = usi_word(P, r + 2, j, sh);
     uint32_t f0 = usi_word(Fu, r, j, sh), f1 = usi_word(Fu, r + 1, j, sh), f2 = usi_word(Fu, r + 2, j, sh);
     int idx = lane;
+#pragma unroll 4
     for (int y = sT; y <= sB; y++, r++, idx += ww) {
       const uint32_t p3 = usi_word(P, r + 3, j, sh), f3 = usi_word(Fu, r + 3, j, sh);
       const uint32_t ps = sad4(bk, p0, p1, p2, p3), fs = sad4(bk, f0, f1, f2, f3);

@@ -122,9 +122,11 @@ def test_engine_staggered_arrivals_equal_solo(ctx, lanes):
         _check_against_golden(rec, st, c)
 
 
-def test_engine_lanes_mixed_batch(ctx):
+@pytest.mark.parametrize("lane_steps", [0, 1 << 20])
+def test_engine_lanes_mixed_batch(ctx, lane_steps):
     """Frames spread over several lanes (QCIF and CIF, Qindex 1..8, shared key
-    frames) decode to the same output as each frame alone on the plain path."""
+    frames) decode to the same output as each frame alone on the plain path,
+    with the lanes in lockstep or stepping independently."""
     from dvc_amd import _lib
     frames = []
     for k, (w, h, sh) in enumerate([(176, 144, 3), (352, 288, 6)]):
@@ -133,7 +135,7 @@ def test_engine_lanes_mixed_batch(ctx):
             for cur in (1, 3):
                 frames.append(dict(past=vid[cur - 1], fut=vid[cur + 1], orig=vid[cur],
                                    wz=W.encode_wz_frame(vid[cur], q), idx=(cur - 1, cur, cur + 1)))
-    eng = _lib.Engine(ctx, max_inflight=len(frames), lanes=4)
+    eng = _lib.Engine(ctx, max_inflight=len(frames), lanes=4, lane_steps=lane_steps)
     outs = eng.decode_all(frames)
     eng.close()
     for fr, (rec, st) in zip(frames, outs):
@@ -161,6 +163,35 @@ def test_engine_graph_replay_equals_eager(ctx, lanes):
         assert np.array_equal(rg, re_)
         assert sg["levels"] == se["levels"] and sg["iterations"] == se["iterations"]
         assert sg["bitrate"] == se["bitrate"] and sg["psnr"] == se["psnr"]
+
+
+def test_engine_reused_across_batches(ctx):
+    """A long-lived engine (device blocks of retired frames and key frames
+    pooled and reused, captured graphs kept) decodes later batches, of other
+    sizes and quantisers, to the same output as solo decodes."""
+    from dvc_amd import _lib
+    batches = []
+    for k, (w, h, sh, qs) in enumerate([(352, 288, 6, (8, 3)), (176, 144, 3, (8, 5, 2)), (352, 288, 6, (8, 1))]):
+        vid = W.synthetic_video(w, h, 5, sh, seed=970 + k)
+        batches.append([dict(past=vid[2 * i], fut=vid[2 * i + 2], orig=vid[2 * i + 1],
+                             wz=W.encode_wz_frame(vid[2 * i + 1], q), idx=(2 * i, 2 * i + 1, 2 * i + 2))
+                        for i, q in zip(range(2), qs)] + [dict(past=vid[1], fut=vid[3], orig=vid[2],
+                                                                wz=W.encode_wz_frame(vid[2], qs[-1]), idx=(1, 2, 3))])
+    eng = _lib.Engine(ctx, max_inflight=4, lanes=2, lane_steps=1 << 20)
+    for gen, frames in enumerate(batches):
+        tickets = [eng.submit(fr, past_key=(gen << 48) | fr["idx"][0], fut_key=(gen << 48) | fr["idx"][2])
+                   for fr in frames]
+        pending = set(tickets)
+        while pending:
+            eng.step()
+            pending = {t for t in pending if not eng.done(t)}
+        for fr, t in zip(frames, tickets):
+            rec, st = eng.result(t)
+            (rec1, st1), = ctx.decode_wz_frames([fr], batched=False)
+            assert np.array_equal(rec, rec1)
+            assert st["levels"] == st1["levels"] and st["iterations"] == st1["iterations"]
+            assert st["bitrate"] == st1["bitrate"] and st["psnr"] == st1["psnr"]
+    eng.close()
 
 
 def test_engine_device_buffers_equal_host(ctx):
