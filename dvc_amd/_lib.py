@@ -32,7 +32,8 @@ class DvcError(RuntimeError):
 
 class Params(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("max_iter", ctypes.c_int),
-                ("data_dir", ctypes.c_char_p), ("bp_threads", ctypes.c_int), ("parallel_mode", ctypes.c_int)]
+                ("data_dir", ctypes.c_char_p), ("bp_threads", ctypes.c_int), ("parallel_mode", ctypes.c_int),
+                ("rate_step", ctypes.c_int)]
 
 
 class BpStats(ctypes.Structure):
@@ -192,8 +193,9 @@ class DeviceBuffer:
 class Context:
     """Owns a dvc_ctx (device, both ladders and their inverses resident)."""
 
-    def __init__(self, device=0, max_iter=50, bp_threads=0, data_dir=None, parallel_mode=0):
-        p = Params(device, max_iter, str(data_dir).encode() if data_dir else None, bp_threads, parallel_mode)
+    def __init__(self, device=0, max_iter=50, bp_threads=0, data_dir=None, parallel_mode=0, rate_step=0):
+        p = Params(device, max_iter, str(data_dir).encode() if data_dir else None, bp_threads, parallel_mode,
+                   rate_step)
         h = ctypes.c_void_p()
         _check(lib.dvc_ctx_create(ctypes.byref(p), ctypes.byref(h)), "dvc_ctx_create")
         self.h = h

@@ -108,6 +108,8 @@ int dvc_ctx_create(const dvc_params *params, dvc_ctx **out) {
   dvc_params p{};
   if (params) p = *params;
   DVC_CHECK(p.parallel_mode == 0 || p.parallel_mode == 1, DVC_EINVAL, "dvc_ctx_create: parallel_mode must be 0 or 1");
+  DVC_CHECK(p.rate_step == 0 || (p.rate_step == 1 && p.parallel_mode == 0), DVC_EINVAL,
+            "dvc_ctx_create: rate_step must be 0, or 1 with parallel_mode 0");
   int ndev = 0;
   DVC_HIP(hipGetDeviceCount(&ndev));
   DVC_CHECK(ndev > 0 && p.device >= 0 && p.device < ndev, DVC_EDEVICE, "dvc_ctx_create: no such HIP device");
@@ -121,6 +123,7 @@ int dvc_ctx_create(const dvc_params *params, dvc_ctx **out) {
   c->max_iter = p.max_iter > 0 ? p.max_iter : 50;
   if (p.bp_threads > 0) c->bp_threads_cif = p.bp_threads;
   c->bp_mode = p.parallel_mode;
+  c->rate_step = p.rate_step;
   c->sm_count = prop.multiProcessorCount;
   std::string dir = p.data_dir ? std::string(p.data_dir) : default_data_dir();
   int rc;
@@ -257,13 +260,14 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
     for (int i = 0; i < n; i++) {
       int level = state[i].start;
       int4 r;
-      for (; level < C - 1; level++) {
+      for (; level < C - 1;
+           level = c->rate_step ? std::min(C - 1, state[i].start + dynstep_next(level - state[i].start)) : level + 1) {
         PlaneState ps{state[i].start, C - 1, state[i].entropy, level};
         int zero[3] = {0, 0, 0};
         DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
         DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
         BpArgs a{ld, dPlanes.p + i, 1, dCounters.p, 1, 1, c->max_iter, dSlots.p, dCounters.p + 1, slotCap, c->bp_poll,
-                 c->bp_mode};
+                 c->bp_mode, c->rate_step};
         TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, 1, s));
         DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + level, sizeof(r), hipMemcpyDeviceToHost, s));
         DVC_HIP(hipStreamSynchronize(s));
@@ -293,7 +297,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
     DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
     const int maxOff = C - 1;
     BpArgs a{ld, dPlanes.p, n, dCounters.p, n * maxOff, maxOff, c->max_iter, dSlots.p, dCounters.p + 1, slotCap,
-             c->bp_poll, c->bp_mode};
+             c->bp_poll, c->bp_mode, c->rate_step};
     const int grid = bp_grid(c, N, n * maxOff);
     TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
     for (int round = 0; mode1; round++) {
@@ -342,6 +346,11 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
     st.iterations += rec[(size_t)i * C + lvl].x;
     st.rate = L->syndLength[lvl] + 8;
     st.requests = lvl - st.start_level + 1;
+    if (c->rate_step) {
+      // codes tried: the DYNAMICSTEP offsets below lvl, then lvl itself
+      st.requests = 1;
+      for (int off = 0; st.start_level + off < lvl; off = dynstep_next(off)) st.requests++;
+    }
     st.used_inverse = rec[(size_t)i * C + lvl].w;
   }
   return DVC_OK;

@@ -95,6 +95,7 @@ struct dvc_ctx {
   int bp_wg_per_cu_qcif = 4;
   int bp_poll = 1;
   int bp_mode = 0;  // 0: ParallelMode 0 BP semantics, 1: ParallelMode 1 (early jump on repeated decisions)
+  int rate_step = 0;  // 1: DYNAMICSTEP rate search (dvc_params.rate_step)
   dvc::Ladder lad1584, lad6336;
   hipStream_t stream = nullptr;
   void *si_scratch = nullptr;  // dvc_side_info_batch's device buffers, kept between calls

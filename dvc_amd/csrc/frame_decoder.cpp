@@ -241,7 +241,7 @@ int dvc::bp_step_launch(dvc_ctx *c, int N, int n, uint8_t *d_slots, int slotCap,
   TIMED_LAUNCH(c, "ce_llr", s, launch_ce_llr(d_planes, n, 1, c->lad1584, c->lad6336, s));
   DVC_HIP(launch_zero_ints(d_counters, 3, s));
   BpArgs a{ld, d_planes, n, d_counters, n * maxOff, maxOff, c->max_iter, d_slots, d_counters + 1, slotCap, c->bp_poll,
-           c->bp_mode};
+           c->bp_mode, c->rate_step};
   const int grid = bp_grid(c, N, n * maxOff);
   TIMED_LAUNCH(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
   for (int round = 0; c->bp_mode == 1; round++) {
@@ -285,12 +285,13 @@ int dvc::bp_step(dvc_ctx *c, int N, const std::vector<PlaneDesc> &planes, bool s
     bool ok = false;
     PlaneState ps;
     int4 r;
-    for (; level < C - 1; level++) {
+    for (; level < C - 1;
+         level = c->rate_step ? std::min(C - 1, st[i].start + dynstep_next(level - st[i].start)) : level + 1) {
       ps = PlaneState{st[i].start, C - 1, st[i].entropy, level};
       DVC_HIP(hipMemcpyAsync(planes[i].state, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
       DVC_HIP(hipMemsetAsync(d_counters, 0, sizeof(int), s));  // queue head only: the step's slots accumulate
       BpArgs a{ld, d_planes + i, 1, d_counters, 1, 1, c->max_iter, d_slots, d_counters + 1, slotCap, c->bp_poll,
-               c->bp_mode};
+               c->bp_mode, c->rate_step};
       TIMED_LAUNCH(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, 1, s));
       DVC_HIP(hipMemcpyAsync(&r, planes[i].rec + level, sizeof(r), hipMemcpyDeviceToHost, s));
       DVC_HIP(hipStreamSynchronize(s));

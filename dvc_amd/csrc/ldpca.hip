@@ -151,6 +151,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         PlaneState *st = a.planes[b].state;
         int level = st->run + off;
         if (off >= a.max_offset || level >= L.numCodes - 1) continue;
+        if (a.rate_step && !dynstep_tries(off)) continue;
         if (__hip_atomic_load(&st->best, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < level) continue;
         item = w;
         break;

@@ -61,7 +61,20 @@ struct BpArgs {
   int slot_capacity;
   int poll;           // iterations between polls of the bitplane's best code (>= 1)
   int mode;           // 0: beliefPropagation, 1: beliefPropagation_OpenMP (ParallelMode 1)
+  int rate_step;      // 1: only the codes the DYNAMICSTEP search tries (dynstep_tries)
 };
+
+// DYNAMICSTEP rate search (LdpcaDecoder.cpp:19-21, 669-672, 760-768): after
+// the r-th failed request (r = 0, 1, ...) the search moves 1 + {0,1,2,3}[min(r,3)]
+// codes up, so it tries the codes start + 0, 1, 3, 6, 10, 14, 18, ... below
+// the full rate, then the full rate.
+__host__ __device__ inline bool dynstep_tries(int off) {
+  return off == 0 || off == 1 || off == 3 || off == 6 || (off >= 10 && (off - 10) % 4 == 0);
+}
+// The offset from the start code tried after offset `off` fails.
+__host__ __device__ inline int dynstep_next(int off) {
+  return off < 1 ? 1 : off < 3 ? 3 : off < 6 ? 6 : off < 10 ? 10 : off + 4;
+}
 
 // record.y status values
 enum { REC_FAILED = 0, REC_CRC_MISMATCH = 1, REC_ACCEPTED = 2, REC_CANCELLED = 3, REC_NOT_RUN = -1 };

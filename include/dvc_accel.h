@@ -45,6 +45,9 @@ typedef struct {
                             1 = beliefPropagation_OpenMP (ParallelMode 1, :164-306): decisions LLR < 0 and a
                             failure after 4 consecutive iterations with unchanged decisions, compared first
                             against the decisions the previous call left (see dvc_ldpca_decode) */
+  int rate_step;         /* rate search: 0 = every code from the start code up (LdpcaDecoder.cpp:674);
+                            1 = the DYNAMICSTEP build's growing steps (:19-21, 669-672, 760-768): codes
+                            start + 0, 1, 3, 6, 10, 14, ... then the full rate; needs parallel_mode 0 */
 } dvc_params;
 
 /* InitCUDA + LDPCA_CUDA_Init_Variable (LDPCA_cuda.cu:36-89, 546-763): selects the
@@ -79,7 +82,7 @@ typedef struct {
   int start_level;   /* nCodeStart: first code with rate >= H(X|Y) (LdpcaDecoder.cpp:657-659) */
   int level;         /* accepted code index */
   int rate;          /* m + 8 bits (syndrome + CRC, LdpcaDecoder.cpp:681) */
-  int requests;      /* codes tried = level - start_level + 1 */
+  int requests;      /* codes tried (level - start_level + 1 with rate_step 0) */
   int iterations;    /* BP iterations summed over the codes tried (sequential search) */
   int used_inverse;  /* 1 when the full-rate GF(2) inverse produced the bitplane */
   float entropy;     /* conditional entropy H(X|Y) used for start_level */

@@ -93,7 +93,7 @@ typedef struct {
   int start_level;   /* nCodeStart */
   int level;         /* accepted code index */
   int rate;          /* m + 8 */
-  int requests;      /* levels tried (level - start_level + 1) */
+  int requests;      /* codes tried (level - start_level + 1 with rate_step 0) */
   int total_iters;   /* BP iterations over all tried levels */
   int used_inverse;  /* 1 when the full-rate inverse produced the result */
 } orc_ldpca_stats;
@@ -104,6 +104,11 @@ typedef struct {
 void orc_ldpca_decode(const orc_ladder *L, float *LR, const uint8_t *acc_bits, uint8_t crc,
                       float entropy, int mode, int max_iter, int nthreads, uint8_t *decoded,
                       orc_ldpca_stats *st);
+/* The same with rate_step 1: the rate search of a build with DYNAMICSTEP
+ * defined (LdpcaDecoder.cpp:19-21, 669-672, 760-768). */
+void orc_ldpca_decode_steps(const orc_ladder *L, float *LR, const uint8_t *acc_bits, uint8_t crc,
+                            float entropy, int mode, int max_iter, int nthreads, int rate_step,
+                            uint8_t *decoded, orc_ldpca_stats *st);
 
 /* LdpcaEncoder (DvcEnc/src/LdpcaEncoder.cpp:64-95): accumulated syndrome
  * bytes (N/8) and CRC of the source bitplane. */
@@ -205,6 +210,7 @@ typedef struct {
   int mode;        /* 0 sequential, 1 OpenMP */
   int nthreads;    /* OpenMP threads for mode 1 */
   int max_iter;    /* BP iteration cap (reference: 50) */
+  int rate_step;   /* 1: the DYNAMICSTEP build's rate search (orc_ldpca_decode_steps) */
 } orc_decode_opts;
 
 typedef struct {

@@ -151,7 +151,8 @@ int orc_decode_wz_frame(const orc_geom *g, int q, const orc_ladder *L, const uin
       float ent = opt->mode == 0 ? orc_compute_ce(LR, N) : orc_compute_ce_omp(LR, N, opt->nthreads);
       orc_bytes_to_bits(acc + (size_t)bpi * (N / 8), N, accBits);
       orc_ldpca_stats st;
-      orc_ldpca_decode(L, LR, accBits, crc[bpi], ent, opt->mode, opt->max_iter, opt->nthreads, decoded, &st);
+      orc_ldpca_decode_steps(L, LR, accBits, crc[bpi], ent, opt->mode, opt->max_iter, opt->nthreads, opt->rate_step,
+                             decoded, &st);
       res->bitrate += st.rate;
       res->levels[bpi] = st.level;
       res->start_levels[bpi] = st.start_level;

@@ -345,7 +345,19 @@ void orc_invh_mul(const orc_ladder *L, const uint8_t *synd, uint8_t *decoded) {
 void orc_ldpca_decode(const orc_ladder *L, float *LR, const uint8_t *acc, uint8_t crc,
                       float entropy, int mode, int max_iter, int nthreads, uint8_t *decoded,
                       orc_ldpca_stats *st) {
+  orc_ldpca_decode_steps(L, LR, acc, crc, entropy, mode, max_iter, nthreads, 0, decoded, st);
+}
+
+/* rate_step 1 (DYNAMICSTEP, LdpcaDecoder.cpp:19-21, 669-672, 760-768): after
+ * the r-th failed request the code index grows by jump_step[min(r, 3)] =
+ * {0,1,2,3} on top of the loop's increment, clamped so that the next code is
+ * at most the full-rate one. */
+void orc_ldpca_decode_steps(const orc_ladder *L, float *LR, const uint8_t *acc, uint8_t crc,
+                            float entropy, int mode, int max_iter, int nthreads, int rate_step,
+                            uint8_t *decoded, orc_ldpca_stats *st) {
+  static const int jump_step[4] = {0, 1, 2, 3};
   int N = L->N;
+  int request = 0, stepInc = 0;
   int start = 0;
   /* LdpcaDecoder.cpp:657-659 (bounded: the last code has rate 1). */
   while (start < L->numCodes - 1 && L->codeRate[start] < entropy) start++;
@@ -358,7 +370,7 @@ void orc_ldpca_decode(const orc_ladder *L, float *LR, const uint8_t *acc, uint8_
     int m = L->syndLength[code];
     st->rate = m + 8;
     st->level = code;
-    st->requests = code - start + 1;
+    st->requests++;
     orc_form_syndrome(L, code, acc, synd);
     if (m == N) { orc_invh_mul(L, synd, decoded); st->used_inverse = 1; break; }
     int it = 0, ok;
@@ -368,6 +380,12 @@ void orc_ldpca_decode(const orc_ladder *L, float *LR, const uint8_t *acc, uint8_
     if (ok) {
       int nb = orc_bits_to_bytes(decoded, N, bytes);
       if (orc_crc8(bytes, nb) == crc) break;
+    }
+    if (rate_step) {
+      if (request < 4) stepInc = jump_step[request];
+      code += stepInc;
+      if (code >= L->numCodes - 1) code = L->numCodes - 2;
+      request++;
     }
   }
   free(synd); free(bytes);

@@ -16,6 +16,10 @@ Fixtures (all small JSON):
   frames.json     reference decodeGOP results for seeded WZ frames: accepted
                   code per bitplane, entropy bits, bitrate, PSNR, hashes of
                   the side information and of the reconstruction
+  frames_mode1.json  the same with ParallelMode 1
+  ldpca_dynstep.json, frames_dynstep.json  the ldpca and frames fixtures from
+                  the decoder built with DYNAMICSTEP defined in
+                  LdpcaDecoder.cpp (DVC_GOLDEN_ONLY=dynstep writes just these)
 Inputs are regenerated from their seeds by dvc_amd.workload, so a fixture
 holds only seeds, sizes and expected outputs.
 """
@@ -112,6 +116,29 @@ def ldpca_fixture():
     return {"max_iter": 50, "mode": 0, "items": out}
 
 
+def ldpca_dynstep_fixture():
+    """The reference LdpcaDecoder built with DYNAMICSTEP (oracle/_ref/
+    libdvcref_dyn.so) on the bitplanes of ldpca_fixture."""
+    R = O.ref_dec(dynstep=True)
+    out = []
+    for (n_cif, n_qcif, seed) in [(0, 24, 31), (8, 0, 32)]:
+        w, h = (352, 288) if n_cif else (176, 144)
+        O.RefDecoder(w, h, 4, dynstep=True)
+        planes = W.ldpca_bitplane_set(n_cif=n_cif, n_qcif=n_qcif, seed=seed)
+        for i, pl in enumerate(planes):
+            lr = pl["LR"].copy()
+            ent = R.ref_compute_ce(O.P(lr))
+            acc_bits = np.unpackbits(pl["acc"]).astype(np.uint8)
+            dec = np.full(pl["N"], 0xAA, np.uint8)
+            rate = ctypes.c_int()
+            R.ref_ldpca(O.P(lr), O.P(acc_bits), ctypes.c_uint8(pl["crc"]), ctypes.c_float(ent), O.P(dec),
+                        ctypes.byref(rate))
+            out.append({"set": [n_cif, n_qcif, seed], "index": i, "N": pl["N"], "rate": rate.value,
+                        "level": (rate.value - 8) // (pl["N"] // 66) - 2, "entropy_bits": fbits(ent),
+                        "decoded_sha256": sha(dec), "source_ok": bool(np.array_equal(dec, pl["bits"]))})
+    return {"max_iter": 50, "mode": 0, "rate_step": 1, "items": out}
+
+
 FRAME_CASES = [
     # (W, H, qindex, seed, shift, frames, wz frame index)  -- config-0 style QCIF and two CIF frames
     (176, 144, 4, 1000, 3, 3, 1),
@@ -121,19 +148,19 @@ FRAME_CASES = [
 ]
 
 
-def frames_fixture(mode=0):
+def frames_fixture(mode=0, dynstep=False):
     out = []
     for (w, h, q, seed, shift, nf, fi) in FRAME_CASES:
         vid = W.synthetic_video(w, h, nf, shift, seed)
         wz = W.encode_wz_frame(vid[fi], q)
-        R = O.RefDecoder(w, h, q, mode=mode, nthreads=1)
+        R = O.RefDecoder(w, h, q, mode=mode, nthreads=1, dynstep=dynstep)
         # side information of the reference (createSideInfo)
         si = np.zeros((h, w), np.uint8)
         res = np.zeros((h, w), np.int32)
         R.lib.ref_set_frame(0, O.P(np.ascontiguousarray(vid[0])), O.P(np.ascontiguousarray(vid[fi])))
         R.lib.ref_set_frame(2, O.P(np.ascontiguousarray(vid[2])), O.P(np.ascontiguousarray(vid[fi])))
         R.lib.ref_create_side_info(0, 2, 1, O.P(si), O.P(res), None, None)
-        R = O.RefDecoder(w, h, q, mode=mode, nthreads=1)
+        R = O.RefDecoder(w, h, q, mode=mode, nthreads=1, dynstep=dynstep)
         rec, info = R.decode_wz_frame(vid[0], vid[2], vid[fi], wz)
         out.append({"W": w, "H": h, "qindex": q, "seed": seed, "shift": shift, "frames": nf, "wz": fi,
                     "mode": mode, "levels": info["levels"], "rates": info["rates"],
@@ -141,11 +168,16 @@ def frames_fixture(mode=0):
                     "psnr": info["psnr"], "si_sha256": sha(si), "residue_sha256": sha(res),
                     "recons_sha256": sha(rec)})
         print(f"frame case {w}x{h} q{q} mode{mode}: psnr {info['psnr']:.4f} bitrate {info['bitrate']}", flush=True)
-    return {"max_iter": 50, "cases": out}
+    return {"max_iter": 50, "cases": out} if not dynstep else {"max_iter": 50, "rate_step": 1, "cases": out}
 
 
 def main():
     assert O.have_reference_build(), "run `make -C oracle ref` first"
+    if os.environ.get("DVC_GOLDEN_ONLY") == "dynstep":
+        # the DYNAMICSTEP build's fixtures alone (the others are unchanged by it)
+        (HERE / "ldpca_dynstep.json").write_text(json.dumps(ldpca_dynstep_fixture(), indent=1))
+        (HERE / "frames_dynstep.json").write_text(json.dumps(frames_fixture(mode=0, dynstep=True), indent=1))
+        return
     (HERE / "ladder.json").write_text(json.dumps(ladder_fixture(), indent=1))
     (HERE / "encoder.json").write_text(json.dumps(encoder_fixture(), indent=1))
     (HERE / "ldpca.json").write_text(json.dumps(ldpca_fixture(), indent=1))

@@ -9,6 +9,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parents[1]
 ORACLE_SO = ROOT / "oracle" / "liboracle.so"
 REF_DEC_SO = ROOT / "oracle" / "_ref" / "libdvcref.so"
+REF_DEC_DYN_SO = ROOT / "oracle" / "_ref" / "libdvcref_dyn.so"
 REF_ENC_SO = ROOT / "oracle" / "_ref" / "libdvcencref.so"
 REFERENCE = Path("/root/reference")
 
@@ -40,7 +41,7 @@ class OrcGeom(ctypes.Structure):
 
 
 class OrcOpts(ctypes.Structure):
-    _fields_ = [("mode", c_int), ("nthreads", c_int), ("max_iter", c_int)]
+    _fields_ = [("mode", c_int), ("nthreads", c_int), ("max_iter", c_int), ("rate_step", c_int)]
 
 
 class OrcFrameResult(ctypes.Structure):
@@ -142,8 +143,9 @@ def compute_ce(lr):
     return float(orc().orc_compute_ce(P(lr), lr.size))
 
 
-def ldpca_decode(lr, acc_bytes, crc, mode=0, max_iter=50, nthreads=1, decoded_init=None, entropy=None):
-    """Oracle LdpcaDecoder: returns (decoded, stats dict, entropy)."""
+def ldpca_decode(lr, acc_bytes, crc, mode=0, max_iter=50, nthreads=1, decoded_init=None, entropy=None, rate_step=0):
+    """Oracle LdpcaDecoder (rate_step 1: the DYNAMICSTEP build's search):
+    returns (decoded, stats dict, entropy)."""
     lr = np.array(lr, np.float32)  # converted to LLR in place, keep the caller's copy
     n = lr.size
     L = ladder(n)
@@ -151,8 +153,8 @@ def ldpca_decode(lr, acc_bytes, crc, mode=0, max_iter=50, nthreads=1, decoded_in
     acc_bits = np.unpackbits(np.asarray(acc_bytes, np.uint8))[:n].copy()
     dec = np.full(n, 0xAA, np.uint8) if decoded_init is None else np.array(decoded_init, np.uint8)
     st = OrcStats()
-    orc().orc_ldpca_decode(L.ref(), P(lr), P(acc_bits), c_uint8(int(crc)), c_float(ent), mode, max_iter, nthreads,
-                           P(dec), ctypes.byref(st))
+    orc().orc_ldpca_decode_steps(L.ref(), P(lr), P(acc_bits), c_uint8(int(crc)), c_float(ent), mode, max_iter,
+                                 nthreads, rate_step, P(dec), ctypes.byref(st))
     return dec, {f: getattr(st, f) for f, _ in st._fields_}, ent
 
 
@@ -178,15 +180,25 @@ def have_reference_build():
     return REF_DEC_SO.exists() and REF_ENC_SO.exists() and REFERENCE.exists()
 
 
-def ref_dec():
-    global _ref_dec
-    if _ref_dec is None:
-        _ref_dec = ctypes.CDLL(str(REF_DEC_SO))
-        _ref_dec.ref_compute_ce.restype = c_float
-        _ref_dec.ref_compute_ce_omp.restype = c_float
-        _ref_dec.ref_get_psnr.restype = c_double
-        _ref_dec.ref_psnr.restype = c_double
-    return _ref_dec
+_ref_dec_dyn = None
+
+
+def ref_dec(dynstep=False):
+    """The reference decoder library; dynstep: the build with DYNAMICSTEP
+    defined in LdpcaDecoder.cpp (its own copy of the reference's globals)."""
+    global _ref_dec, _ref_dec_dyn
+    lib = _ref_dec_dyn if dynstep else _ref_dec
+    if lib is None:
+        lib = ctypes.CDLL(str(REF_DEC_DYN_SO if dynstep else REF_DEC_SO))
+        lib.ref_compute_ce.restype = c_float
+        lib.ref_compute_ce_omp.restype = c_float
+        lib.ref_get_psnr.restype = c_double
+        lib.ref_psnr.restype = c_double
+        if dynstep:
+            _ref_dec_dyn = lib
+        else:
+            _ref_dec = lib
+    return lib
 
 
 def ref_enc():
@@ -197,7 +209,7 @@ def ref_enc():
 
 
 # ------------------------------------------------------------ whole frames
-def decode_wz_frame(past, fut, orig, wz, qindex, mode=0, nthreads=1, max_iter=50, idx=(0, 1, 2)):
+def decode_wz_frame(past, fut, orig, wz, qindex, mode=0, nthreads=1, max_iter=50, idx=(0, 1, 2), rate_step=0):
     """Oracle decodeGOP body for one WZ frame.  Returns (recons, result dict)."""
     h, w = past.shape
     g = geom(w, h)
@@ -210,7 +222,7 @@ def decode_wz_frame(past, fut, orig, wz, qindex, mode=0, nthreads=1, max_iter=50
     crc = np.ascontiguousarray(wz.crc, np.uint8)
     acc = np.ascontiguousarray(wz.acc, np.uint8)
     rec = np.zeros((h, w), np.uint8)
-    opt = OrcOpts(mode, nthreads, max_iter)
+    opt = OrcOpts(mode, nthreads, max_iter, rate_step)
     res = OrcFrameResult()
     rc = orc().orc_decode_wz_frame(ctypes.byref(g), qindex, L.ref(), P(past), P(fut), idx[0], idx[1], idx[2],
                                    P(acr), P(crc), P(acc), P(orig), ctypes.byref(opt), P(rec), ctypes.byref(res))
@@ -245,8 +257,8 @@ class RefDecoder:
     for one frame geometry.  The reference keeps global state, so one geometry
     per process: call setup() again to switch."""
 
-    def __init__(self, w, h, qindex, mode=0, nthreads=1, nframes=3, tmpdir="/tmp"):
-        self.lib = ref_dec()
+    def __init__(self, w, h, qindex, mode=0, nthreads=1, nframes=3, tmpdir="/tmp", dynstep=False):
+        self.lib = ref_dec(dynstep)
         self.w, self.h, self.q = w, h, qindex
         self.lib.ref_setup(w, h, qindex, mode, nthreads, nframes)
         n = (w // 4) * (h // 4)

@@ -116,15 +116,19 @@ def test_encoder_matches_reference_fixture():
         assert acr.tolist() == c["ac_range"] and crc.tolist() == c["crc"] and sha(acc) == c["acc_sha256"]
 
 
-def test_ldpca_matches_reference_fixture():
-    g = json.loads((GOLD / "ldpca.json").read_text())
+@pytest.mark.parametrize("fixture", ["ldpca.json", "ldpca_dynstep.json"])
+def test_ldpca_matches_reference_fixture(fixture):
+    """ldpca_dynstep.json comes from the reference decoder built with
+    DYNAMICSTEP; the oracle's rate_step 1 must reproduce it."""
+    g = json.loads((GOLD / fixture).read_text())
     sets = {}
     for it in g["items"]:
         key = tuple(it["set"])
         if key not in sets:
             sets[key] = W.ldpca_bitplane_set(n_cif=key[0], n_qcif=key[1], seed=key[2])
         pl = sets[key][it["index"]]
-        dec, st, ent = O.ldpca_decode(pl["LR"], pl["acc"], pl["crc"], max_iter=g["max_iter"])
+        dec, st, ent = O.ldpca_decode(pl["LR"], pl["acc"], pl["crc"], max_iter=g["max_iter"],
+                                      rate_step=g.get("rate_step", 0))
         assert fbits(ent) == it["entropy_bits"]
         assert st["rate"] == it["rate"] and st["level"] == it["level"]
         assert sha(dec) == it["decoded_sha256"]
@@ -136,12 +140,14 @@ def _frame_cases(name):
 
 
 @pytest.mark.parametrize("case", range(4))
-@pytest.mark.parametrize("fixture", ["frames.json", "frames_mode1.json"])
+@pytest.mark.parametrize("fixture", ["frames.json", "frames_mode1.json", "frames_dynstep.json"])
 def test_frame_decode_matches_reference_fixture(case, fixture):
     c = _frame_cases(fixture)[case]
     vid = W.synthetic_video(c["W"], c["H"], c["frames"], c["shift"], c["seed"])
     wz = W.encode_wz_frame(vid[c["wz"]], c["qindex"])
-    rec, info = O.decode_wz_frame(vid[0], vid[2], vid[c["wz"]], wz, c["qindex"], mode=c["mode"], nthreads=1)
+    rate_step = json.loads((GOLD / fixture).read_text()).get("rate_step", 0)
+    rec, info = O.decode_wz_frame(vid[0], vid[2], vid[c["wz"]], wz, c["qindex"], mode=c["mode"], nthreads=1,
+                                  rate_step=rate_step)
     assert info["levels"] == c["levels"]
     assert [fbits(e) for e in info["entropy"]] == c["entropy_bits"]
     assert info["bitrate"] == c["bitrate"]
