@@ -162,6 +162,10 @@ class DeviceInputs:
                                              fut_key=(gen << 48) | kf))
         return tickets
 
+    def release_keys(self, eng, jobs, gen):
+        for k in {(gen << 48) | ((j["seq"] + 1) << 32) | f for j in jobs for f in (j["idx"][0], j["idx"][2])}:
+            eng.release_key(k)
+
     def recons(self, i):
         return self.rec[i].download(self.shapes[i], np.uint8)
 
@@ -198,6 +202,7 @@ def bench_wz(args):
     gen = 0
     for i in range(args.warmup):
         run_engine(eng, dev.submit_all(eng, jobs, gen))
+        dev.release_keys(eng, jobs, gen)
         gen += 1
         log(f"[bench] warmup {i} done")
     ctx.profile(True)
@@ -206,6 +211,7 @@ def bench_wz(args):
         t = time.perf_counter()
         stats = run_engine(eng, dev.submit_all(eng, jobs, gen))
         times.append(time.perf_counter() - t)
+        dev.release_keys(eng, jobs, gen)  # a server releases a stream's key frames once no frame needs them
         gen += 1
         log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms for {len(jobs)} frames")
     eng.close()
