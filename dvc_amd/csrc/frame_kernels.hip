@@ -653,15 +653,24 @@ __global__ __launch_bounds__(64) void k_condbit(const FrameDev *F, const int *li
   __syncthreads();
   const size_t stride = f.distStride;
   float p0 = 0.0f, p1 = 0.0f;
-  for (int c0 = 0; c0 < span; c0 += 32) {
+  // chunk c0 of every block's span, 32 values per block: lane (j parity,
+  // i) fetches value i of blocks jj + (lane >> 5); the next chunk's loads are
+  // issued before this chunk is summed
+  float nxt[32];
+  auto fetch = [&](int c0) {
+#pragma unroll
     for (int jj = 0; jj < 64; jj += 2) {
       const int j = jj + (lane >> 5), i = lane & 31;
       const int k = s_lo[j] + c0 + i;
-      float v = 0.0f;
-      if (b0 + j < f.N && k < s_hi[j]) v = f.dist[(size_t)(b0 + j) * stride + k];
-      s_v[j * 33 + i] = v;
+      nxt[jj >> 1] = (b0 + j < f.N && k < s_hi[j]) ? f.dist[(size_t)(b0 + j) * stride + k] : 0.0f;
     }
+  };
+  if (span > 0) fetch(0);
+  for (int c0 = 0; c0 < span; c0 += 32) {
+#pragma unroll
+    for (int jj = 0; jj < 64; jj += 2) s_v[(jj + (lane >> 5)) * 33 + (lane & 31)] = nxt[jj >> 1];
     __syncthreads();
+    if (c0 + 32 < span) fetch(c0 + 32);
     for (int i = 0; i < 32; i++) {
       const int k = lo + c0 + i;
       const float v = s_v[lane * 33 + i];
