@@ -40,7 +40,7 @@ struct DevBuf {
 int bp_threads(const dvc_ctx *c, int N) { return N == 1584 ? c->bp_threads_qcif : c->bp_threads_cif; }
 
 int bp_grid(const dvc_ctx *c, int N, int items) {
-  // a CIF codeword fills a CU's LDS (~155 KB); QCIF codewords (~39 KB) share one
+  // a CIF codeword takes 118 KB of a CU's 160 KB LDS; QCIF codewords (30 KB) share one
   const int perCU = N == 6336 ? 1 : c->bp_wg_per_cu_qcif;
   return std::max(1, std::min(c->sm_count * perCU, items));
 }
