@@ -112,6 +112,9 @@ def bench_ldpca(args):
                        "ce_llr": ce_ms / max(ce_n, 1)},
         "wrong_bitplanes": bad,
         "edge_updates_per_step": edge_updates,
+        # BP iterations the batch ran (speculative and cancelled codes included)
+        # over those the sequential search needs
+        "bp_iterations_run_over_needed": sum(st["iterations_run"] for st in sts) / max(1, sum(st["iterations"] for st in sts)),
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_ldpca(planes, args)

@@ -39,7 +39,7 @@ class Params(ctypes.Structure):
 class BpStats(ctypes.Structure):
     _fields_ = [("start_level", ctypes.c_int), ("level", ctypes.c_int), ("rate", ctypes.c_int),
                 ("requests", ctypes.c_int), ("iterations", ctypes.c_int),
-                ("used_inverse", ctypes.c_int), ("entropy", ctypes.c_float)]
+                ("used_inverse", ctypes.c_int), ("entropy", ctypes.c_float), ("iterations_run", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

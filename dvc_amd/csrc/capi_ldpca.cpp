@@ -26,17 +26,6 @@ static std::string default_data_dir() {
   return "data";
 }
 
-template <typename T>
-struct DevBuf {
-  T *p = nullptr;
-  size_t n = 0;
-  ~DevBuf() { if (p) (void)hipFree(p); }
-  hipError_t alloc(size_t count) {
-    n = count;
-    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
-  }
-};
-
 int bp_threads(const dvc_ctx *c, int N) { return N == 1584 ? c->bp_threads_qcif : c->bp_threads_cif; }
 
 int bp_grid(const dvc_ctx *c, int N, int items) {
@@ -146,6 +135,8 @@ int dvc_ctx_destroy(dvc_ctx *c) {
   if (c->d_costFme) (void)hipFree(c->d_costFme);
   if (c->d_costUsi) (void)hipFree(c->d_costUsi);
   if (c->si_scratch) (void)hipFree(c->si_scratch);
+  if (c->ld_dev) (void)hipFree(c->ld_dev);
+  if (c->ld_host) (void)hipHostFree(c->ld_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DVC_OK;
@@ -198,6 +189,58 @@ int dvc_ladder_export(dvc_ctx *c, int N, int code, int *ir_out, uint8_t *inv_out
   return DVC_OK;
 }
 
+// Byte offsets of decode_group's buffers in the context's LDPCA workspace.
+// The input region [lr, inEnd) is filled in pinned host memory at the same
+// offsets and uploaded in one copy; [state, outEnd) comes back in one copy.
+struct LdpcaLayout {
+  size_t lr, acc, carry, planes, state, rec, counters, inEnd, out, outEnd, llr, slots, bcase, total;
+};
+
+static LdpcaLayout ldpca_layout(size_t n, size_t N, size_t C, size_t slotCap, bool mode1) {
+  LdpcaLayout o;
+  size_t at = 0;
+  auto take = [&at](size_t bytes) {
+    const size_t r = at;
+    at += (bytes + 255) & ~(size_t)255;
+    return r;
+  };
+  o.lr = take(sizeof(float) * n * N);
+  o.acc = take(n * (N / 8));
+  o.carry = take(mode1 ? n * N : 0);
+  o.planes = take(sizeof(PlaneDesc) * n);
+  o.state = take(sizeof(PlaneState) * n);
+  o.rec = take(sizeof(int4) * n * C);
+  o.counters = take(sizeof(int) * 3);
+  o.inEnd = at;
+  o.out = take(n * N);
+  o.outEnd = o.out + n * N;
+  o.llr = take(sizeof(float) * n * N);
+  o.slots = take(slotCap * N);
+  o.bcase = take(mode1 ? sizeof(int) * n * C : 0);
+  o.total = at;
+  return o;
+}
+
+// Grows the context's LDPCA workspace (device) and its pinned staging (host);
+// both are kept between calls and freed by dvc_ctx_destroy.
+static int ldpca_workspace(dvc_ctx *c, size_t devBytes, size_t hostBytes) {
+  if (c->ld_dev_bytes < devBytes) {
+    if (c->ld_dev) DVC_HIP(hipFree(c->ld_dev));
+    c->ld_dev = nullptr;
+    c->ld_dev_bytes = 0;
+    DVC_HIP(hipMalloc(&c->ld_dev, devBytes));
+    c->ld_dev_bytes = devBytes;
+  }
+  if (c->ld_host_bytes < hostBytes) {
+    if (c->ld_host) DVC_HIP(hipHostFree(c->ld_host));
+    c->ld_host = nullptr;
+    c->ld_host_bytes = 0;
+    DVC_HIP(hipHostMalloc(&c->ld_host, hostBytes, hipHostMallocDefault));
+    c->ld_host_bytes = hostBytes;
+  }
+  return DVC_OK;
+}
+
 // Shared driver for one group of bitplanes with the same code length.
 // sequential != 0: one code per launch in increasing order (the plain path,
 // which mirrors the reference's per-code beliefPropagation_CUDA call).
@@ -207,49 +250,45 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
   Ladder *L = c->ladder(N);
   const int C = L->numCodes, nb = N / 8;
   const bool mode1 = c->bp_mode == 1;
-  DevBuf<float> dLR, dLLR;
-  DevBuf<uint8_t> dAcc, dSlots, dCarry;
-  DevBuf<PlaneDesc> dPlanes;
-  DevBuf<PlaneState> dState;
-  DevBuf<int4> dRec;
-  DevBuf<int> dCounters, dBcase;  // counters: [0] queue head, [1] slot count, [2] bitplanes to resume
   // slots: each (bitplane, code) accepts at most once; ParallelMode 1 keeps
   // every run code's decisions and its repeat-case decisions as well
   const int slotCap = sequential ? (mode1 ? 2 * C + 2 : 2) : (mode1 ? 2 * C * n + 1 : C * n + 1);
-  DVC_HIP(dLR.alloc((size_t)n * N));
-  DVC_HIP(dLLR.alloc((size_t)n * N));
-  DVC_HIP(dAcc.alloc((size_t)n * nb));
-  DVC_HIP(dSlots.alloc((size_t)slotCap * N));
-  DVC_HIP(dPlanes.alloc(n));
-  DVC_HIP(dState.alloc(n));
-  DVC_HIP(dRec.alloc((size_t)n * C));
-  DVC_HIP(dCounters.alloc(3));
-  if (mode1) {
-    DVC_HIP(dCarry.alloc((size_t)n * N));
-    DVC_HIP(dBcase.alloc((size_t)n * C));
-  }
-  std::vector<PlaneDesc> planes(n);
-  std::vector<PlaneState> state(n);
+  const LdpcaLayout o = ldpca_layout(n, N, C, slotCap, mode1);
+  int rc = ldpca_workspace(c, o.total, o.outEnd);
+  if (rc) return rc;
+  uint8_t *dev = (uint8_t *)c->ld_dev, *host = (uint8_t *)c->ld_host;
+  float *dLR = (float *)(dev + o.lr), *dLLR = (float *)(dev + o.llr);
+  uint8_t *dAcc = dev + o.acc, *dCarry = dev + o.carry, *dSlots = dev + o.slots, *dOut = dev + o.out;
+  PlaneDesc *dPlanes = (PlaneDesc *)(dev + o.planes);
+  PlaneState *dState = (PlaneState *)(dev + o.state);
+  int4 *dRec = (int4 *)(dev + o.rec);
+  int *dCounters = (int *)(dev + o.counters);  // [0] queue head, [1] slot count, [2] bitplanes to resume
+  int *dBcase = (int *)(dev + o.bcase);
+  PlaneDesc *hPlanes = (PlaneDesc *)(host + o.planes);
+  PlaneState *hState = (PlaneState *)(host + o.state);
+  int4 *hRec = (int4 *)(host + o.rec);
+  int *hCounters = (int *)(host + o.counters);
   for (int i = 0; i < n; i++) {
     DVC_CHECK(items[i].LR && items[i].acc && items[i].decoded, DVC_EINVAL, "dvc_ldpca: NULL item pointer");
-    DVC_HIP(hipMemcpyAsync(dLR.p + (size_t)i * N, items[i].LR, sizeof(float) * N, hipMemcpyHostToDevice, s));
-    DVC_HIP(hipMemcpyAsync(dAcc.p + (size_t)i * nb, items[i].acc, nb, hipMemcpyHostToDevice, s));
-    if (mode1)
-      DVC_HIP(hipMemcpyAsync(dCarry.p + (size_t)i * N, items[i].decoded, N, hipMemcpyHostToDevice, s));
-    planes[i] = PlaneDesc{N, dLR.p + (size_t)i * N, dLLR.p + (size_t)i * N, dAcc.p + (size_t)i * nb,
-                          (uint32_t)items[i].crc, dState.p + i, dRec.p + (size_t)i * C,
-                          mode1 ? dCarry.p + (size_t)i * N : nullptr, mode1 ? dBcase.p + (size_t)i * C : nullptr};
-    int st0 = items[i].start_level;
+    const int st0 = items[i].start_level;
     DVC_CHECK(st0 < C, DVC_EINVAL, "dvc_ldpca: start_level beyond the ladder");
-    state[i] = PlaneState{st0 >= 0 ? st0 : -1, C - 1, 0.0f, 0};
+    memcpy(host + o.lr + sizeof(float) * i * N, items[i].LR, sizeof(float) * N);
+    memcpy(host + o.acc + (size_t)i * nb, items[i].acc, nb);
+    if (mode1) memcpy(host + o.carry + (size_t)i * N, items[i].decoded, N);
+    hPlanes[i] = PlaneDesc{N, dLR + (size_t)i * N, dLLR + (size_t)i * N, dAcc + (size_t)i * nb,
+                           (uint32_t)items[i].crc, dState + i, dRec + (size_t)i * C,
+                           mode1 ? dCarry + (size_t)i * N : nullptr, mode1 ? dBcase + (size_t)i * C : nullptr};
+    hState[i] = PlaneState{st0 >= 0 ? st0 : -1, C - 1, 0.0f, 0};
+    for (int l = 0; l < C; l++) hRec[(size_t)i * C + l] = make_int4(0, REC_NOT_RUN, -1, 0);
   }
-  std::vector<int4> recInit((size_t)n * C, make_int4(0, REC_NOT_RUN, -1, 0));
-  DVC_HIP(hipMemcpyAsync(dPlanes.p, planes.data(), sizeof(PlaneDesc) * n, hipMemcpyHostToDevice, s));
-  DVC_HIP(hipMemcpyAsync(dState.p, state.data(), sizeof(PlaneState) * n, hipMemcpyHostToDevice, s));
-  DVC_HIP(hipMemcpyAsync(dRec.p, recInit.data(), sizeof(int4) * recInit.size(), hipMemcpyHostToDevice, s));
-  TIMED(c, "ce_llr", s, launch_ce_llr(dPlanes.p, n, 1, c->lad1584, c->lad6336, s));
-  DVC_HIP(hipMemcpyAsync(state.data(), dState.p, sizeof(PlaneState) * n, hipMemcpyDeviceToHost, s));
-  DVC_HIP(hipStreamSynchronize(s));
+  hCounters[0] = hCounters[1] = hCounters[2] = 0;
+  DVC_HIP(hipMemcpyAsync(dev, host, o.inEnd, hipMemcpyHostToDevice, s));
+  TIMED(c, "ce_llr", s, launch_ce_llr(dPlanes, n, 1, c->lad1584, c->lad6336, s));
+  std::vector<PlaneState> state(n);
+  if (sequential) {
+    DVC_HIP(hipMemcpyAsync(state.data(), dState, sizeof(PlaneState) * n, hipMemcpyDeviceToHost, s));
+    DVC_HIP(hipStreamSynchronize(s));
+  }
 
   const int threads = bp_threads(c, N);
   LadderDev ld = ladder_dev(*L);
@@ -264,39 +303,37 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
            level = c->rate_step ? std::min(C - 1, state[i].start + dynstep_next(level - state[i].start)) : level + 1) {
         PlaneState ps{state[i].start, C - 1, state[i].entropy, level};
         int zero[3] = {0, 0, 0};
-        DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
-        DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
-        BpArgs a{ld, dPlanes.p + i, 1, dCounters.p, 1, 1, c->max_iter, dSlots.p, dCounters.p + 1, slotCap, c->bp_poll,
+        DVC_HIP(hipMemcpyAsync(dState + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
+        DVC_HIP(hipMemcpyAsync(dCounters, zero, sizeof(zero), hipMemcpyHostToDevice, s));
+        BpArgs a{ld, dPlanes + i, 1, dCounters, 1, 1, c->max_iter, dSlots, dCounters + 1, slotCap, c->bp_poll,
                  c->bp_mode, c->rate_step};
         TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, 1, s));
-        DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + level, sizeof(r), hipMemcpyDeviceToHost, s));
+        DVC_HIP(hipMemcpyAsync(&r, dRec + (size_t)i * C + level, sizeof(r), hipMemcpyDeviceToHost, s));
         DVC_HIP(hipStreamSynchronize(s));
         if (r.y == REC_ACCEPTED) {
-          DVC_HIP(hipMemcpy(items[i].decoded, dSlots.p + (size_t)r.z * N, N, hipMemcpyDeviceToHost));
+          DVC_HIP(hipMemcpy(items[i].decoded, dSlots + (size_t)r.z * N, N, hipMemcpyDeviceToHost));
           break;
         }
         if (mode1) {
           DVC_CHECK(r.z >= 0 && r.z < slotCap, DVC_ESTATE, "dvc_ldpca: missing decisions of a failed code");
-          DVC_HIP(hipMemcpyAsync(dCarry.p + (size_t)i * N, dSlots.p + (size_t)r.z * N, N, hipMemcpyDeviceToDevice, s));
+          DVC_HIP(hipMemcpyAsync(dCarry + (size_t)i * N, dSlots + (size_t)r.z * N, N, hipMemcpyDeviceToDevice, s));
         }
       }
       if (level == C - 1) {
         PlaneState ps{state[i].start, C - 1, state[i].entropy, C};
         int zero[3] = {0, 0, 0};
-        DVC_HIP(hipMemcpyAsync(dState.p + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
-        DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
-        TIMED(c, "invh", s, launch_invh(dPlanes.p + i, 1, ld, dSlots.p, dCounters.p + 1, slotCap, s));
-        DVC_HIP(hipMemcpyAsync(&r, dRec.p + (size_t)i * C + (C - 1), sizeof(r), hipMemcpyDeviceToHost, s));
+        DVC_HIP(hipMemcpyAsync(dState + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
+        DVC_HIP(hipMemcpyAsync(dCounters, zero, sizeof(zero), hipMemcpyHostToDevice, s));
+        TIMED(c, "invh", s, launch_invh(dPlanes + i, 1, ld, dSlots, dCounters + 1, slotCap, s));
+        DVC_HIP(hipMemcpyAsync(&r, dRec + (size_t)i * C + (C - 1), sizeof(r), hipMemcpyDeviceToHost, s));
         DVC_HIP(hipStreamSynchronize(s));
-        DVC_HIP(hipMemcpy(items[i].decoded, dSlots.p + (size_t)r.z * N, N, hipMemcpyDeviceToHost));
+        DVC_HIP(hipMemcpy(items[i].decoded, dSlots + (size_t)r.z * N, N, hipMemcpyDeviceToHost));
       }
     }
-    DVC_HIP(hipMemcpy(rec.data(), dRec.p, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost));
+    DVC_HIP(hipMemcpy(rec.data(), dRec, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost));
   } else {
-    int zero[3] = {0, 0, 0};
-    DVC_HIP(hipMemcpyAsync(dCounters.p, zero, sizeof(zero), hipMemcpyHostToDevice, s));
     const int maxOff = C - 1;
-    BpArgs a{ld, dPlanes.p, n, dCounters.p, n * maxOff, maxOff, c->max_iter, dSlots.p, dCounters.p + 1, slotCap,
+    BpArgs a{ld, dPlanes, n, dCounters, n * maxOff, maxOff, c->max_iter, dSlots, dCounters + 1, slotCap,
              c->bp_poll, c->bp_mode, c->rate_step};
     const int grid = bp_grid(c, N, n * maxOff);
     TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
@@ -304,30 +341,33 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
       // ParallelMode 1: resolve with the carried decisions; resume the
       // bitplanes that reach a code this launch did not finish
       DVC_CHECK(round <= C, DVC_ESTATE, "dvc_ldpca: ParallelMode 1 resolution does not settle");
-      DVC_HIP(launch_zero_ints(dCounters.p + 2, 1, s));
-      DVC_HIP(launch_mode1_resolve(dPlanes.p, n, C, N, dSlots.p, slotCap, dCounters.p + 2, s));
+      DVC_HIP(launch_zero_ints(dCounters + 2, 1, s));
+      DVC_HIP(launch_mode1_resolve(dPlanes, n, C, N, dSlots, slotCap, dCounters + 2, s));
       int resume = 0;
-      DVC_HIP(hipMemcpyAsync(&resume, dCounters.p + 2, sizeof(int), hipMemcpyDeviceToHost, s));
+      DVC_HIP(hipMemcpyAsync(&resume, dCounters + 2, sizeof(int), hipMemcpyDeviceToHost, s));
       DVC_HIP(hipStreamSynchronize(s));
       DVC_CHECK(resume < (1 << 20), DVC_ENOMEM, "ParallelMode 1: decision slots exhausted");
     if (!resume) break;
-      DVC_HIP(launch_zero_ints(dCounters.p, 1, s));
+      DVC_HIP(launch_zero_ints(dCounters, 1, s));
       TIMED(c, N == 6336 ? "bp_6336" : "bp_1584", s, launch_bp_any(N, threads, a, grid, s));
     }
-    TIMED(c, "invh", s, launch_invh(dPlanes.p, n, ld, dSlots.p, dCounters.p + 1, slotCap, s));
-    int counters[2];
-    DVC_HIP(hipMemcpyAsync(rec.data(), dRec.p, sizeof(int4) * rec.size(), hipMemcpyDeviceToHost, s));
-    DVC_HIP(hipMemcpyAsync(state.data(), dState.p, sizeof(PlaneState) * n, hipMemcpyDeviceToHost, s));
-    DVC_HIP(hipMemcpyAsync(counters, dCounters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
+    TIMED(c, "invh", s, launch_invh(dPlanes, n, ld, dSlots, dCounters + 1, slotCap, s));
+    // every bitplane's accepted decisions to dOut, then states, records,
+    // counters and decisions back in one copy
+    DVC_HIP(launch_gather_accepted(dPlanes, n, N, dSlots, slotCap, dOut, s));
+    DVC_HIP(hipMemcpyAsync(host + o.state, dev + o.state, o.outEnd - o.state, hipMemcpyDeviceToHost, s));
     DVC_HIP(hipStreamSynchronize(s));
-    DVC_CHECK(counters[1] <= slotCap, DVC_ENOMEM, "dvc_ldpca_decode_batch: decision slots exhausted");
+    DVC_CHECK(hCounters[1] <= slotCap, DVC_ENOMEM, "dvc_ldpca_decode_batch: decision slots exhausted");
+    memcpy(state.data(), hState, sizeof(PlaneState) * n);
+    memcpy(rec.data(), hRec, sizeof(int4) * rec.size());
     for (int i = 0; i < n; i++) {
-      int best = state[i].best;
+      const int best = state[i].best;
+      DVC_CHECK(best >= 0 && best < C, DVC_ESTATE, "dvc_ldpca_decode_batch: bad accepted code");
       const int4 r = rec[(size_t)i * C + best];
-      DVC_CHECK(r.y == REC_ACCEPTED && r.z >= 0, DVC_ESTATE, "dvc_ldpca_decode_batch: missing accepted record");
-      DVC_HIP(hipMemcpyAsync(items[i].decoded, dSlots.p + (size_t)r.z * N, N, hipMemcpyDeviceToHost, s));
+      DVC_CHECK(r.y == REC_ACCEPTED && r.z >= 0 && r.z < slotCap, DVC_ESTATE,
+                "dvc_ldpca_decode_batch: missing accepted record");
+      memcpy(items[i].decoded, host + o.out + (size_t)i * N, N);
     }
-    DVC_HIP(hipStreamSynchronize(s));
   }
   // Stats as the sequential search reports them.
   for (int i = 0; i < n; i++) {
@@ -352,6 +392,10 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
       for (int off = 0; st.start_level + off < lvl; off = dynstep_next(off)) st.requests++;
     }
     st.used_inverse = rec[(size_t)i * C + lvl].w;
+    for (int l = 0; l < C; l++) {
+      const int4 r = rec[(size_t)i * C + l];
+      if (r.y != REC_NOT_RUN) st.iterations_run += r.x;
+    }
   }
   return DVC_OK;
 }

@@ -100,5 +100,9 @@ struct dvc_ctx {
   hipStream_t stream = nullptr;
   void *si_scratch = nullptr;  // dvc_side_info_batch's device buffers, kept between calls
   size_t si_scratch_bytes = 0;
+  void *ld_dev = nullptr;      // the LDPCA entry points' device workspace, kept between calls
+  size_t ld_dev_bytes = 0;
+  void *ld_host = nullptr;     // its pinned host staging (inputs up, results back)
+  size_t ld_host_bytes = 0;
   dvc::Ladder *ladder(int N) { return N == 1584 ? &lad1584 : (N == 6336 ? &lad6336 : nullptr); }
 };

@@ -55,8 +55,10 @@ __device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
 // an LDS read would also wait for it.  These views make such loads global.
 typedef const float __attribute__((address_space(1))) gfloat_t;
 typedef const uint8_t __attribute__((address_space(1))) gbyte_t;
+typedef const uint32_t __attribute__((address_space(1))) gword_t;
 __device__ __forceinline__ gfloat_t *as_global(const float *p) { return (gfloat_t *)p; }
 __device__ __forceinline__ gbyte_t *as_global(const uint8_t *p) { return (gbyte_t *)p; }
+__device__ __forceinline__ gword_t *as_global(const uint32_t *p) { return (gword_t *)p; }
 
 // A column's three edges, packed by the ladder upload as 21-bit fields
 // (row | rank-in-row << 13): x = f0 | f1 << 21, y = f1 >> 11 | f2 << 10.
@@ -459,44 +461,98 @@ __global__ __launch_bounds__(64) void mode1_resolve_kernel(const PlaneDesc *plan
 
 // ------------------------------------------------------------------ full rate
 // invH_mul (LdpcaDecoder.cpp:624-643) for every bitplane whose BP codes all
-// failed; one thread per output bit.
-__global__ void invh_kernel(const PlaneDesc *planes, int nplanes, LadderDev L, uint8_t *slots,
-                            int *slot_count, int slot_capacity) {
-  const int p = blockIdx.y;
+// failed, in two launches: invh_slot_kernel gives each such bitplane its
+// decision slot (one thread per bitplane), then invh_kernel multiplies.
+
+__global__ void invh_slot_kernel(const PlaneDesc *planes, int nplanes, int numCodes, int *slot_count) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= nplanes) return;
-  PlaneDesc d = planes[p];
-  PlaneState *st = d.state;
-  if (st->best < L.numCodes - 1) return;
-  const int N = L.N, nw = (N + 31) / 32;
-  __shared__ uint32_t sw[(6336 + 31) / 32];
-  __shared__ int s_slot;
-  // full-rate syndrome: s_k = a_k ^ a_{k-1} (identity transmission pattern)
-  for (int wd = threadIdx.x; wd < nw; wd += blockDim.x) {
-    uint32_t v = 0;
-    for (int q = 0; q < 32; q++) {
-      int k = wd * 32 + q;
-      uint32_t s = 0;
-      if (k < N) s = acc_bit(as_global(d.acc), k) ^ (k > 0 ? acc_bit(as_global(d.acc), k - 1) : 0u);
-      v |= s << (31 - (q & 31));  // MSB-first within the 32-bit chunk
+  const PlaneDesc d = planes[p];
+  if (d.state->best < numCodes - 1) return;
+  d.rec[numCodes - 1] = make_int4(0, REC_ACCEPTED, atomicAdd(slot_count, 1), 1);
+}
+
+// Grid (ceil(N / INVH_ROWS), INVH_COLS).  Every wave finds the bitplanes that
+// need the inverse with a ballot over the plane list; column y takes the
+// k-th of them for k = y, y + INVH_COLS, ...  Workgroup x computes rows
+// [x * INVH_ROWS, + INVH_ROWS) of the product, INVH_ROWS / 4 per wave: a row
+// of invH (N bits, MSB-first bytes, rows padded to 32-bit words) is read as
+// little-endian words by consecutive lanes, ANDed with the byte-swapped
+// syndrome words (a common byte permutation leaves the parity of the AND
+// unchanged), and the parity is the popcount of the wave's ballot.
+constexpr int INVH_ROWS = 64, INVH_COLS = 8, INVH_THREADS = 256;
+constexpr int INVH_MAX_WORDS = (6336 + 31) / 32;  // <= 4 words per lane
+
+__global__ __launch_bounds__(INVH_THREADS) void invh_kernel(const PlaneDesc *planes, int nplanes, LadderDev L,
+                                                            uint8_t *slots, int slot_capacity) {
+  __shared__ uint32_t sw[INVH_MAX_WORDS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int N = L.N, nw = (N + 31) / 32, C = L.numCodes;
+  const int row0 = blockIdx.x * INVH_ROWS + wave * (INVH_ROWS / 4);
+  int k = 0;  // bitplanes needing the inverse seen so far (wave-uniform)
+  for (int base = 0; base < nplanes; base += 64) {
+    const int p = base + lane;
+    const bool need = p < nplanes && !(planes[p].state->best < C - 1);
+    uint64_t mask = __ballot(need);
+    while (mask) {
+      const int b = base + __builtin_ctzll(mask);
+      mask &= mask - 1;
+      if (k++ % INVH_COLS != (int)blockIdx.y) continue;
+      const PlaneDesc d = planes[b];
+      gbyte_t *acc = as_global(d.acc);
+      // full-rate syndrome s_k = a_k ^ a_{k-1} (identity transmission pattern),
+      // as MSB-first 32-bit words, bits at k >= N cleared
+      __syncthreads();  // the previous bitplane's words are no longer read
+      for (int t = tid; t < nw; t += INVH_THREADS) {
+        uint32_t a = 0;
+        for (int q = 0; q < 4; q++) {
+          const int byte = 4 * t + q;
+          a = (a << 8) | (byte < N / 8 ? (uint32_t)acc[byte] : 0u);
+        }
+        const uint32_t prev = t > 0 ? (uint32_t)acc[4 * t - 1] & 1u : 0u;
+        uint32_t s = a ^ (a >> 1) ^ (prev << 31);
+        const int valid = N - 32 * t;
+        if (valid < 32) s &= ~0u << (32 - valid);
+        sw[t] = __builtin_bswap32(s);
+      }
+      __syncthreads();
+      uint32_t w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) w[q] = lane + 64 * q < nw ? sw[lane + 64 * q] : 0u;
+      const int slot = d.rec[C - 1].z;
+      uint8_t *dst = slot >= 0 && slot < slot_capacity ? slots + (size_t)slot * N : nullptr;
+      uint32_t mine = 0;  // parity of row row0 + lane (lane < INVH_ROWS / 4)
+#pragma unroll 4
+      for (int i = 0; i < INVH_ROWS / 4; i++) {
+        const int j = row0 + i;
+        if (j >= N) break;  // wave-uniform
+        const uint32_t *row = (const uint32_t *)(L.invH + (size_t)j * nw * 4);
+        uint32_t x = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (lane + 64 * q < nw) x ^= as_global(row)[lane + 64 * q] & w[q];
+        const uint32_t par = __popcll(__ballot(__popc(x) & 1u)) & 1u;
+        if (lane == i) mine = par;
+      }
+      if (dst && lane < INVH_ROWS / 4 && row0 + lane < N) dst[row0 + lane] = (uint8_t)mine;
     }
-    sw[wd] = v;
   }
-  if (threadIdx.x == 0) {
-    s_slot = atomicAdd(slot_count, 1);
-    d.rec[L.numCodes - 1] = make_int4(0, 2, s_slot, 1);
-  }
-  __syncthreads();
-  const int slot = s_slot;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < N; j += gridDim.x * blockDim.x) {
-    const uint8_t *row = L.invH + (size_t)j * nw * 4;
-    uint32_t acc = 0;
-    for (int wd = 0; wd < nw; wd++) {
-      uint32_t r = ((uint32_t)row[4 * wd] << 24) | ((uint32_t)row[4 * wd + 1] << 16) |
-                   ((uint32_t)row[4 * wd + 2] << 8) | (uint32_t)row[4 * wd + 3];
-      acc ^= r & sw[wd];
-    }
-    if (slot >= 0 && slot < slot_capacity) slots[(size_t)slot * N + j] = (uint8_t)(__popc(acc) & 1);
-  }
+}
+
+// One workgroup per bitplane: its accepted code's decisions from their slot
+// to out + p * N (16-byte words; slots and out are 16-byte aligned).
+__global__ void gather_accepted_kernel(const PlaneDesc *planes, int nplanes, int N, const uint8_t *slots,
+                                       int slot_capacity, uint8_t *out) {
+  const int p = blockIdx.x;
+  if (p >= nplanes) return;
+  const PlaneDesc d = planes[p];
+  const int best = d.state->best;
+  if (best < 0) return;
+  const int4 r = d.rec[best];
+  if (r.y != REC_ACCEPTED || r.z < 0 || r.z >= slot_capacity) return;
+  const uint4 *src = (const uint4 *)(slots + (size_t)r.z * N);
+  uint4 *dst = (uint4 *)(out + (size_t)p * N);
+  for (int i = threadIdx.x; i < N / 16; i += blockDim.x) dst[i] = src[i];
 }
 
 // Clears n ints.  Used instead of hipMemsetAsync for the launch counters so a
@@ -544,10 +600,19 @@ hipError_t launch_ce_llr(const PlaneDesc *d_planes, int n, int fixed_start, cons
 
 hipError_t launch_invh(const PlaneDesc *d_planes, int n, const LadderDev &L, uint8_t *slots, int *slot_count,
                        int slot_capacity, hipStream_t s) {
-  // One workgroup column per bitplane; a single workgroup per bitplane keeps
-  // the slot assignment in blockIdx.x == 0 ordered before its use.
-  hipLaunchKernelGGL(invh_kernel, dim3(1, n), dim3(256), 0, s, d_planes, n, L, slots, slot_count,
-                     slot_capacity);
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(invh_slot_kernel, dim3((n + 63) / 64), dim3(64), 0, s, d_planes, n, L.numCodes, slot_count);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(invh_kernel, dim3((L.N + INVH_ROWS - 1) / INVH_ROWS, INVH_COLS), dim3(INVH_THREADS), 0, s,
+                     d_planes, n, L, slots, slot_capacity);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_accepted(const PlaneDesc *d_planes, int n, int N, const uint8_t *slots, int slot_capacity,
+                                  uint8_t *out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_accepted_kernel, dim3(n), dim3(128), 0, s, d_planes, n, N, slots, slot_capacity, out);
   return hipGetLastError();
 }
 

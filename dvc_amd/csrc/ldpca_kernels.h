@@ -88,5 +88,9 @@ hipError_t launch_ce_llr(const PlaneDesc *d_planes, int n, int fixed_start, cons
                          const Ladder &l6336, hipStream_t s);
 hipError_t launch_invh(const PlaneDesc *d_planes, int n, const LadderDev &L, uint8_t *slots, int *slot_count,
                        int slot_capacity, hipStream_t s);
+// Copies bitplane p's accepted decisions (N bytes, N a multiple of 16) to
+// out + p * N for every p < n; one with no accepted record in range is skipped.
+hipError_t launch_gather_accepted(const PlaneDesc *d_planes, int n, int N, const uint8_t *slots, int slot_capacity,
+                                  uint8_t *out, hipStream_t s);
 
 }  // namespace dvc

@@ -86,6 +86,9 @@ typedef struct {
   int iterations;    /* BP iterations summed over the codes tried (sequential search) */
   int used_inverse;  /* 1 when the full-rate GF(2) inverse produced the bitplane */
   float entropy;     /* conditional entropy H(X|Y) used for start_level */
+  int iterations_run; /* BP iterations the GPU ran for this bitplane: every code launched,
+                         speculative codes above `level` and cancelled ones included
+                         (equals `iterations` on the sequential path) */
 } dvc_bp_stats;
 
 /* One bitplane, sequential rate search: LdpcaDecoder (LdpcaDecoder.cpp:646-771)

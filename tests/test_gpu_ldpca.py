@@ -47,20 +47,47 @@ def test_batch_equals_plain(ctx):
     for pl, dec, st in zip(planes, decs, sts):
         pdec, pst = ctx.ldpca_decode(pl["LR"], pl["acc"], pl["crc"])
         assert np.array_equal(dec, pdec)
+        assert pst["iterations_run"] == pst["iterations"]  # the plain search runs only the codes it tries
+        assert st["iterations_run"] >= st["iterations"]
+        st.pop("iterations_run"), pst.pop("iterations_run")
         assert st == pst
 
 
-def test_forced_inverse_on_bad_crc(ctx):
+@pytest.mark.parametrize("n", [1584, 6336])
+def test_forced_inverse_on_bad_crc(ctx, n):
     """A wrong CRC makes every BP code fail the CRC test; the search ends at
     the full-rate inverse (LdpcaDecoder.cpp:695-716), which is not
     CRC-checked and still returns the source."""
-    pl = W.ldpca_bitplane_set(n_cif=0, n_qcif=1, seed=14)[0]
+    pl = W.ldpca_bitplane_set(n_cif=int(n == 6336), n_qcif=int(n == 1584), seed=14)[0]
     dec, st = ctx.ldpca_decode(pl["LR"], pl["acc"], (pl["crc"] + 1) & 0xFF)
     odec, ost, _ = O.ldpca_decode(pl["LR"], pl["acc"], (pl["crc"] + 1) & 0xFF)
     assert st["used_inverse"] == 1 and ost["used_inverse"] == 1
-    assert st["level"] == 64 and st["rate"] == 1584 + 8
+    assert st["level"] == 64 and st["rate"] == n + 8
     assert np.array_equal(dec, pl["bits"]) and np.array_equal(dec, odec)
     assert st["iterations"] == ost["total_iters"]
+
+
+def test_batched_inverse_many_planes(ctx):
+    """Ten bitplanes of each length need the full-rate inverse in one batch
+    (more than the inverse kernel's eight plane columns, so columns take
+    several), beside two of each that converge: every plane equals the
+    oracle and the source."""
+    planes = W.ldpca_bitplane_set(n_cif=12, n_qcif=12, seed=16)
+    seen = {1584: 0, 6336: 0}
+    crcs = []
+    for p in planes:
+        n = len(p["LR"])
+        crcs.append((p["crc"] + (seen[n] < 10)) & 0xFF)
+        seen[n] += 1
+    decs, sts = ctx.ldpca_decode_batch([p["LR"] for p in planes], [p["acc"] for p in planes], crcs)
+    inv = 0
+    for pl, crc, dec, st in zip(planes, crcs, decs, sts):
+        odec, ost, oent = O.ldpca_decode(pl["LR"], pl["acc"], crc)
+        assert st["used_inverse"] == ost["used_inverse"]
+        assert (st["level"], st["rate"], st["iterations"]) == (ost["level"], ost["rate"], ost["total_iters"])
+        assert np.array_equal(dec, odec) and np.array_equal(dec, pl["bits"])
+        inv += st["used_inverse"]
+    assert inv == 20
 
 
 def test_fixed_start_level(ctx):
