@@ -510,10 +510,12 @@ __global__ void k_smf_init(const FrameDev *F, const int *list) {
 // non-negative sum and is skipped when a is finite.  The exponent depends on
 // the integer distance d = |k - range - SI_m| only, so each workgroup first
 // tabulates expf(-a*(float)d) for the distances its block can reach (the same
-// values, bit for bit) and the mixture sums then read the table.  One
-// workgroup per (4x4 block, frame).
-#define ND_TAB 6144  // table entries (24 KB of LDS); larger distances call expf directly
-__global__ __launch_bounds__(512) void k_noise_dist(const FrameDev *F, const int *list) {
+// values, bit for bit) and the mixture sums then read the table.  One wave
+// per (4x4 block, frame): a block has 2*range+1 <= ~520 values at most
+// Qindex 8, and the small workgroup with a 4 KB table lets a CU hold many
+// blocks at once to hide the LDS and global latencies of each one's phases.
+#define ND_TAB 1024  // table entries (4 KB of LDS); larger distances call expf directly
+__global__ __launch_bounds__(64) void k_noise_dist(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int b = blockIdx.x;
   if (b >= f.N) return;
@@ -912,7 +914,7 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
 // updateSMF (motionLearning.cpp:70-127): p = expf(-MU*SSE) over the +-4
 // window, normalised by the float sum in raster order.  One wave per 4x4
 // block: lane l evaluates window positions l and l+64; the raster-order sum
-// is then formed identically on every lane from shuffled values.
+// is then formed identically on every lane from the lanes' values.
 __global__ __launch_bounds__(256) void k_update_smf(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -943,8 +945,13 @@ __global__ __launch_bounds__(256) void k_update_smf(const FrameDev *F, const int
       v[h] = dvc_expf(-0.01f * sse);
     }
   }
+  // the window is the wave's own (uniform): each term is read from its lane
+  // into a scalar register, with no LDS round trip between the ordered adds
+  const int len = __builtin_amdgcn_readfirstlane(s.len);
+  const int len0 = len < 64 ? len : 64;
   float sum = 0.0f;
-  for (int p = 0; p < s.len; p++) sum += __shfl(p < 64 ? v[0] : v[1], p & 63);
+  for (int p = 0; p < len0; p++) sum += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[0]), p));
+  for (int p = 64; p < len; p++) sum += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[1]), p - 64));
   float *pr = f.smf + (size_t)81 * b;
   const bool norm = (double)sum > 0.0;
 #pragma unroll
@@ -956,16 +963,19 @@ __global__ __launch_bounds__(256) void k_update_smf(const FrameDev *F, const int
 
 // update_si_ML (motionLearning.cpp:129-160): SMF-weighted over-complete
 // coefficients for the bands after the frame's current band, accumulated in
-// window raster order and truncated.  16 lanes per 4x4 block, one per
-// coefficient index, so a window position's 16 coefficients are one
-// contiguous 64-byte load; lanes of already decoded bands idle.
-__constant__ int c_zigpos[16] = {0, 1, 5, 6, 2, 4, 7, 12, 3, 8, 11, 13, 9, 10, 14, 15};  // inverse of c_zigzag
+// window raster order and truncated.  One lane per (4x4 block, coefficient of
+// a band not yet decoded): the 16 - (band + 1) lanes of a block read the same
+// 64-byte coefficient record per window position.  A window row's (at most 9)
+// weights and coefficients are loaded before they are added in order.
 __global__ __launch_bounds__(256) void k_update_si_ml(const FrameDev *F, const int *list, int) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int bandStart = f.band + 1;  // update_si_ML(..., band+1, ...) (DVC_Decoder.cpp:369)
+  const int cnt = 16 - bandStart;    // coefficients still to predict per block
+  if (cnt <= 0) return;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = t >> 4, index = t & 15;
-  if (b >= f.N || c_zigpos[index] < bandStart) return;
+  const int b = t / cnt;
+  if (b >= f.N) return;
+  const int index = c_zigzag[bandStart + t % cnt];
   const int W = f.W, ow = W - 3;
   const SmfWin s = smf_window(f, b);
   const float *pr = f.smf + (size_t)81 * b;
@@ -973,7 +983,17 @@ __global__ __launch_bounds__(256) void k_update_si_ml(const FrameDev *F, const i
   int p = 0;
   for (int m = s.sT; m <= s.sB; m++) {
     const int *o = f.oct + (size_t)16 * (m * ow + s.sL) + index;
-    for (int q = 0; q < s.w; q++) acc += pr[p++] * (float)o[16 * q];
+    float pv[9] = {}, ov[9] = {};
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+      if (q < s.w) {
+        pv[q] = pr[p + q];
+        ov[q] = (float)o[16 * q];
+      }
+#pragma unroll
+    for (int q = 0; q < 9; q++)
+      if (q < s.w) acc += pv[q] * ov[q];
+    p += s.w;
   }
   f.sideTrans[(s.top + (index >> 2)) * W + s.left + (index & 3)] = (int)acc;
 }
@@ -1002,9 +1022,8 @@ hipError_t launch_oct(const FrameDev *F, const int *list, int n, int maxPos, int
   return hipGetLastError();
 }
 hipError_t launch_noise_dist(const FrameDev *F, const int *list, int n, int maxN, int maxStride, hipStream_t s) {
-  // a thread per distribution value where that fits (64..512 threads)
-  const int threads = std::min(512, std::max(64, (maxStride + 63) / 64 * 64));
-  hipLaunchKernelGGL(k_noise_dist, dim3(maxN, n), dim3(threads), 0, s, F, list);
+  (void)maxStride;  // one wave per block whatever the band's value count
+  hipLaunchKernelGGL(k_noise_dist, dim3(maxN, n), dim3(64), 0, s, F, list);
   return hipGetLastError();
 }
 hipError_t launch_condbit(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s) {
