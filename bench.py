@@ -7,7 +7,7 @@ Workloads
                 CIF 352x288, 33 frames, GOP 2, Qindex 8 (16 WZ frames, 63
                 bitplanes each, N = 6336), synthetic sequence as described in
                 dvc_amd.workload.  One step decodes `--streams` such sequences
-                (16 WZ frames each, 32 sequences = 512 frames by default) through
+                (16 WZ frames each, 64 sequences = 1024 frames by default) through
                 one long-lived continuous-batching engine (2 lanes that step
                 independently) with the speculative persistent LDPCA launch;
                 value = WZ frames/s.  The plain path (one frame at a time,
@@ -559,7 +559,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch", "hbm_copy", "serve", "si_batch"])
-    ap.add_argument("--streams", type=int, default=32, help="CIF sequences of 16 WZ frames decoded per step")
+    ap.add_argument("--streams", type=int, default=64, help="CIF sequences of 16 WZ frames decoded per step")
     ap.add_argument("--plain-frames", type=int, default=4)
     ap.add_argument("--max-iter", type=int, default=100)
     ap.add_argument("--bp-threads", type=int, default=0, help="threads per CIF BP workgroup (0: library default)")
@@ -579,7 +579,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(streams shard across GPUs without any exchange; see DESIGN.md).", file=sys.stderr)
+              "(streams would shard across GPUs without any exchange).", file=sys.stderr)
         sys.exit(2)
     res = {"wz_cif_q8": bench_wz, "ldpca_batch": bench_ldpca, "hbm_copy": bench_copy,
            "serve": bench_serve, "si_batch": bench_si}[args.workload](args)
