@@ -118,3 +118,26 @@ def test_autotune_and_every_tuning_decode_alike():
             assert np.array_equal(d0, d1) and np.array_equal(d1, p["bits"])
             assert (s0["level"], s0["iterations"]) == (s1["level"], s1["iterations"])
     base.close()
+
+
+def test_workspace_reused_across_call_sizes(ctx):
+    """The context's LDPCA workspace grows to the largest call and is reused:
+    a large batch, a small one, a plain call and the large batch again give
+    the oracle's results every time, and the repeat equals the first."""
+    big = W.ldpca_bitplane_set(n_cif=8, n_qcif=8, seed=17)
+    small = W.ldpca_bitplane_set(n_cif=2, n_qcif=1, seed=18)
+    args = lambda ps: ([p["LR"] for p in ps], [p["acc"] for p in ps], [p["crc"] for p in ps])
+    d1, s1 = ctx.ldpca_decode_batch(*args(big))
+    d2, s2 = ctx.ldpca_decode_batch(*args(small))
+    for pl, dec, st in zip(small, d2, s2):
+        odec, ost, oent = O.ldpca_decode(pl["LR"], pl["acc"], pl["crc"])
+        _check(dec, st, pl, odec, ost, oent)
+    pl = small[0]
+    dec, st = ctx.ldpca_decode(pl["LR"], pl["acc"], pl["crc"])
+    assert np.array_equal(dec, d2[0]) and st["level"] == s2[0]["level"]
+    d3, s3 = ctx.ldpca_decode_batch(*args(big))
+    for pl, a, b, sa, sb in zip(big, d1, d3, s1, s3):
+        # iterations_run includes cancelled speculative work, which depends on scheduling
+        sa.pop("iterations_run"), sb.pop("iterations_run")
+        assert np.array_equal(a, b) and sa == sb
+        assert np.array_equal(a, pl["bits"])
