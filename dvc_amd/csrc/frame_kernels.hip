@@ -849,8 +849,12 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
       const uint32_t ps = sad4(bk, p0, p1, p2, p3), fs = sad4(bk, f0, f1, f2, f3);
       const double fac = facRow[33 * abs(y - top)];
       const double pc = (double)ps * fac, fc = (double)fs * fac;
-      if (pc < pBest) { pBest = pc; pIdx = idx; }
-      if (fc < fBest) { fBest = fc; fIdx = idx; }
+      // costs are positive and never NaN: the running minimum is fmin, and
+      // only the index needs the strict comparison
+      pIdx = pc < pBest ? idx : pIdx;
+      fIdx = fc < fBest ? idx : fIdx;
+      pBest = fmin(pBest, pc);
+      fBest = fmin(fBest, fc);
       p0 = p1; p1 = p2; p2 = p3;
       f0 = f1; f1 = f2; f2 = f3;
     }
