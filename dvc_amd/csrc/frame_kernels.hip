@@ -572,13 +572,20 @@ __global__ __launch_bounds__(64) void k_noise_dist(const FrameDev *F, const int 
     }
     for (int m = tid; m < n; m += T) s_ps[m].y = D - range - s_ps[m].y;
     __syncthreads();
-    for (int k = tid; k < rmax; k += T) {
-      float sum = 0.0f;
+    // two values per lane (k0 and k0 + T) share each term's read and carry
+    // independent sums; each value still adds its terms in window order
+    for (int k0 = tid; k0 < rmax; k0 += 2 * T) {
+      const bool two = k0 + T < rmax;
+      const int k1 = two ? k0 + T : k0;  // a lane without a second value repeats the first
+      float s0 = 0.0f, s1 = 0.0f;
       for (int m = 0; m < n; m++) {
         const int2 ps = s_ps[m];
-        sum += __int_as_float(ps.x) * s_tab[k + ps.y];
+        const float pr = __int_as_float(ps.x);
+        s0 += pr * s_tab[k0 + ps.y];
+        s1 += pr * s_tab[k1 + ps.y];
       }
-      out[k] = sum;
+      out[k0] = s0;
+      if (two) out[k1] = s1;
     }
   } else {
     const int need = D + 1;
