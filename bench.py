@@ -10,9 +10,15 @@ Workloads
                 (16 WZ frames each, 64 sequences = 1024 frames by default) through
                 one long-lived continuous-batching engine (2 lanes that step
                 independently) with the speculative persistent LDPCA launch;
-                value = WZ frames/s.  The plain path (one frame at a time,
-                sequential rate search) is timed on one sequence beside it, and
-                the same workload again with ParallelMode 1 BP semantics.
+                value = WZ frames/s, timed without per-kernel events (steady
+                steps replay captured HIP graphs), with per-frame latency.
+                Beside it: one more step with HIP events around every launch
+                (kernel durations, the roofline), the plain path (one frame in
+                flight through a long-lived engine, sequential rate search) on
+                --plain-frames frames, the same workload with ParallelMode 1
+                BP semantics and with host buffers (PCIe-inclusive), a bounded
+                run of the serving configuration, and the VALU issue-rate
+                calibration that prices the BP roofline.
   ldpca_batch   512 seeded bitplanes (384 x N=6336, 128 x N=1584) decoded in
                 one speculative persistent launch per step; one step = the
                 whole set.  value = BP edge-updates/s, counting the
@@ -20,14 +26,16 @@ Workloads
                 one edge's check-to-variable plus variable-to-check update in
                 one iteration; E = 3N per code).
 
-  serve, si_batch, hbm_copy: the serving loop, batched side information and
-                a device copy (the HBM traffic calibration).
+  serve, si_batch, hbm_copy, valu_peak: the serving loop, batched side
+                information, a device copy (the HBM traffic calibration) and
+                the VALU issue-rate calibration.
 
 Prints one JSON line (see the end of main()).  The CPU baseline leg runs the
 oracle (ParallelMode 1 over the host cores, and ParallelMode 0 on one core)
 on a bounded sample of the same workload.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -42,7 +50,6 @@ sys.path.insert(0, str(ROOT))
 METRIC = "WZ frames/sec and BP edge-updates/s (batched vs plain, 1 GPU); p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz
-VALU_PEAK_WAVE_INSTR = 256 * 2 * 2.4e9  # 256 CUs x 2 wave64 fp32 instructions/clk x 2.4 GHz
 
 
 def log(*a):
@@ -107,7 +114,8 @@ def bench_ldpca(args):
         "dtype": "f32", "data": "synthetic (seeded Bernoulli bitplanes, Laplacian soft inputs)",
         "config": {"workload": "ldpca_batch: 512 bitplanes (384 N=6336 + 128 N=1584), all codes speculative, "
                                f"max_iter={args.max_iter}", "parallelism": "single GPU"},
-        "roofline": bp_roofline(bp_avg_ms, hbm_bytes / per_step, lds_bytes / 16.0 / per_step, "ldpca_batch"),
+        "roofline": bp_roofline(bp_avg_ms, hbm_bytes / per_step, lds_bytes / 16.0 / per_step, "ldpca_batch",
+                                valu_calibration(ctx)),
         "kernels_ms": {"bp_6336": bp_ms / max(bp_n, 1), "bp_1584": bpq_ms / max(bpq_n, 1),
                        "ce_llr": ce_ms / max(ce_n, 1)},
         "wrong_bitplanes": bad,
@@ -151,12 +159,15 @@ class DeviceInputs:
             self.rec.append(_lib.DeviceBuffer(ctx, j["past"].size))
         self.shapes = [j["past"].shape for j in jobs]
 
-    def submit_all(self, eng, jobs, gen=0):
-        """Submits the frames; key frames go in under keys of generation
-        `gen`, so a later generation brings them in (and filters them) again
-        rather than finding them resident."""
+    def submit_all(self, eng, jobs, gen=0, index=None):
+        """Submits the frames (jobs[k] is the workload's frame index[k], all
+        of them by default); key frames go in under keys of generation `gen`,
+        so a later generation brings them in (and filters them) again rather
+        than finding them resident."""
+        index = range(len(jobs)) if index is None else index
         tickets = []
-        for j, acc, rec in zip(jobs, self.acc, self.rec):
+        for j, i in zip(jobs, index):
+            acc, rec = self.acc[i], self.rec[i]
             kp = ((j["seq"] + 1) << 32) | j["idx"][0]
             kf = ((j["seq"] + 1) << 32) | j["idx"][2]
             h, w = j["past"].shape
@@ -173,13 +184,19 @@ class DeviceInputs:
         return self.rec[i].download(self.shapes[i], np.uint8)
 
 
-def run_engine(eng, tickets):
+def run_engine(eng, tickets, t_submit=None, lat=None):
+    """Steps the engine until every ticket is done; returns the frames'
+    stats.  With t_submit (perf_counter time of the submissions) and a list
+    `lat`, appends each frame's latency from submission to completion."""
     pending = set(tickets)
     while pending:
         eng.step()
+        now = time.perf_counter()
         for t in list(pending):
             if eng.done(t):
                 pending.discard(t)
+                if lat is not None:
+                    lat.append(now - t_submit)
     return [eng.result(t)[1] for t in tickets]
 
 
@@ -187,6 +204,11 @@ def psnr(a, b):
     """frame.cpp:115-124."""
     mse = float(np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2))
     return 0.0 if mse == 0.0 else 10.0 * np.log10(255.0 * 255.0 / mse)
+
+
+KERNELS = ["create_si", "keyframe_filters", "noise_param", "oct", "smf_init", "noise_dist", "condbit", "ce_llr",
+           "bp_6336", "invh", "put_bitplane", "reconstruct", "inverse_transform", "update_si", "update_smf",
+           "update_si_ml"]
 
 
 def bench_wz(args):
@@ -208,48 +230,41 @@ def bench_wz(args):
         dev.release_keys(eng, jobs, gen)
         gen += 1
         log(f"[bench] warmup {i} done")
-    ctx.profile(True)
-    times = []
+    # timed steps: no per-kernel timing (graph replay of steady steps on)
+    times, frame_lat = [], []
     for i in range(args.steps):
         t = time.perf_counter()
-        stats = run_engine(eng, dev.submit_all(eng, jobs, gen))
+        stats = run_engine(eng, dev.submit_all(eng, jobs, gen), t, frame_lat)
         times.append(time.perf_counter() - t)
         dev.release_keys(eng, jobs, gen)  # a server releases a stream's key frames once no frame needs them
         gen += 1
         log(f"[bench] step {i}: {times[-1] * 1e3:.1f} ms for {len(jobs)} frames")
-    eng.close()
     outs = [(dev.recons(i), dict(st, psnr=psnr(dev.recons(i), j["orig"]))) for i, (j, st) in enumerate(zip(jobs, stats))]
-    # per-kernel HIP-event totals of the timed steps only
-    names = ["create_si", "keyframe_filters", "noise_param", "oct", "smf_init", "noise_dist", "condbit", "ce_llr",
-             "bp_6336", "invh", "put_bitplane", "reconstruct", "inverse_transform", "update_si", "update_smf",
-             "update_si_ml"]
+    # kernel durations: the same step again with HIP events around every
+    # launch on the stream it runs on (this turns graph replay off)
     kern = {}
-    for nm in names:
+    ctx.profile(True)
+    for i in range(args.kernel_steps):
+        t = time.perf_counter()
+        run_engine(eng, dev.submit_all(eng, jobs, gen))
+        dev.release_keys(eng, jobs, gen)
+        gen += 1
+        log(f"[bench] kernel-timing step {i}: {(time.perf_counter() - t) * 1e3:.1f} ms")
+    for nm in KERNELS:
         ms, cnt = ctx.profile_get(nm)
         if cnt:
-            kern[nm] = {"total_ms": ms / args.steps, "launches": cnt // args.steps}
+            kern[nm] = {"total_ms": ms / args.kernel_steps, "launches": cnt // args.kernel_steps,
+                        "avg_launch_ms": ms / cnt}
     ctx.profile(False)
+    eng.close()
     mode1 = bench_wz_mode1(args, jobs, dev) if args.mode1_steps else None
     pcie = bench_wz_host_buffers(args, ctx, jobs) if args.pcie_steps else None
-    # correctness of the measured outputs against the encoder's source planes:
-    # every decoded bitplane must equal the source (a CRC-accepted or full-rate
-    # decode), which the parity tests pin bit-exactly against the reference.
+    plain = bench_wz_plain(args, ctx, jobs, dev, outs) if args.plain_frames else None
+    serve_leg = bench_serve_leg(args, ctx) if args.serve_rate > 0 else None
+    valu = valu_calibration(ctx)
     psnrs = [st["psnr"] for _, st in outs]
-    # plain drop-in path on the first sequence: one frame at a time
-    plain_n = min(args.plain_frames, len(jobs))
-    lat = []
-    plain_mismatch = 0
-    for i, j in enumerate(jobs[:plain_n]):
-        t = time.perf_counter()
-        (prec, pst), = ctx.decode_wz_frames([j], batched=False)
-        lat.append(time.perf_counter() - t)
-        plain_mismatch += int(not (np.array_equal(prec, outs[i][0]) and pst["levels"] == outs[i][1]["levels"]
-                                   and pst["psnr"] == outs[i][1]["psnr"]))
-    if plain_n:
-        log(f"[bench] plain: {plain_n} frames, mean {np.mean(lat) * 1e3:.1f} ms/frame")
     t_step = float(np.mean(times))
     value = len(jobs) / t_step
-    plain_fps = plain_n / float(np.sum(lat)) if plain_n else None
     eu = sum(sum(st["iterations"]) * 3 * 6336 for _, st in outs)
     dom = max(kern.items(), key=lambda kv: kv[1]["total_ms"]) if kern else ("", {"total_ms": 0, "launches": 1})
     res = {
@@ -260,22 +275,109 @@ def bench_wz(args):
         "config": {"workload": f"wz_cif_q8: CIF 352x288, 33 frames, GOP 2, Qindex 8, {args.streams} sequences "
                                f"({len(jobs)} WZ frames, 63 bitplanes each) per step, batched, max_iter={args.max_iter}",
                    "parallelism": "single GPU"},
-        "plain_fps": plain_fps, "batched_over_plain": value / plain_fps if plain_fps else None,
+        "plain_fps": plain["fps"] if plain else None,
+        "batched_over_plain": value / plain["fps"] if plain else None,
+        "plain": plain,
         "bp_edge_updates_per_s": eu / t_step,
-        "latency_ms": {"plain_p50": float(np.percentile(lat, 50) * 1e3) if lat else None,
-                       "plain_p99": float(np.percentile(lat, 99) * 1e3) if lat else None,
-                       "batch_p50": float(np.percentile(times, 50) * 1e3), "batch_p99": float(np.percentile(times, 99) * 1e3)},
+        "latency_ms": {"frame_p50": float(np.percentile(frame_lat, 50) * 1e3),
+                       "frame_p99": float(np.percentile(frame_lat, 99) * 1e3),
+                       "frames_sampled": len(frame_lat),
+                       "plain_frame_p50": plain["latency_ms"]["p50"] if plain else None,
+                       "plain_frame_p99": plain["latency_ms"]["p99"] if plain else None,
+                       "note": "batched: each frame from the submission of its step's batch to its completion"},
         "psnr_mean": float(np.mean(psnrs)),
-        "plain_vs_batched_mismatches": plain_mismatch,
+        "recon_host_roundtrips": 0,  # reconstruction's exp runs on the device (dvc_exp)
         "inputs": "resident in HBM before the timed region (key frames, syndromes, output frames)",
+        "timing": "timed steps run without per-kernel events (steady steps replay captured HIP graphs); "
+                  f"kernel durations come from {args.kernel_steps} further step(s) of the same workload with HIP "
+                  "events around each launch on its stream",
         "kernels_ms_per_step": kern, "dominant_kernel": dom[0], "bp_tuning": tuning,
         "parallel_mode_1": mode1,
         "pcie_inclusive": pcie,
+        "serve": serve_leg,
+        "valu_calibration": valu,
     }
-    res["roofline"] = roofline_wz(dom, kern, args, len(jobs), outs)
+    res["roofline"] = roofline_wz(dom, kern, args, len(jobs), outs, valu)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline_wz(jobs, args)
     return res
+
+
+def bench_wz_plain(args, ctx, jobs, dev, outs):
+    """The plain drop-in path: one frame at a time (an engine with one frame
+    in flight) with the sequential rate search (one BP launch per code, the
+    host choosing the next code), inputs resident in HBM as in the batched
+    leg, through one long-lived engine.  Also one dvc_decode_wz_frames call
+    (batched = 0), which sets up and tears down its own engine, timed
+    separately as the per-call set-up cost."""
+    from dvc_amd import _lib
+    n = min(args.plain_frames, len(jobs))
+    eng = _lib.Engine(ctx, max_inflight=1, lanes=1, sequential_bp=True, device_buffers=True)
+    sub = jobs[:n]
+    run_engine(eng, dev.submit_all(eng, sub[:1], 3000, [0]))  # warm the engine's pool
+    dev.release_keys(eng, sub[:1], 3000)
+    lat, mism = [], 0
+    t_all = time.perf_counter()
+    for i in range(n):
+        t = time.perf_counter()
+        (st,) = run_engine(eng, dev.submit_all(eng, [sub[i]], 3001 + i, [i]))
+        lat.append(time.perf_counter() - t)
+        dev.release_keys(eng, [sub[i]], 3001 + i)
+        rec = dev.recons(i)
+        mism += int(not (np.array_equal(rec, outs[i][0]) and st["levels"] == outs[i][1]["levels"]))
+    dt = time.perf_counter() - t_all
+    eng.close()
+    t = time.perf_counter()
+    (crec, cst), = ctx.decode_wz_frames([jobs[0]], batched=False)
+    call_ms = (time.perf_counter() - t) * 1e3
+    mism += int(not (np.array_equal(crec, outs[0][0]) and cst["psnr"] == outs[0][1]["psnr"]))
+    log(f"[bench] plain: {n} frames, mean {np.mean(lat) * 1e3:.1f} ms/frame; one dvc_decode_wz_frames call "
+        f"{call_ms:.1f} ms")
+    return {"fps": n / dt, "frames": n, "ms_per_frame": float(np.mean(lat) * 1e3),
+            "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3), "p99": float(np.percentile(lat, 99) * 1e3)},
+            "engine": "long-lived, max_inflight 1, sequential_bp 1, HBM-resident inputs",
+            "decode_wz_frames_call_ms": call_ms,
+            "mismatches_vs_batched": mism}
+
+
+def bench_serve_leg(args, ctx):
+    """A bounded run of the serving configuration beside the headline: 32
+    streams (24 QCIF + 8 CIF, GOP U{2,4,8}, Qindex U{1..8}), Poisson arrivals
+    at --serve-rate requests/s, with a sample of served frames checked
+    against their decodes alone."""
+    from dvc_amd import serve
+    t0 = time.time()
+    streams = serve.make_streams(frames=args.serve_leg_frames, seed=4001)
+    log(f"[bench] serve leg: encoded {sum(len(s.order) for s in streams)} frames in {time.time() - t0:.1f}s")
+    res, _ = serve.run(ctx, streams, args.serve_rate, max_inflight=args.serve_inflight, lanes=args.lanes,
+                       verify=serve.verify_solo(ctx, sample=8))
+    log(f"[bench] serve leg: {res['frames_per_s']:.1f} frames/s, p50 {res['latency_ms']['p50']:.0f} ms, "
+        f"p99 {res['latency_ms']['p99']:.0f} ms, {res['verified']}")
+    res["config"] = (f"32 streams (24 QCIF + 8 CIF), {args.serve_leg_frames} frames each, GOP U{{2,4,8}}, "
+                     f"Qindex U{{1..8}}, seed 4001, Poisson arrivals at {args.serve_rate:.0f} requests/s")
+    return res
+
+
+def valu_calibration(ctx, waves_per_cu=16, iters=4000):
+    """Measured VALU issue rates (wave64 instructions/s) of dependent-free
+    chains built with the library's flags (packed FP32 off): the peaks the
+    BP roofline is priced against."""
+    from dvc_amd import _lib
+    out = {}
+    for kind, name in enumerate(["fp32_fma", "fp32_add", "fp32_mul", "fp64_fma", "int32_add", "sad_u8", "mqsad_u32_u8"]):
+        ms, wi = ctypes.c_double(), ctypes.c_double()
+        _lib._check(_lib.lib.dvc_selftest_valu_rate(ctx.h, kind, waves_per_cu, iters, ctypes.byref(ms),
+                                                     ctypes.byref(wi)), "dvc_selftest_valu_rate")
+        out[name] = {"wave_instr_per_s": wi.value / (ms.value * 1e-3), "kernel_ms": ms.value,
+                     "wave_instr": wi.value}
+    # the BP roofline's peak: the highest rate of the arithmetic kinds (the SAD
+    # kinds price the SI refresh: 4 and 16 absolute differences per lane)
+    out["peak_wave_instr_per_s"] = max(out[k]["wave_instr_per_s"] for k in
+                                       ("fp32_fma", "fp32_add", "fp32_mul", "fp64_fma", "int32_add"))
+    out["sad_abs_diffs_per_s"] = {"sad_u8": 64 * 4 * out["sad_u8"]["wave_instr_per_s"],
+                                  "mqsad_u32_u8": 64 * 16 * out["mqsad_u32_u8"]["wave_instr_per_s"]}
+    out["per_cu_cycle_at_2.4GHz"] = out["peak_wave_instr_per_s"] / (256 * 2.4e9)
+    return out
 
 
 def bench_wz_host_buffers(args, ctx, jobs):
@@ -356,33 +458,41 @@ def profiled_value(kernel, workload, key):
     return None
 
 
-def bp_roofline(kernel_ms_avg, hbm_bytes_per_launch, edge_updates_per_launch, workload):
-    """Roofline of the BP kernel.  HBM: per (bitplane, code) the sequential
-    search runs, 4N (LLR) + ceil(m/8) (syndrome) + ceil(N/8) (decisions) + 8N
-    (packed column edges).  LDS: 16 bytes per edge-update (v2c read and
-    write, c2v read and write).  VALU: the profiled VALU wave-instructions
-    per launch over the live launch time, against the fp32 issue peak (two
-    wave64 instructions per CU-cycle: 157.3 TFLOP/s / (256 CUs x 2.4 GHz x
-    64 lanes x 2)); the exact glibc-identical transcendentals make this the
-    bound."""
+def bp_roofline(kernel_ms_avg, hbm_bytes_per_launch, edge_updates_per_launch, workload, valu=None):
+    """Roofline of the BP kernel.  Its bound is the VALU issue port: the exact
+    glibc-identical transcendentals (two phi per edge-update) are about 200
+    VALU instructions per edge-update, against 16 LDS bytes and a few HBM
+    bytes.  achieved = the profiled VALU wave64 instructions per launch (the
+    committed rocprofv3 SQ_INSTS_VALU pass of this workload) over the average
+    launch duration measured here; peak = the issue rate measured by
+    valu_calibration (dependent-free chains built with the library's flags).
+    Also reported: HBM (per (bitplane, code) the sequential search runs, 4N
+    LLR + ceil(m/8) syndrome + ceil(N/8) decisions + 8N packed column edges)
+    against 8 TB/s, with the profiled traffic, and LDS (16 bytes per
+    edge-update) against 256 CUs x 128 B/clk x 2.4 GHz."""
     t = kernel_ms_avg * 1e-3
     hbm = hbm_bytes_per_launch / t / 1e9
     lds = 16.0 * edge_updates_per_launch / t / 1e9
-    res = {"bound": "hbm", "kernel": "bp_6336", "achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": hbm / HBM_PEAK_GBS, "traffic": profiled_traffic("bp_6336", workload),
-           "kernel_ms_avg": kernel_ms_avg, "algorithmic_bytes_per_launch": hbm_bytes_per_launch,
+    vi = profiled_value("bp_6336", workload, "valu_wave_instr_per_launch")
+    res = {"kernel": "bp_6336", "kernel_ms_avg": kernel_ms_avg,
+           "traffic": profiled_traffic("bp_6336", workload),
+           "hbm": {"achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm / HBM_PEAK_GBS,
+                   "algorithmic_bytes_per_launch": hbm_bytes_per_launch},
            "lds": {"achieved": lds, "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": lds / LDS_PEAK_GBS,
                    "bytes_per_launch": 16.0 * edge_updates_per_launch},
            "profile": f"profiles/bp_6336_{workload}.md"}
-    vi = profiled_value("bp_6336", workload, "valu_wave_instr_per_launch")
-    if vi:
-        peak = VALU_PEAK_WAVE_INSTR
-        res["valu"] = {"achieved": vi / t, "peak": peak, "unit": "wave-instr/s", "frac": vi / t / peak,
-                       "wave_instr_per_launch": vi}
+    if vi and valu:
+        peak = valu["peak_wave_instr_per_s"]
+        res.update({"bound": "valu", "achieved": vi / t, "peak": peak, "unit": "VALU wave64-instr/s",
+                    "frac": vi / t / peak, "wave_instr_per_launch": vi,
+                    "peak_source": "valu_calibration (measured in this run)"})
+    else:
+        res.update({"bound": "hbm", "achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": hbm / HBM_PEAK_GBS})
     return res
 
 
-def roofline_wz(dom, kern, args, nframes, outs):
+def roofline_wz(dom, kern, args, nframes, outs, valu=None):
     """Roofline of the dominant kernel (the BP launch of each bitplane step)."""
     name, d = dom
     if name != "bp_6336":
@@ -397,7 +507,7 @@ def roofline_wz(dom, kern, args, nframes, outs):
                 if lvl < 64:
                     byts += 4 * 6336 + (96 * (lvl + 2) + 7) // 8 + 792 + 8 * 6336
             eu += st["iterations"][b] * 3 * 6336
-    return bp_roofline(d["total_ms"] / launches, byts / launches, eu / launches, "wz_cif_q8")
+    return bp_roofline(d["total_ms"] / launches, byts / launches, eu / launches, "wz_cif_q8", valu)
 
 
 def cpu_baseline_wz(jobs, args):
@@ -553,14 +663,44 @@ def bench_copy(args):
             "bytes_read_per_step": nbytes, "bytes_written_per_step": nbytes, "copy_verified": bool(ok)}
 
 
+def bench_valu(args):
+    """VALU issue-rate calibration: one launch per step of each chain kind
+    (fp32 fma/add/mul, fp64 fma, int32 add), 16 waves per CU; value = the
+    highest rate in wave64 instructions/s.  Its rocprofv3 kernel trace and
+    SQ_INSTS_VALU pass are the committed calibration of the BP roofline."""
+    from dvc_amd import _lib
+    ctx = _lib.Context(device=0)
+    for _ in range(args.warmup):
+        valu_calibration(ctx)
+    runs = [valu_calibration(ctx) for _ in range(args.steps)]
+    kinds = [k for k in runs[0] if isinstance(runs[0][k], dict) and "kernel_ms" in runs[0][k]]
+    best = {k: max(r[k]["wave_instr_per_s"] for r in runs) for k in kinds}
+    peak = max(best.values())
+    return {"metric": "VALU issue rate (calibration)", "value": peak, "unit": "VALU wave64-instr/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": float(np.mean([sum(r[k]["kernel_ms"] for k in kinds) for r in runs])),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32/f64/i32",
+            "data": "synthetic",
+            "config": {"workload": "valu_peak: 8 independent chains per lane, 128 VALU instructions per trip, "
+                                   "4000 trips, 16 waves per CU, library compile flags (packed FP32 off)"},
+            "per_kind_wave_instr_per_s": best, "per_cu_cycle_at_2.4GHz": peak / (256 * 2.4e9),
+            "wave_instr_per_launch": runs[0]["fp32_fma"]["wave_instr"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch", "hbm_copy", "serve", "si_batch"])
+    ap.add_argument("--workload", default="wz_cif_q8", choices=["wz_cif_q8", "ldpca_batch", "hbm_copy", "serve", "si_batch", "valu_peak"])
     ap.add_argument("--streams", type=int, default=64, help="CIF sequences of 16 WZ frames decoded per step")
-    ap.add_argument("--plain-frames", type=int, default=4)
+    ap.add_argument("--plain-frames", type=int, default=24,
+                    help="wz_cif_q8: frames of the plain leg (one frame in flight, sequential rate search)")
+    ap.add_argument("--kernel-steps", type=int, default=1,
+                    help="wz_cif_q8: untimed steps with per-kernel HIP events after the timed ones")
+    ap.add_argument("--serve-rate", type=float, default=200.0,
+                    help="wz_cif_q8: arrival rate of the bounded serving leg (0 skips it)")
+    ap.add_argument("--serve-leg-frames", type=int, default=9, help="wz_cif_q8: frames per stream of the serving leg")
     ap.add_argument("--max-iter", type=int, default=100)
     ap.add_argument("--bp-threads", type=int, default=0, help="threads per CIF BP workgroup (0: library default)")
     ap.add_argument("--lanes", type=int, default=2, help="engine lanes (0: library default)")
@@ -582,7 +722,7 @@ def main():
               "(streams would shard across GPUs without any exchange).", file=sys.stderr)
         sys.exit(2)
     res = {"wz_cif_q8": bench_wz, "ldpca_batch": bench_ldpca, "hbm_copy": bench_copy,
-           "serve": bench_serve, "si_batch": bench_si}[args.workload](args)
+           "serve": bench_serve, "si_batch": bench_si, "valu_peak": bench_valu}[args.workload](args)
     print(json.dumps(res), flush=True)
 
 

@@ -33,7 +33,7 @@ class DvcError(RuntimeError):
 class Params(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("max_iter", ctypes.c_int),
                 ("data_dir", ctypes.c_char_p), ("bp_threads", ctypes.c_int), ("parallel_mode", ctypes.c_int),
-                ("rate_step", ctypes.c_int)]
+                ("rate_step", ctypes.c_int), ("qcif_ladder", ctypes.c_int)]
 
 
 class BpStats(ctypes.Structure):
@@ -55,14 +55,13 @@ class FrameStats(ctypes.Structure):
     _fields_ = [("nbitplanes", ctypes.c_int), ("bitrate", ctypes.c_int), ("levels", ctypes.c_int * 128),
                 ("start_levels", ctypes.c_int * 128), ("iterations", ctypes.c_int * 128),
                 ("used_inverse", ctypes.c_int * 128), ("entropy", ctypes.c_float * 128),
-                ("psnr", ctypes.c_double), ("recon_host_fixups", ctypes.c_int)]
+                ("psnr", ctypes.c_double)]
 
     def as_dict(self):
         n = self.nbitplanes
         return dict(nbitplanes=n, bitrate=self.bitrate, levels=list(self.levels[:n]),
                     start_levels=list(self.start_levels[:n]), iterations=list(self.iterations[:n]),
-                    used_inverse=list(self.used_inverse[:n]), entropy=list(self.entropy[:n]), psnr=self.psnr,
-                    recon_host_fixups=self.recon_host_fixups)
+                    used_inverse=list(self.used_inverse[:n]), entropy=list(self.entropy[:n]), psnr=self.psnr)
 
 
 class WZFrameSpec(ctypes.Structure):
@@ -113,11 +112,22 @@ SIGNATURES = {
     "dvc_get_tuning": (_c.c_int, [_c.c_void_p, _c.POINTER(Tuning)]),
     "dvc_set_tuning": (_c.c_int, [_c.c_void_p, _c.POINTER(Tuning)]),
     "dvc_selftest_phi_sweep": (_c.c_int, [_c.c_void_p, _c.c_uint32, _c.c_uint32, _c.POINTER(_c.c_ulonglong)]),
+    "dvc_selftest_valu_rate": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_int, _c.POINTER(_c.c_double),
+                                          _c.POINTER(_c.c_double)]),
     "dvc_decode_wz_frames": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WZFrameSpec), _c.c_int]),
     "dvc_side_info_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(SIRequest)]),
     "dvc_buffer_alloc": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_void_p)]),
     "dvc_buffer_free": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
     "dvc_buffer_copy": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_size_t, _c.c_int]),
+    "dvc_band_frame_init": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_int, _c.c_void_p, _c.c_void_p,
+                                       _c.c_int, _c.c_int, _c.c_int, _c.c_void_p, _c.c_int]),
+    "dvc_noise_distribution": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_void_p,
+                                          _c.c_void_p]),
+    "dvc_cond_bit_prob": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int, _c.c_int, _c.c_void_p, _c.c_int]),
+    "dvc_update_si": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p]),
+    "dvc_update_smf": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_void_p]),
+    "dvc_forward_me": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_int, _c.c_int,
+                                  _c.c_int, _c.c_void_p]),
     "dvc_engine_create": (_c.c_int, [_c.c_void_p, _c.POINTER(EngineParams), _c.POINTER(_c.c_void_p)]),
     "dvc_engine_destroy": (_c.c_int, [_c.c_void_p]),
     "dvc_engine_submit": (_c.c_int, [_c.c_void_p, _c.POINTER(WZFrameSpec), _c.c_uint64, _c.c_uint64,
@@ -193,9 +203,12 @@ class DeviceBuffer:
 class Context:
     """Owns a dvc_ctx (device, both ladders and their inverses resident)."""
 
-    def __init__(self, device=0, max_iter=50, bp_threads=0, data_dir=None, parallel_mode=0, rate_step=0):
+    def __init__(self, device=0, max_iter=50, bp_threads=0, data_dir=None, parallel_mode=0, rate_step=0,
+                 qcif_ladder="reg"):
+        """qcif_ladder: "reg" (RegDecoder1584.lad) or "irreg" (irReg1584.lad)."""
+        assert qcif_ladder in ("reg", "irreg")
         p = Params(device, max_iter, str(data_dir).encode() if data_dir else None, bp_threads, parallel_mode,
-                   rate_step)
+                   rate_step, 1 if qcif_ladder == "irreg" else 0)
         h = ctypes.c_void_p()
         _check(lib.dvc_ctx_create(ctypes.byref(p), ctypes.byref(h)), "dvc_ctx_create")
         self.h = h
@@ -351,6 +364,73 @@ class Context:
             items[i] = LdpcaItem(lr.size, lr.ctypes.data, acc.ctypes.data, int(crcs[i]), sl, out.ctypes.data, BpStats())
         _check(lib.dvc_ldpca_decode_batch(self.h, k, items), "dvc_ldpca_decode_batch")
         return outs, [items[i].stats.as_dict() for i in range(k)]
+
+
+# Reference data layouts of the per-band entry points (DvcDec/inc/global.h):
+# SMF, SearchInfo and MV records.
+SMF_DTYPE = np.dtype([("prob", "<f4", 81), ("searchTop", "<i4"), ("searchLeft", "<i4"), ("searchRight", "<i4"),
+                      ("searchBottom", "<i4"), ("searchWidth", "<i4"), ("blkIdx", "<i4"), ("probLen", "<i4")])
+SEARCH_INFO_DTYPE = np.dtype([(n, "<i4") for n in ("searchTop", "searchBottom", "searchLeft", "searchRight", "top",
+                                                     "left", "searchWidth", "searchRange", "blkIdx")])
+MV_DTYPE = np.dtype([("past", "<f8", 2), ("future", "<f8", 2), ("vector", "<f8", 2)])
+
+
+class BandSession:
+    """The per-band entry points (dvc_band_frame_init, dvc_noise_distribution,
+    dvc_cond_bit_prob, dvc_update_si, dvc_update_smf) on one WZ frame, with
+    numpy arrays in the reference's layouts."""
+
+    def __init__(self, ctx, past, fut, qindex, idx=(0, 1, 2), sir=None):
+        self.ctx = ctx
+        past = np.ascontiguousarray(past, np.uint8)
+        fut = np.ascontiguousarray(fut, np.uint8)
+        self.h, self.w = past.shape
+        sp = None if sir is None else np.ascontiguousarray(sir, SEARCH_INFO_DTYPE)
+        _check(lib.dvc_band_frame_init(ctx.h, self.w, self.h, qindex, past.ctypes.data, fut.ctypes.data, idx[0],
+                                       idx[1], idx[2], None if sp is None else sp.ctypes.data,
+                                       0 if sp is None else sp.size), "dvc_band_frame_init")
+
+    def noise_distribution(self, rng, index, alpha, smf, oct_, want=True):
+        alpha = np.ascontiguousarray(alpha, np.float32)
+        smf = np.ascontiguousarray(smf, SMF_DTYPE)
+        oct_ = np.ascontiguousarray(oct_, np.int32)
+        n = smf.size
+        per = 2 * rng + 1 if index else rng
+        out = np.zeros((n, per), np.float32) if want else None
+        _check(lib.dvc_noise_distribution(self.ctx.h, rng, index, alpha.ctypes.data, smf.ctypes.data, oct_.ctypes.data,
+                                          None if out is None else out.ctypes.data), "dvc_noise_distribution")
+        return out
+
+    def cond_bit_prob(self, index, bit, step, recons_trans, rng):
+        rt = np.ascontiguousarray(recons_trans, np.int32)
+        lr = np.zeros(self.w * self.h // 16, np.float32)
+        _check(lib.dvc_cond_bit_prob(self.ctx.h, lr.ctypes.data, index, bit, step, rt.ctypes.data, rng),
+               "dvc_cond_bit_prob")
+        return lr
+
+    def update_si(self, si, recons):
+        si = np.array(si, np.uint8, copy=True, order="C")
+        recons = np.ascontiguousarray(recons, np.uint8)
+        _check(lib.dvc_update_si(self.ctx.h, si.ctypes.data, recons.ctypes.data), "dvc_update_si")
+        return si
+
+    def update_smf(self, smf, recons, si):
+        smf = np.array(smf, SMF_DTYPE, copy=True)
+        recons = np.ascontiguousarray(recons, np.uint8)
+        si = np.ascontiguousarray(si, np.uint8)
+        _check(lib.dvc_update_smf(self.ctx.h, smf.ctypes.data, recons.ctypes.data, si.ctypes.data), "dvc_update_smf")
+        return smf
+
+
+def forward_me(ctx, fut_lp, past_lp, idx=(0, 1, 2)):
+    """dvc_forward_me: macroblock MVs (MV_DTYPE records) of the WZ frame."""
+    fut_lp = np.ascontiguousarray(fut_lp, np.uint8)
+    past_lp = np.ascontiguousarray(past_lp, np.uint8)
+    h, w = fut_lp.shape
+    out = np.zeros((h // 16) * (w // 16), MV_DTYPE)
+    _check(lib.dvc_forward_me(ctx.h, w, h, fut_lp.ctypes.data, past_lp.ctypes.data, idx[0], idx[1], idx[2],
+                              out.ctypes.data), "dvc_forward_me")
+    return out
 
 
 def _frame_spec(fr, stats):

@@ -99,6 +99,7 @@ int dvc_ctx_create(const dvc_params *params, dvc_ctx **out) {
   DVC_CHECK(p.parallel_mode == 0 || p.parallel_mode == 1, DVC_EINVAL, "dvc_ctx_create: parallel_mode must be 0 or 1");
   DVC_CHECK(p.rate_step == 0 || (p.rate_step == 1 && p.parallel_mode == 0), DVC_EINVAL,
             "dvc_ctx_create: rate_step must be 0, or 1 with parallel_mode 0");
+  DVC_CHECK(p.qcif_ladder == 0 || p.qcif_ladder == 1, DVC_EINVAL, "dvc_ctx_create: qcif_ladder must be 0 or 1");
   int ndev = 0;
   DVC_HIP(hipGetDeviceCount(&ndev));
   DVC_CHECK(ndev > 0 && p.device >= 0 && p.device < ndev, DVC_EDEVICE, "dvc_ctx_create: no such HIP device");
@@ -116,7 +117,8 @@ int dvc_ctx_create(const dvc_params *params, dvc_ctx **out) {
   c->sm_count = prop.multiProcessorCount;
   std::string dir = p.data_dir ? std::string(p.data_dir) : default_data_dir();
   int rc;
-  if ((rc = load_ladder(dir, 1584, c->lad1584)) || (rc = load_ladder(dir, 6336, c->lad6336)) ||
+  if ((rc = load_ladder(dir, p.qcif_ladder ? "irreg" : "reg", 1584, c->lad1584)) ||
+      (rc = load_ladder(dir, "reg", 6336, c->lad6336)) ||
       (rc = upload_ladder(c->lad1584)) || (rc = upload_ladder(c->lad6336))) {
     dvc_ctx_destroy(c);
     return rc;
@@ -137,6 +139,7 @@ int dvc_ctx_destroy(dvc_ctx *c) {
   if (c->si_scratch) (void)hipFree(c->si_scratch);
   if (c->ld_dev) (void)hipFree(c->ld_dev);
   if (c->ld_host) (void)hipHostFree(c->ld_host);
+  band_free(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return DVC_OK;
@@ -144,9 +147,10 @@ int dvc_ctx_destroy(dvc_ctx *c) {
 
 int dvc_prepare_data(const char *data_dir) {
   std::string dir = data_dir ? std::string(data_dir) : default_data_dir();
-  for (int N : {1584, 6336}) {
+  const std::pair<const char *, int> packs[] = {{"reg", 1584}, {"reg", 6336}, {"irreg", 1584}};
+  for (const auto &pk : packs) {
     Ladder L;
-    int rc = load_ladder(dir, N, L);
+    int rc = load_ladder(dir, pk.first, pk.second, L);
     if (rc) return rc;
   }
   return DVC_OK;
@@ -278,7 +282,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
     hPlanes[i] = PlaneDesc{N, dLR + (size_t)i * N, dLLR + (size_t)i * N, dAcc + (size_t)i * nb,
                            (uint32_t)items[i].crc, dState + i, dRec + (size_t)i * C,
                            mode1 ? dCarry + (size_t)i * N : nullptr, mode1 ? dBcase + (size_t)i * C : nullptr};
-    hState[i] = PlaneState{st0 >= 0 ? st0 : -1, C - 1, 0.0f, 0};
+    hState[i] = PlaneState{st0 >= 0 ? st0 : -1, C - 1, 0.0f, 0, 0};
     for (int l = 0; l < C; l++) hRec[(size_t)i * C + l] = make_int4(0, REC_NOT_RUN, -1, 0);
   }
   hCounters[0] = hCounters[1] = hCounters[2] = 0;
@@ -301,7 +305,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
       int4 r;
       for (; level < C - 1;
            level = c->rate_step ? std::min(C - 1, state[i].start + dynstep_next(level - state[i].start)) : level + 1) {
-        PlaneState ps{state[i].start, C - 1, state[i].entropy, level};
+        PlaneState ps{state[i].start, C - 1, state[i].entropy, level, state[i].taint};
         int zero[3] = {0, 0, 0};
         DVC_HIP(hipMemcpyAsync(dState + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
         DVC_HIP(hipMemcpyAsync(dCounters, zero, sizeof(zero), hipMemcpyHostToDevice, s));
@@ -320,7 +324,7 @@ static int decode_group(dvc_ctx *c, int N, int n, dvc_ldpca_item *items, int seq
         }
       }
       if (level == C - 1) {
-        PlaneState ps{state[i].start, C - 1, state[i].entropy, C};
+        PlaneState ps{state[i].start, C - 1, state[i].entropy, C, state[i].taint};
         int zero[3] = {0, 0, 0};
         DVC_HIP(hipMemcpyAsync(dState + i, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
         DVC_HIP(hipMemcpyAsync(dCounters, zero, sizeof(zero), hipMemcpyHostToDevice, s));

@@ -25,10 +25,15 @@ void set_error(const std::string &msg);
     if (!(cond)) { dvc::set_error(msg); return (code); }                               \
   } while (0)
 
+// The BP kernels' threads per workgroup on an irregular ladder (its
+// column-to-thread table is built for this count).
+constexpr int kIrregThreads = 256;
+
 // Host copy of one ladder (all codes of one code length) and its device image.
 struct Ladder {
   int N = 0, E = 0, period = 0, numCodes = 0;
-  std::vector<int> jc;             // N+1 (every column has degree 3)
+  bool regular = true;             // every column has degree 3 (the Reg*.lad ladders)
+  std::vector<int> jc;             // N+1 column pointers
   std::vector<int> encIr;          // E encoder rows
   std::vector<int> bitsPerPeriod;  // numCodes
   std::vector<int> idx;            // numCodes*period
@@ -48,18 +53,37 @@ struct Ladder {
   //                            r's messages sit at rowPtr[r] + rank
   //   d_tx[c*N + k]            accumulated-syndrome position of syndrome bit k
   uint2 *d_colRows = nullptr;
+  // Irregular ladders (column degrees differ) instead of d_colRows:
+  //   d_colEdges[c*E + l]   edge l (column order) as row | rank << 16
+  //   d_jc[k]               column pointers (N+1)
+  //   d_colSlots[t*S + i]   the i-th column thread t of a kIrregThreads
+  //                         workgroup updates (-1: none), S = colSlotsPerThread,
+  //                         columns dealt to threads so each thread's edge
+  //                         count is nearly equal
+  uint32_t *d_colEdges = nullptr;
+  int *d_jc = nullptr;
+  int16_t *d_colSlots = nullptr;
+  int colSlotsPerThread = 0;
   uint16_t *d_rowPtr = nullptr, *d_tx = nullptr;
   int *d_syndLength = nullptr;
   float *d_codeRate = nullptr;
   uint8_t *d_invH = nullptr, *d_crcBasis = nullptr;
 };
 
-int load_ladder(const std::string &dir, int N, Ladder &L);
+// Loads dir/<kind><N>.ladder (kind "reg" or "irreg") and its cached inverse.
+int load_ladder(const std::string &dir, const char *kind, int N, Ladder &L);
 // BP launch shape of the context's tuning: threads per workgroup and the
 // persistent grid for `items` queued (bitplane, code) items.
 int bp_threads(const dvc_ctx *c, int N);
 int bp_grid(const dvc_ctx *c, int N, int items);
 int upload_ladder(Ladder &L);
+// Quantisation table entry Qtable[q][index] (reconstruction.cpp:9-78), 0 outside 1..8 x 0..15.
+int qtable_entry(int q, int index);
+// Motion-cost factor tables of the context (made on first use).
+int ensure_cost_tables(dvc_ctx *c);
+// Per-band entry points' device state (band.hip), freed with the context.
+struct BandState;
+void band_free(dvc_ctx *c);
 void free_ladder(Ladder &L);
 
 }  // namespace dvc
@@ -104,5 +128,6 @@ struct dvc_ctx {
   size_t ld_dev_bytes = 0;
   void *ld_host = nullptr;     // its pinned host staging (inputs up, results back)
   size_t ld_host_bytes = 0;
+  dvc::BandState *band = nullptr;   // per-band entry points' frame state (dvc_band_frame_init)
   dvc::Ladder *ladder(int N) { return N == 1584 ? &lad1584 : (N == 6336 ? &lad6336 : nullptr); }
 };

@@ -17,8 +17,9 @@ struct JobInitDev {
   uint8_t *carry;             // N bytes set to 0xAA (ParallelMode 1's carried decisions)
   int N;
   PlaneState *state;          // nbp rate-search states: start from the entropy, nothing accepted
-  int4 *rec;                  // 65 * nbp run records: not run
+  int4 *rec;                  // kMaxCodes * nbp run records: not run
   int nbp;
+  int numCodes;               // codes of the frame's ladder ("nothing accepted" = numCodes - 1)
   const uint8_t *accSrc;      // accumulated syndromes to copy (device), or null
   uint8_t *accDst;
   int accBytes;

@@ -10,8 +10,8 @@ __global__ __launch_bounds__(256) void k_job_init(const JobInitDev *items) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   for (int i = t; i < it.fs; i += stride) it.reconsTrans[i] = 0;
   for (int i = t; i < it.N; i += stride) it.carry[i] = 0xAA;
-  for (int i = t; i < it.nbp; i += stride) it.state[i] = PlaneState{-1, 64, 0.0f, 0};
-  for (int i = t; i < 65 * it.nbp; i += stride) it.rec[i] = make_int4(0, REC_NOT_RUN, -1, 0);
+  for (int i = t; i < it.nbp; i += stride) it.state[i] = PlaneState{-1, it.numCodes - 1, 0.0f, 0, 0};
+  for (int i = t; i < kMaxCodes * it.nbp; i += stride) it.rec[i] = make_int4(0, REC_NOT_RUN, -1, 0);
   if (it.accSrc)
     for (int i = t; i < it.accBytes; i += stride) it.accDst[i] = it.accSrc[i];
 }

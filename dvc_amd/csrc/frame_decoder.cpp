@@ -53,6 +53,8 @@ static const int kQBits[9][16] = {
     {7, 6, 5, 4, 6, 5, 4, 3, 5, 4, 3, 2, 4, 3, 2, 0}};
 static const int kZigzag[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
 
+int qtable_entry(int q, int index) { return (q >= 1 && q <= 8 && index >= 0 && index < 16) ? kQTable[q][index] : 0; }
+
 #define ROUNDD(x) ((x) >= 0.0 ? (int)((x) + 0.5) : -(int)(-(x) + 0.5))
 
 struct BandPlan {
@@ -111,10 +113,9 @@ struct Job {
   int4 *d_rec = nullptr;
   float *d_LLR = nullptr;
   uint8_t *d_carry = nullptr;  // ParallelMode 1: decisions carried from code to code and bitplane to bitplane
-  int *d_bcase = nullptr;      // ParallelMode 1: [nbp][65] repeat-case decision slots
+  int *d_bcase = nullptr;      // ParallelMode 1: [nbp][kMaxCodes] repeat-case decision slots
   // cursor
   int bandPos = 0, bit = 0, bp = 0;
-  int fixups = 0;                       // reconstruction entries recomputed on the host
   bool done() const { return bandPos >= (int)bands.size(); }
 };
 
@@ -130,9 +131,9 @@ static size_t job_bytes(const Job &j) {
                    (size_t)64 * (W - 3) * (H - 3), (size_t)4 * 81 * N, (size_t)4 * N * std::max(maxStride, 1),
                    4 * (size_t)N, (size_t)N, (size_t)FS, sizeof(MVd) * (size_t)(FS / 256), sizeof(MVd) * (size_t)(FS / 64),
                    sizeof(MVd) * (size_t)(FS / 64), sizeof(MVd) * (size_t)(FS / 256), 16 * (size_t)(FS / 256),
-                   8 * (size_t)(FS / 64), (size_t)4, 4 * (size_t)N, (size_t)j.nbp * (N / 8),
-                   sizeof(PlaneState) * (size_t)j.nbp, sizeof(int4) * 65 * (size_t)j.nbp, 4 * (size_t)N, (size_t)N,
-                   4 * 65 * (size_t)j.nbp})
+                   8 * (size_t)(FS / 64), (size_t)j.nbp * (N / 8),
+                   sizeof(PlaneState) * (size_t)j.nbp, sizeof(int4) * kMaxCodes * (size_t)j.nbp, 4 * (size_t)N,
+                   (size_t)N, 4 * kMaxCodes * (size_t)j.nbp})
     t = align_up(t + b);
   return t;
 }
@@ -149,10 +150,10 @@ static void carve_job(Job &j, uint8_t *base) {
                o_dist = take((size_t)4 * N * std::max(maxStride, 1)), o_lr = take(4 * (size_t)N), o_dec = take(N),
                o_rec = take(FS), o_mb = take(sizeof(MVd) * (size_t)(FS / 256)), o_bmv = take(sizeof(MVd) * (size_t)(FS / 64)),
                o_bmvt = take(sizeof(MVd) * (size_t)(FS / 64)), o_fwd = take(sizeof(MVd) * (size_t)(FS / 256)),
-               o_holes = take(16 * (size_t)(FS / 256)), o_mse = take(8 * (size_t)(FS / 64)), o_tc = take(4),
-               o_tl = take(4 * (size_t)N), o_acc = take((size_t)j.nbp * (N / 8)), o_state = take(sizeof(PlaneState) * (size_t)j.nbp),
-               o_recd = take(sizeof(int4) * 65 * (size_t)j.nbp), o_llr = take(4 * (size_t)N), o_carry = take(N),
-               o_bcase = take(4 * 65 * (size_t)j.nbp);
+               o_holes = take(16 * (size_t)(FS / 256)), o_mse = take(8 * (size_t)(FS / 64)),
+               o_acc = take((size_t)j.nbp * (N / 8)), o_state = take(sizeof(PlaneState) * (size_t)j.nbp),
+               o_recd = take(sizeof(int4) * kMaxCodes * (size_t)j.nbp), o_llr = take(4 * (size_t)N),
+               o_carry = take(N), o_bcase = take(4 * kMaxCodes * (size_t)j.nbp);
   j.mem = base;
   j.bytes = off;
   FrameDev &f = j.fd;
@@ -161,32 +162,11 @@ static void carve_job(Job &j, uint8_t *base) {
   f.smf = (float *)(base + o_smf); f.dist = (float *)(base + o_dist); f.LR = (float *)(base + o_lr);
   f.decoded = base + o_dec; f.recons = base + o_rec; f.mbmv = (MVd *)(base + o_mb); f.bmv = (MVd *)(base + o_bmv);
   f.bmv_tmp = (MVd *)(base + o_bmvt); f.fwd = (MVd *)(base + o_fwd); f.holes = (double *)(base + o_holes);
-  f.mse = (double *)(base + o_mse); f.tieCount = (int *)(base + o_tc); f.tieList = (int *)(base + o_tl);
+  f.mse = (double *)(base + o_mse);
   j.d_acc = base + o_acc; j.d_state = (PlaneState *)(base + o_state); j.d_rec = (int4 *)(base + o_recd);
   j.d_LLR = (float *)(base + o_llr);
   j.d_carry = base + o_carry;
   j.d_bcase = (int *)(base + o_bcase);
-}
-
-// reconstruction (reconstruction.cpp:184-289) of one coefficient with the C
-// library's exp, for entries the kernel left near a rounding boundary.
-static int recon_host(int rt, int side, float alphaF, int index, int step, int ls) {
-  int lo, hi;
-  const double a = alphaF;
-  if (index == 0) {
-    lo = rt * step; hi = (rt + 1) * step;
-  } else {
-    const int q = rt - ls;
-    if (q == 0) { lo = -step + 1; hi = step; }
-    else if (q > 0) { lo = q * step; hi = (q + 1) * step; }
-    else { lo = (q - 1) * step + 1; hi = q * step + 1; }
-  }
-  if (a <= 0.0001) return (lo + hi) / 2;
-  if (side < lo) return lo + ROUNDD(1.0 / a + (double)step / (1.0 - exp(a * (double)step)));
-  if (side >= hi) return hi - ROUNDD(1.0 / a + (double)step / (1.0 - exp(a * (double)step)));
-  const double r = side - lo, d = hi - side;
-  const double er = exp(-a * r), ed = exp(-a * d);
-  return side + ROUNDD(((r + 1.0 / a) * er - (d + 1.0 / a) * ed) / (2.0 - (er + ed)));
 }
 
 }  // namespace dvc
@@ -287,7 +267,7 @@ int dvc::bp_step(dvc_ctx *c, int N, const std::vector<PlaneDesc> &planes, bool s
     int4 r;
     for (; level < C - 1;
          level = c->rate_step ? std::min(C - 1, st[i].start + dynstep_next(level - st[i].start)) : level + 1) {
-      ps = PlaneState{st[i].start, C - 1, st[i].entropy, level};
+      ps = PlaneState{st[i].start, C - 1, st[i].entropy, level, st[i].taint};
       DVC_HIP(hipMemcpyAsync(planes[i].state, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
       DVC_HIP(hipMemsetAsync(d_counters, 0, sizeof(int), s));  // queue head only: the step's slots accumulate
       BpArgs a{ld, d_planes + i, 1, d_counters, 1, 1, c->max_iter, d_slots, d_counters + 1, slotCap, c->bp_poll,
@@ -302,7 +282,7 @@ int dvc::bp_step(dvc_ctx *c, int N, const std::vector<PlaneDesc> &planes, bool s
         DVC_HIP(hipMemcpyAsync(planes[i].carry, d_slots + (size_t)r.z * N, N, hipMemcpyDeviceToDevice, s));
       }
     }
-    ps = PlaneState{st[i].start, ok ? level : C - 1, st[i].entropy, C};
+    ps = PlaneState{st[i].start, ok ? level : C - 1, st[i].entropy, C, st[i].taint};
     DVC_HIP(hipMemcpyAsync(planes[i].state, &ps, sizeof(ps), hipMemcpyHostToDevice, s));
     if (!ok) TIMED_LAUNCH(c, "invh", s, launch_invh(d_planes + i, 1, ld, d_slots, d_counters + 1, slotCap, s));
     DVC_HIP(hipStreamSynchronize(s));
@@ -349,11 +329,10 @@ struct Lane {
   size_t outBytes = 0;
   uint8_t *d_slots = nullptr;
   int *d_counters = nullptr;
-  int *d_ties = nullptr;
   int rc = 0;
   std::string err;
   // host-side lists of one step
-  std::vector<int> l0, l1, l2, l3, l4, ties;
+  std::vector<int> l0, l1, l2, l3, l4;
   std::vector<PlaneDesc> groups[2];
   // Captured launch sequences of steady steps (no frame starts or ends a
   // band), keyed by their launch shape: frames stepped, planes of each code
@@ -519,7 +498,8 @@ static int admit(dvc_engine *e, Job *j, hipStream_t s, std::vector<JobInitDev> &
   // written to it, DVC_Decoder.cpp:124, and in its golden runs that buffer
   // holds the MALLOC_PERTURB_ fill 0xAA, which never matches a decision)
   inits.push_back(JobInitDev{f.reconsTrans, j->W * j->H, j->d_carry, j->N, j->d_state, j->d_rec, j->nbp,
-                             e->deviceBuffers ? j->spec.acc : nullptr, j->d_acc, j->nbp * (j->N / 8)});
+                             e->c->ladder(j->N)->numCodes, e->deviceBuffers ? j->spec.acc : nullptr, j->d_acc,
+                             j->nbp * (j->N / 8)});
   j->bandPos = 0;
   j->bit = j->bands.empty() ? 0 : j->bands[0].qbits;
   j->bp = 0;
@@ -532,21 +512,33 @@ static bool job_needs_host_recons(const dvc_engine *e, const Job &j) {
   return !e->deviceBuffers || (j.spec.orig && j.spec.stats);
 }
 static size_t job_out_bytes(const dvc_engine *e, const Job &j) {
-  return align_up(sizeof(PlaneState) * j.nbp) + align_up(sizeof(int4) * 65 * (size_t)j.nbp) +
+  return align_up(sizeof(PlaneState) * j.nbp) + align_up(sizeof(int4) * kMaxCodes * (size_t)j.nbp) +
          (job_needs_host_recons(e, j) ? align_up((size_t)j.W * j.H) : 0);
 }
 
 // A finished frame's outputs from its gathered results `out` (host copy, in
 // the layout of job_out_bytes; a device request's reconstruction was copied
 // to the caller's buffer on the device).
-static int finish_job(dvc_engine *e, Job &j, const uint8_t *out) {
+// Every bitplane's accepted record must name a decision slot below the
+// lane's capacity (slot_cap): k_put_bitplane skips a bitplane whose slot lies
+// beyond it, so such a frame is reported as DVC_ENOMEM instead of finishing.
+static int finish_job(dvc_engine *e, Job &j, const uint8_t *out, int slot_cap) {
   dvc_ctx *c = e->c;
   const size_t FS = (size_t)j.W * j.H;
   const PlaneState *ps = (const PlaneState *)out;
   const int4 *rec = (const int4 *)(out + align_up(sizeof(PlaneState) * j.nbp));
   const uint8_t *rec8 = job_needs_host_recons(e, j)
-                            ? out + align_up(sizeof(PlaneState) * j.nbp) + align_up(sizeof(int4) * 65 * (size_t)j.nbp)
+                            ? out + align_up(sizeof(PlaneState) * j.nbp) + align_up(sizeof(int4) * kMaxCodes * (size_t)j.nbp)
                             : nullptr;
+  const int C = c->ladder(j.N)->numCodes;
+  for (int b = 0; b < j.nbp; b++) {
+    const int best = ps[b].best;
+    DVC_CHECK(best >= 0 && best < C, DVC_ESTATE, "WZ frame decode: bitplane without accepted code");
+    const int4 r = rec[(size_t)kMaxCodes * b + best];
+    DVC_CHECK(r.y == REC_ACCEPTED, DVC_ESTATE, "WZ frame decode: bitplane without accepted code");
+    DVC_CHECK(r.z >= 0 && r.z < slot_cap, DVC_ENOMEM,
+              "WZ frame decode: decision slots exhausted (an accepted bitplane's decisions were not stored)");
+  }
   if (!e->deviceBuffers) memcpy(j.spec.recons, rec8, FS);
   dvc_frame_stats *st = j.spec.stats;
   if (!st) return DVC_OK;
@@ -555,9 +547,9 @@ static int finish_job(dvc_engine *e, Job &j, const uint8_t *out) {
   st->bitrate = j.bitrate0;
   Ladder *L = c->ladder(j.N);
   for (int b = 0; b < j.nbp && b < 128; b++) {
-    const int4 *r = &rec[(size_t)65 * b];
+    const int4 *r = &rec[(size_t)kMaxCodes * b];
     int lvl = -1, it = 0;
-    for (int l = std::max(ps[b].start, 0); l < 65; l++) {
+    for (int l = std::max(ps[b].start, 0); l < C; l++) {
       if (r[l].y == REC_ACCEPTED) { lvl = l; break; }
       it += r[l].x;
     }
@@ -569,7 +561,6 @@ static int finish_job(dvc_engine *e, Job &j, const uint8_t *out) {
     st->entropy[b] = ps[b].entropy;
     st->bitrate += L->syndLength[lvl] + 8;
   }
-  st->recon_host_fixups = j.fixups;
   if (j.spec.orig) {
     double mse = 0.0;
     for (size_t p = 0; p < FS; p++) {
@@ -597,7 +588,6 @@ static int lane_step(dvc_engine *e, Lane &ln) {
     maxN = std::max(maxN, j->N);
     maxMB = std::max(maxMB, j->W * j->H / 256);
     maxPos = std::max(maxPos, (j->W - 3) * (j->H - 3));
-    j->fd.tieCount = ln.d_ties + i;
   }
   CostTables ct{c->d_costFme, c->d_costUsi};
   // ---- the step's frame lists (all known before any kernel runs)
@@ -641,10 +631,10 @@ static int lane_step(dvc_engine *e, Lane &ln) {
   for (size_t a = 0; a < active.size(); a++) {
     Job &j = *ln.inflight[active[a]];
     PlaneDesc pd{j.N, j.fd.LR, j.d_LLR, j.d_acc + (size_t)j.bp * (j.N / 8), (uint32_t)j.crc[j.bp],
-                 j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp, m1 ? j.d_carry : nullptr,
-                 m1 ? j.d_bcase + (size_t)65 * j.bp : nullptr};
+                 j.d_state + j.bp, j.d_rec + (size_t)kMaxCodes * j.bp, m1 ? j.d_carry : nullptr,
+                 m1 ? j.d_bcase + (size_t)kMaxCodes * j.bp : nullptr};
     ln.groups[j.N == 6336].push_back(pd);
-    ln.h_links[a] = BpFrameLink{j.d_state + j.bp, j.d_rec + (size_t)65 * j.bp,
+    ln.h_links[a] = BpFrameLink{j.d_state + j.bp, j.d_rec + (size_t)kMaxCodes * j.bp,
                                 ln.d_slots + (size_t)(j.N == 6336 ? ln.groupCap : 0) * 6336, ln.groupCap,
                                 m1 ? j.d_carry : nullptr};
   }
@@ -674,7 +664,7 @@ static int lane_step(dvc_engine *e, Lane &ln) {
   for (size_t k = 0; k < fin.size(); k++) {
     const Job &j = *ln.inflight[fin[k]];
     uint8_t *o = ln.d_out + outOff[k];
-    const size_t bs = sizeof(PlaneState) * j.nbp, br = sizeof(int4) * 65 * (size_t)j.nbp, FS = (size_t)j.W * j.H;
+    const size_t bs = sizeof(PlaneState) * j.nbp, br = sizeof(int4) * kMaxCodes * (size_t)j.nbp, FS = (size_t)j.W * j.H;
     ln.h_segs[nseg++] = CopySeg{(const uint8_t *)j.d_state, o, bs};
     ln.h_segs[nseg++] = CopySeg{(const uint8_t *)j.d_rec, o + align_up(bs), br};
     if (job_needs_host_recons(e, j)) ln.h_segs[nseg++] = CopySeg{j.fd.recons, o + align_up(bs) + align_up(br), FS};
@@ -749,32 +739,7 @@ static int lane_step(dvc_engine *e, Lane &ln) {
     }
     if (!ending.empty()) {
       const int ne = (int)ending.size();
-      DVC_HIP(hipMemsetAsync(ln.d_ties, 0, sizeof(int) * n, s));
       TIMED_LAUNCH(c, "reconstruct", s, launch_reconstruct(ln.d_frames, L2, ne, maxN, s));
-      ln.ties.assign(n, 0);
-      DVC_HIP(hipMemcpyAsync(ln.ties.data(), ln.d_ties, sizeof(int) * n, hipMemcpyDeviceToHost, s));
-      DVC_HIP(hipStreamSynchronize(s));
-      for (int i : ending) {
-        Job &j = *ln.inflight[i];
-        const int nt = ln.ties[i];
-        DVC_CHECK(nt <= j.N, DVC_ESTATE, "reconstruction tie list overflow");
-        if (nt == 0) continue;
-        std::vector<int> idx(nt);
-        DVC_HIP(hipMemcpyAsync(idx.data(), j.fd.tieList, sizeof(int) * nt, hipMemcpyDeviceToHost, s));
-        DVC_HIP(hipStreamSynchronize(s));
-        for (int ci : idx) {
-          int rt, side;
-          float al;
-          DVC_HIP(hipMemcpyAsync(&rt, j.fd.reconsTrans + ci, sizeof(int), hipMemcpyDeviceToHost, s));
-          DVC_HIP(hipMemcpyAsync(&side, j.fd.sideTrans + ci, sizeof(int), hipMemcpyDeviceToHost, s));
-          DVC_HIP(hipMemcpyAsync(&al, j.fd.alpha + ci, sizeof(float), hipMemcpyDeviceToHost, s));
-          DVC_HIP(hipStreamSynchronize(s));
-          const int v = recon_host(rt, side, al, j.fd.index, j.fd.step, j.fd.levelShift);
-          DVC_HIP(hipMemcpyAsync(j.fd.sideTrans + ci, &v, sizeof(int), hipMemcpyHostToDevice, s));
-          DVC_HIP(hipStreamSynchronize(s));
-          j.fixups++;
-        }
-      }
       TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(ln.d_frames, L2, ne, maxN, s));
       TIMED_LAUNCH(c, "update_si", s, launch_update_si(ln.d_frames, L2, ne, maxN, ct, s));
       TIMED_LAUNCH(c, "oct", s, launch_oct(ln.d_frames, L2, ne, maxPos, 0, s));
@@ -797,7 +762,7 @@ static int lane_step(dvc_engine *e, Lane &ln) {
     for (size_t k = 0; k < fin.size(); k++) {
       Job &j = *ln.inflight[fin[k]];
       DVC_CHECK(j.setup && j.done(), DVC_ESTATE, "engine: a finishing frame has bands left");
-      if ((rc = finish_job(e, j, ln.h_out + outOff[k]))) return rc;
+      if ((rc = finish_job(e, j, ln.h_out + outOff[k], ln.groupCap))) return rc;
     }
     std::vector<Job *> keep;
     for (int i = 0; i < n; i++) {
@@ -942,7 +907,7 @@ static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **ou
   for (int k = 0; k < nl && err == hipSuccess; k++) {
     Lane &l = e->lanes[k];
     l.cap = (e->maxInflight + nl - 1) / nl;
-    l.groupCap = (c->bp_mode == 1 ? 130 : 65) * l.cap + 1;  // ParallelMode 1 keeps two decision sets per code
+    l.groupCap = (c->bp_mode == 1 ? 2 * kMaxCodes : kMaxCodes) * l.cap + 1;  // ParallelMode 1 keeps two decision sets per code
     const size_t M = l.cap;
     const size_t oFrames = 0, oLists = align_up(oFrames + sizeof(FrameDev) * M), oLinks = align_up(oLists + sizeof(int) * 5 * M),
                  oPlanes = align_up(oLinks + sizeof(BpFrameLink) * M), oSegs = align_up(oPlanes + sizeof(PlaneDesc) * M),
@@ -961,7 +926,6 @@ static int engine_create(dvc_ctx *c, const dvc_engine_params *p, dvc_engine **ou
     }
     if (err == hipSuccess) err = hipMalloc((void **)&l.d_slots, (size_t)2 * l.groupCap * 6336);
     if (err == hipSuccess) err = hipMalloc((void **)&l.d_counters, 6 * sizeof(int));
-    if (err == hipSuccess) err = hipMalloc((void **)&l.d_ties, sizeof(int) * M);
   }
   if (err != hipSuccess) {
     set_error(std::string("dvc_engine_create: ") + hipGetErrorString(err));
@@ -983,7 +947,7 @@ static void engine_destroy(dvc_engine *e) {
     for (Job *j : l.inflight) { if (j->mem) (void)hipFree(j->mem); delete j; }
     for (Job *j : l.finished) { if (j->mem) (void)hipFree(j->mem); delete j; }
     for (auto &g : l.graphs) (void)hipGraphExecDestroy(g.second);
-    void *ptrs[] = {l.d_stage, l.d_out, l.d_slots, l.d_counters, l.d_ties};
+    void *ptrs[] = {l.d_stage, l.d_out, l.d_slots, l.d_counters};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     if (l.h_stage) (void)hipHostFree(l.h_stage);

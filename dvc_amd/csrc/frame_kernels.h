@@ -39,8 +39,6 @@ struct FrameDev {
   MVd *fwd;                  // forward-ME result per macroblock
   double *holes;             // 2 per macroblock
   double *mse;               // per 8x8 block
-  int *tieCount;             // reconstruction entries left to the host (near a rounding tie)
-  int *tieList;              // their coefficient indices (capacity N)
   // per-band parameters (set by the host before the band's kernels)
   int band;                  // zig-zag band number
   int index;                 // coefficient index in the 4x4 block (zigzag[band])
@@ -57,6 +55,8 @@ struct CostTables {
 
 hipError_t launch_keyframe_filters(const uint8_t *in, uint8_t *lp, uint8_t *half, int W, int H, hipStream_t s);
 hipError_t launch_create_si(const FrameDev *F, const int *list, int n, int maxMB, CostTables ct, hipStream_t s);
+// forwardME alone (sideInfoCreation.cpp:61-145): macroblock MVs (mbmv, unscaled) from past_lp/fut_lp.
+hipError_t launch_forward_me(const FrameDev *F, const int *list, int n, int maxMB, CostTables ct, hipStream_t s);
 hipError_t launch_noise_param(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s);
 hipError_t launch_oct(const FrameDev *F, const int *list, int n, int maxPos, int initSideTrans, hipStream_t s);
 hipError_t launch_noise_dist(const FrameDev *F, const int *list, int n, int maxN, int maxStride, hipStream_t s);

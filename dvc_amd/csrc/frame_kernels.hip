@@ -217,8 +217,10 @@ __device__ void bidir_sweep(const FrameDev &f, MVd *mv, int blockLen) {
 }
 
 // The rest of createSideInfo (sideInfoCreation.cpp:119-143, 510-550) and
-// motionFilterAndComp (:312-436) for one frame per workgroup.
-__global__ __launch_bounds__(1024) void k_si_chain(const FrameDev *F, const int *list) {
+// motionFilterAndComp (:312-436) for one frame per workgroup.  With
+// fwdOnly set it stops after forwardME's nearest-hole assignment (:119-143)
+// and leaves the macroblock MVs unscaled (forwardME's own output).
+__global__ __launch_bounds__(1024) void k_si_chain(const FrameDev *F, const int *list, int fwdOnly) {
   const FrameDev &f = F[list[blockIdx.x]];
   const int tid = threadIdx.x, T = blockDim.x, W = f.W, H = f.H;
   const int bw16 = W / 16, nmb = bw16 * (H / 16), bw8 = W / 8, bh8 = H / 8, nb8 = bw8 * bh8;
@@ -240,10 +242,11 @@ __global__ __launch_bounds__(1024) void k_si_chain(const FrameDev *F, const int 
     o.fy = (double)end_point(top, -b.vy, f.iFC, 0, H - 16);
     o.px = (double)end_point(left, b.vx, f.iPC, 0, W - 16);
     o.py = (double)end_point(top, b.vy, f.iPC, 0, H - 16);
-    o.fx *= 2; o.fy *= 2; o.px *= 2; o.py *= 2;
+    if (!fwdOnly) { o.fx *= 2; o.fy *= 2; o.px *= 2; o.py *= 2; }
     o.vx = o.px - o.fx; o.vy = o.py - o.fy;
     f.mbmv[i] = o;
   }
+  if (fwdOnly) return;
   __threadfence_block();
   __syncthreads();
   bidir_sweep(f, f.mbmv, 16);
@@ -708,7 +711,7 @@ __global__ void k_put_bitplane(const FrameDev *F, const int *list, const BpFrame
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= f.N) return;
   const int best = lk.state->best;
-  const int slot = (best >= 0 && best < 65) ? lk.records[best].z : -1;
+  const int slot = (best >= 0 && best < kMaxCodes) ? lk.records[best].z : -1;
   if (slot < 0 || slot >= lk.slot_capacity) return;  // the host reports the missing record
   const uint8_t bitv = lk.slots[(size_t)slot * f.N + b];
   f.decoded[b] = bitv;
@@ -719,11 +722,10 @@ __global__ void k_put_bitplane(const FrameDev *F, const int *list, const BpFrame
 }
 
 // ------------------------------------------------------------ reconstruction
-// reconstruction (reconstruction.cpp:184-289).  exp() here is the device's
-// double exponential; an entry whose pre-rounding value lies within a margin
-// of a rounding boundary (where a last-bit difference from the host's exp
-// could change the integer) is left to the host, which recomputes it with the
-// C library (dvc_frame_* host code).
+// reconstruction (reconstruction.cpp:184-289).  exp is dvc_exp, the host C
+// library's double exp restated bit for bit, and the remaining operations are
+// the same IEEE double operations in the same order without contraction, so
+// every reconstructed coefficient equals the CPU path's.
 __global__ void k_reconstruct(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -742,26 +744,19 @@ __global__ void k_reconstruct(const FrameDev *F, const int *list) {
     else if (q > 0) { lo = q * step; hi = (q + 1) * step; }
     else { lo = (q - 1) * step + 1; hi = q * step + 1; }
   }
-  if (a <= 0.0001) { f.sideTrans[ci] = (lo + hi) / 2; return; }
-  double v;
-  int base, sign;
-  if (side < lo) { v = 1.0 / a + (double)step / (1.0 - exp(a * (double)step)); base = lo; sign = 1; }
-  else if (side >= hi) { v = 1.0 / a + (double)step / (1.0 - exp(a * (double)step)); base = hi; sign = -1; }
-  else {
+  int v;
+  if (a <= 0.0001) {
+    v = (lo + hi) / 2;
+  } else if (side < lo) {
+    v = lo + round_d(1.0 / a + (double)step / (1.0 - dvc_exp(a * (double)step)));
+  } else if (side >= hi) {
+    v = hi - round_d(1.0 / a + (double)step / (1.0 - dvc_exp(a * (double)step)));
+  } else {
     const double r = side - lo, d = hi - side;
-    const double er = exp(-a * r), ed = exp(-a * d);
-    v = ((r + 1.0 / a) * er - (d + 1.0 / a) * ed) / (2.0 - (er + ed));
-    base = side; sign = 1;
+    const double er = dvc_exp(-a * r), ed = dvc_exp(-a * d);
+    v = side + round_d(((r + 1.0 / a) * er - (d + 1.0 / a) * ed) / (2.0 - (er + ed)));
   }
-  const double av = fabs(v) + 0.5;
-  const double frac = av - floor(av);
-  const double margin = 1e-7 * (1.0 + fabs(v));
-  if (!isfinite(v) || frac < margin || frac > 1.0 - margin) {
-    const int slot = atomicAdd(f.tieCount, 1);
-    if (slot < f.N) f.tieList[slot] = ci;
-    return;
-  }
-  f.sideTrans[ci] = base + sign * round_d(v);
+  f.sideTrans[ci] = v;
 }
 
 // inverseTransform (transform.cpp:234-273).
@@ -1021,7 +1016,12 @@ hipError_t launch_keyframe_batch(const KeyFrameJob *jobs, int n, int maxW, int m
 }
 hipError_t launch_create_si(const FrameDev *F, const int *list, int n, int maxMB, CostTables ct, hipStream_t s) {
   hipLaunchKernelGGL(k_forward_me, dim3(maxMB, n), dim3(256), 0, s, F, list, ct);
-  hipLaunchKernelGGL(k_si_chain, dim3(n), dim3(1024), 0, s, F, list);
+  hipLaunchKernelGGL(k_si_chain, dim3(n), dim3(1024), 0, s, F, list, 0);
+  return hipGetLastError();
+}
+hipError_t launch_forward_me(const FrameDev *F, const int *list, int n, int maxMB, CostTables ct, hipStream_t s) {
+  hipLaunchKernelGGL(k_forward_me, dim3(maxMB, n), dim3(256), 0, s, F, list, ct);
+  hipLaunchKernelGGL(k_si_chain, dim3(n), dim3(1024), 0, s, F, list, 1);
   return hipGetLastError();
 }
 hipError_t launch_noise_param(const FrameDev *F, const int *list, int n, int, hipStream_t s) {

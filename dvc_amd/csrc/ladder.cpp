@@ -3,7 +3,7 @@
 //
 // The reference reads text ladders (ReadLdpcaLadderFile_LdpcaDecoder.cpp:131-250)
 // and a packed inverse (Inverse_Matrix_H_Reg<N>.bin, :199-201).  This library
-// rebuilds the same data from a compact pack (dvc_amd/data/reg<N>.ladder: the
+// rebuilds the same data from a compact pack (dvc_amd/data/<kind><N>.ladder: the
 // encoder parity matrix and the per-code transmission pattern): decoder row k
 // of code c merges the encoder rows (tx_{k-1}, tx_k], tx_k = idx[k mod b] +
 // floor(k/b)*period, which is how the accumulated syndrome is sampled
@@ -74,8 +74,9 @@ static bool invert_gf2(const Ladder &L, std::vector<uint8_t> &inv) {
   return true;
 }
 
-int load_ladder(const std::string &dir, int N, Ladder &L) {
-  std::string path = dir + "/reg" + std::to_string(N) + ".ladder";
+int load_ladder(const std::string &dir, const char *kind, int N, Ladder &L) {
+  const std::string base = dir + "/" + kind + std::to_string(N);
+  const std::string path = base + ".ladder";
   std::vector<uint8_t> blob;
   DVC_CHECK(read_file(path, blob), DVC_EIO, "cannot read ladder pack " + path);
   DVC_CHECK(blob.size() >= 24 && memcmp(blob.data(), "DVCL", 4) == 0, DVC_EIO, "bad ladder pack " + path);
@@ -94,9 +95,12 @@ int load_ladder(const std::string &dir, int N, Ladder &L) {
   for (int i = 0; i < L.numCodes; i++) { uint16_t v; memcpy(&v, p, 2); p += 2; L.bitsPerPeriod[i] = v; }
   L.idx.resize((size_t)L.numCodes * L.period);
   for (size_t i = 0; i < L.idx.size(); i++) { uint16_t v; memcpy(&v, p, 2); p += 2; L.idx[i] = v; }
-  for (int k = 0; k < N; k++)
-    DVC_CHECK(L.jc[k + 1] - L.jc[k] == 3, DVC_EIO, "ladder pack: column degree is not 3");
-  DVC_CHECK(L.jc[N] == L.E && N % L.period == 0, DVC_EIO, "ladder pack: inconsistent sizes");
+  L.regular = true;
+  for (int k = 0; k < N; k++) {
+    DVC_CHECK(L.jc[k + 1] > L.jc[k], DVC_EIO, "ladder pack: empty column");
+    if (L.jc[k + 1] - L.jc[k] != 3) L.regular = false;
+  }
+  DVC_CHECK(L.jc[0] == 0 && L.jc[N] == L.E && N % L.period == 0, DVC_EIO, "ladder pack: inconsistent sizes");
 
   // Decoder codes (codeRate as ReadLdpcaLadderFile_LdpcaDecoder.cpp:182-183).
   L.syndLength.resize(L.numCodes);
@@ -120,7 +124,7 @@ int load_ladder(const std::string &dir, int N, Ladder &L) {
 
   // Full-rate inverse, cached as <pack>.invh.
   L.invRowBytes = 4 * ((N + 31) / 32);
-  std::string invPath = dir + "/reg" + std::to_string(N) + ".invh";
+  const std::string invPath = base + ".invh";
   std::vector<uint8_t> inv;
   if (!(read_file(invPath, inv) && inv.size() == (size_t)N * L.invRowBytes)) {
     DVC_CHECK(invert_gf2(L, inv), DVC_EIO, "ladder pack: parity matrix is singular");
@@ -141,9 +145,32 @@ int load_ladder(const std::string &dir, int N, Ladder &L) {
   return DVC_OK;
 }
 
+// Deals the columns of an irregular ladder to kIrregThreads threads: by
+// decreasing degree, each to the thread with the fewest edges so far, so the
+// threads of a workgroup carry nearly equal variable-node work.
+static void column_slots(const Ladder &L, std::vector<int16_t> &slots, int &perThread) {
+  const int T = kIrregThreads, N = L.N;
+  std::vector<int> order(N), load(T, 0), cnt(T, 0), owner(N);
+  for (int k = 0; k < N; k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&L](int a, int b) { return L.jc[a + 1] - L.jc[a] > L.jc[b + 1] - L.jc[b]; });
+  for (int k : order) {
+    int t = 0;
+    for (int u = 1; u < T; u++)
+      if (load[u] < load[t]) t = u;
+    owner[k] = t;
+    load[t] += L.jc[k + 1] - L.jc[k];
+    cnt[t]++;
+  }
+  perThread = *std::max_element(cnt.begin(), cnt.end());
+  slots.assign((size_t)T * perThread, (int16_t)-1);
+  std::fill(cnt.begin(), cnt.end(), 0);
+  for (int k = 0; k < N; k++) slots[(size_t)owner[k] * perThread + cnt[owner[k]]++] = (int16_t)k;
+}
+
 int upload_ladder(Ladder &L) {
   const int N = L.N, E = L.E, C = L.numCodes;
-  std::vector<uint32_t> colRows((size_t)C * N * 2);
+  std::vector<uint32_t> colRows(L.regular ? (size_t)C * N * 2 : 0), colEdges(L.regular ? 0 : (size_t)C * E);
   std::vector<uint16_t> rowPtr((size_t)C * (N + 1)), tx((size_t)C * N);
   DVC_CHECK(E <= 0xffff && N <= 0x2000, DVC_EIO, "ladder: more than 65535 edges or 8192 checks");
   for (int c = 0; c < C; c++) {
@@ -158,10 +185,14 @@ int upload_ladder(Ladder &L) {
     std::vector<uint32_t> field(E);
     for (int e = 0; e < E; e++) {
       const int rank = fill[dir[e]]++ - cnt[dir[e]];
+      if (!L.regular) {
+        colEdges[(size_t)c * E + e] = (uint32_t)dir[e] | ((uint32_t)rank << 16);
+        continue;
+      }
       DVC_CHECK(rank <= 0xff, DVC_EIO, "ladder: check degree above 256");
       field[e] = (uint32_t)dir[e] | ((uint32_t)rank << 13);
     }
-    for (int k = 0; k < N; k++) {  // every column has degree 3: edges 3k..3k+2
+    for (int k = 0; L.regular && k < N; k++) {  // every column has degree 3: edges 3k..3k+2
       const uint32_t f0 = field[3 * k], f1 = field[3 * k + 1], f2 = field[3 * k + 2];
       colRows[((size_t)c * N + k) * 2] = f0 | (f1 << 21);
       colRows[((size_t)c * N + k) * 2 + 1] = (f1 >> 11) | (f2 << 10);
@@ -169,7 +200,18 @@ int upload_ladder(Ladder &L) {
     const int *ix = &L.idx[(size_t)c * L.period];
     for (int k = 0; k < m; k++) tx[(size_t)c * N + k] = (uint16_t)(ix[k % b] + (k / b) * L.period);
   }
-  DVC_HIP(hipMalloc(&L.d_colRows, colRows.size() * 4));
+  if (L.regular) {
+    DVC_HIP(hipMalloc(&L.d_colRows, colRows.size() * 4));
+    } else {
+    std::vector<int16_t> slots;
+    column_slots(L, slots, L.colSlotsPerThread);
+    DVC_HIP(hipMalloc(&L.d_colEdges, colEdges.size() * 4));
+    DVC_HIP(hipMalloc(&L.d_jc, (N + 1) * sizeof(int)));
+    DVC_HIP(hipMalloc(&L.d_colSlots, slots.size() * sizeof(int16_t)));
+    DVC_HIP(hipMemcpy(L.d_colEdges, colEdges.data(), colEdges.size() * 4, hipMemcpyHostToDevice));
+    DVC_HIP(hipMemcpy(L.d_jc, L.jc.data(), (N + 1) * sizeof(int), hipMemcpyHostToDevice));
+    DVC_HIP(hipMemcpy(L.d_colSlots, slots.data(), slots.size() * sizeof(int16_t), hipMemcpyHostToDevice));
+  }
   DVC_HIP(hipMalloc(&L.d_rowPtr, rowPtr.size() * 2));
   DVC_HIP(hipMalloc(&L.d_tx, tx.size() * 2));
   DVC_HIP(hipMalloc(&L.d_syndLength, C * sizeof(int)));
@@ -187,10 +229,12 @@ int upload_ladder(Ladder &L) {
 }
 
 void free_ladder(Ladder &L) {
-  void *ptrs[] = {L.d_colRows, L.d_rowPtr, L.d_tx, L.d_syndLength, L.d_codeRate, L.d_invH, L.d_crcBasis};
+  void *ptrs[] = {L.d_colRows, L.d_colEdges, L.d_jc, L.d_colSlots, L.d_rowPtr,  L.d_tx,
+                  L.d_syndLength, L.d_codeRate, L.d_invH, L.d_crcBasis};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   L.d_colRows = nullptr;
+  L.d_colEdges = nullptr; L.d_jc = nullptr; L.d_colSlots = nullptr;
   L.d_rowPtr = L.d_tx = nullptr;
   L.d_syndLength = nullptr; L.d_codeRate = nullptr; L.d_invH = L.d_crcBasis = nullptr;
 }

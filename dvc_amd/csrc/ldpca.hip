@@ -84,31 +84,40 @@ __device__ __forceinline__ uint32_t acc_bit(gbyte_t *acc, int i) {
 #define DVC_LOG2 (0.693147181f)
 __global__ __launch_bounds__(256) void ce_llr_kernel(const PlaneDesc *planes, int nplanes, int fixed_start,
                                                      const float *codeRate1584, const float *codeRate6336,
-                                                     int numCodes) {
+                                                     int numCodes1584, int numCodes6336) {
   __shared__ float terms[6336];
   const int p = blockIdx.x;
   if (p >= nplanes) return;
   PlaneDesc d = planes[p];
   const int N = d.N;
+  int nan = 0;
   for (int k = threadIdx.x; k < N; k += blockDim.x) {
     float lr = d.LR[k];
     float Pco = lr / (1.0f + lr);
     float tmp = dvc_logf(1 - Pco);
     terms[k] = (Pco * (-dvc_logf(Pco) + tmp) - tmp) / DVC_LOG2;
-    d.LLR[k] = dvc_logf(lr);
+    const float llr = dvc_logf(lr);
+    d.LLR[k] = llr;
+    nan |= llr != llr;
   }
-  __syncthreads();
+  nan = __syncthreads_or(nan);
   if (threadIdx.x == 0) {
     float rlb = 0.0f;
     for (int k = 0; k < N; k++) rlb += terms[k];
     rlb /= (float)N;
+    // A NaN likelihood ratio makes the entropy NaN; the x86 path's is its
+    // default NaN (sign set), which the GPU's NaN bits need not be.  The
+    // start-code loop below then stops at 0, as :657-659 does.
+    if (rlb != rlb) rlb = __uint_as_float(0xffc00000u);
     const float *rate = N == 1584 ? codeRate1584 : codeRate6336;
+    const int numCodes = N == 1584 ? numCodes1584 : numCodes6336;
     int s = 0;
     while (s < numCodes - 1 && rate[s] < rlb) s++;
     PlaneState *st = d.state;
     st->entropy = rlb;
     if (st->start < 0 || !fixed_start) st->start = s;
     st->run = st->start;
+    st->taint = nan;
   }
 }
 // ------------------------------------------------------------------ BP
@@ -120,7 +129,20 @@ struct BpShape {
   static constexpr int SYN_WORDS = (N + 31) / 32;
 };
 
-template <int N, int T>
+// MODE1: beliefPropagation_OpenMP's semantics (ParallelMode 1), else
+// beliefPropagation's (ParallelMode 0).
+//
+// NaN likelihood ratios (PlaneState.taint).  On the x86 path a NaN LLR makes
+// every message that depends on it NaN.  With ParallelMode 0's sign-bit
+// decisions the outcome then hangs on NaN sign bits, and every code of such a
+// bitplane fails after MAX_ITERATIONS (the reference run over the golden GOP
+// frames and over seeded bitplanes with injected NaNs never converges), so a
+// tainted code is recorded as failed after max_iter iterations without being
+// run.  ParallelMode 1 decides LLR < 0, false for NaN, so only which values
+// are NaN matters: the kernel then keeps NaN through phi (whose branch-free
+// form would turn it into a finite value) and IEEE arithmetic carries it like
+// the CPU path.
+template <int N, int T, bool MODE1>
 __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
   using S = BpShape<N, T>;
   constexpr int E = S::E;
@@ -168,10 +190,21 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
     PlaneState *st = pd.state;
     const int run = st->run;
     const int level = run + w / a.nplanes;
+    const bool taint = st->taint != 0;
+    if (!MODE1 && taint) {
+      if (tid == 0) pd.rec[level] = make_int4(a.max_iter, REC_FAILED, -1, 0);
+      __syncthreads();  // every thread has read s_item before the next grab rewrites it
+      continue;
+    }
+    // phi of the kernel, NaN-preserving on a tainted ParallelMode 1 bitplane
+    auto phi_of = [&](float x) {
+      const float v = dvc_bp_phi_fast(x, s_logtab);
+      return (MODE1 && taint && x != x) ? x : v;
+    };
     // ParallelMode 1: the decisions carried into this code are known only for
     // the queue's first code; a later code runs as if its first decisions
     // differ from them and records the one case where that matters
-    const bool mode1 = a.mode == 1, exact = level == run;
+    const bool mode1 = MODE1, exact = level == run;
     const int m = L.syndLength[level];
     const uint2 *gcol = L.colRows + (size_t)level * N;
 
@@ -198,7 +231,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         float v = as_global(pd.LLR)[k];
         uint32_t s = f2u(v) >> 31;
         float mag = (s ? -1.0f : 1.0f) * v;
-        uint32_t f = (f2u(dvc_bp_phi_fast(mag, s_logtab)) & 0x7fffffffu) | (s << 31);
+        uint32_t f = (f2u(phi_of(mag)) & 0x7fffffffu) | (s << 31);
         int row[3], rank[3];
         col_edges(gcol[k], row, rank);
 #pragma unroll
@@ -302,7 +335,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
 #pragma unroll
           for (int q = 0; q < 3; q++) x[q] = u2f(R[q] & 0x7fffffffu) - u2f(ph[q] & 0x7fffffffu);
 #pragma unroll
-          for (int q = 0; q < 3; q++) v[q] = dvc_bp_phi_fast(x[q], s_logtab);
+          for (int q = 0; q < 3; q++) v[q] = phi_of(x[q]);
 #pragma unroll
           for (int q = 0; q < 3; q++) c2v[q] = ((R[q] ^ ph[q]) >> 31) ? v[q] * -1.0f : v[q] * 1.0f;
           float tmp = llr;
@@ -328,7 +361,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
             x[q] = (sg[q] ? -1.0f : 1.0f) * v2c;
           }
 #pragma unroll
-          for (int q = 0; q < 3; q++) v[q] = dvc_bp_phi_fast(x[q], s_logtab);
+          for (int q = 0; q < 3; q++) v[q] = phi_of(x[q]);
 #pragma unroll
           for (int q = 0; q < 3; q++) phi[pos[q]] = u2f((f2u(v[q]) & 0x7fffffffu) | (sg[q] << 31));
         }
@@ -410,8 +443,9 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
 // code the launch cancelled or never ran, the bitplane is queued again from
 // that code with the carried decisions now known (rerun_count counts them).
 // One wave per bitplane.
-// A slot index at or beyond slot_capacity (its decisions were not stored)
-// adds 1 << 20 to rerun_count, which the host reports as exhausted slots.
+// A slot index at or beyond slot_capacity (its decisions were not stored),
+// for a failed, mismatching or accepted code or a repeat case, adds 1 << 20
+// to rerun_count, which the host reports as exhausted slots.
 __global__ __launch_bounds__(64) void mode1_resolve_kernel(const PlaneDesc *planes, int nplanes, int numCodes, int N,
                                                            const uint8_t *slots, int slot_capacity, int *rerun_count) {
   const int p = blockIdx.x;
@@ -430,7 +464,7 @@ __global__ __launch_bounds__(64) void mode1_resolve_kernel(const PlaneDesc *plan
       return;
     }
     const int bs = L != run0 ? d.bcase[L] : 0;
-    if (bs - 1 >= slot_capacity || ((r.y == REC_FAILED || r.y == REC_CRC_MISMATCH) && (r.z < 0 || r.z >= slot_capacity))) {
+    if (bs - 1 >= slot_capacity || r.z < 0 || r.z >= slot_capacity) {  // every record reached here names a slot
       if (lane == 0) atomicAdd(rerun_count, 1 << 20);
       return;
     }
@@ -577,11 +611,13 @@ hipError_t launch_zero_ints(int *p, int n, hipStream_t s) {
 
 template <int N, int T>
 static hipError_t launch_bp(const BpArgs &a, int grid, hipStream_t s) {
-  hipLaunchKernelGGL((bp_persistent_kernel<N, T>), dim3(grid), dim3(T), 0, s, a);
+  if (a.mode == 1) hipLaunchKernelGGL((bp_persistent_kernel<N, T, true>), dim3(grid), dim3(T), 0, s, a);
+  else hipLaunchKernelGGL((bp_persistent_kernel<N, T, false>), dim3(grid), dim3(T), 0, s, a);
   return hipGetLastError();
 }
 
 hipError_t launch_bp_any(int N, int threads, const BpArgs &a, int grid, hipStream_t s) {
+  if (!a.lad.regular) return launch_bp_irregular(a, grid, s);
   if (N == 6336) {
     if (threads == 704) return launch_bp<6336, 704>(a, grid, s);
     if (threads == 512) return launch_bp<6336, 512>(a, grid, s);
@@ -594,7 +630,7 @@ hipError_t launch_bp_any(int N, int threads, const BpArgs &a, int grid, hipStrea
 hipError_t launch_ce_llr(const PlaneDesc *d_planes, int n, int fixed_start, const Ladder &l1584,
                          const Ladder &l6336, hipStream_t s) {
   hipLaunchKernelGGL(ce_llr_kernel, dim3(n), dim3(256), 0, s, d_planes, n, fixed_start, l1584.d_codeRate,
-                     l6336.d_codeRate, l6336.numCodes);
+                     l6336.d_codeRate, l1584.numCodes, l6336.numCodes);
   return hipGetLastError();
 }
 
@@ -619,7 +655,9 @@ hipError_t launch_gather_accepted(const PlaneDesc *d_planes, int n, int N, const
 LadderDev ladder_dev(const Ladder &L) {
   LadderDev d;
   d.N = L.N; d.E = L.E; d.numCodes = L.numCodes;
+  d.regular = L.regular ? 1 : 0;
   d.colRows = L.d_colRows; d.rowPtr = L.d_rowPtr; d.tx = L.d_tx;
+  d.colEdges = L.d_colEdges; d.jc = L.d_jc; d.colSlots = L.d_colSlots; d.colSlotsPerThread = L.colSlotsPerThread;
   d.syndLength = L.d_syndLength; d.invH = L.d_invH; d.crcBasis = L.d_crcBasis;
   return d;
 }

@@ -15,6 +15,9 @@ struct PlaneState {
   float entropy;  // H(X|Y) from ComputeCE
   int run;        // first code of the current launch's queue (start, or the code a
                   // ParallelMode 1 resolution resumes at; numCodes = nothing left)
+  int taint;      // 1: some LLR of the bitplane is NaN (set by the CE kernel): a likelihood
+                  // ratio condBitProb formed as 0/0 when both of a block's interval sums
+                  // underflow to zero (noiseModel.cpp:295-303)
 };
 
 struct PlaneDesc {
@@ -40,9 +43,18 @@ struct BpFrameLink {
   uint8_t *carry;  // ParallelMode 1: receives the accepted decisions (the next bitplane's carry)
 };
 
+// Run records per bitplane in the frame decoder's buffers: the most codes a
+// ladder has (the irregular QCIF ladder's 66; the regular ladders have 65).
+constexpr int kMaxCodes = 66;
+
 struct LadderDev {
   int N, E, numCodes;
+  int regular;           // every column has degree 3 (colRows), else colEdges/jc/colSlots
   const uint2 *colRows;  // per code and column: its three edges as (row, rank in row), see col_edges
+  const uint32_t *colEdges;  // irregular: per code and edge (column order) row | rank << 16
+  const int *jc;             // irregular: column pointers
+  const int16_t *colSlots;   // irregular: columns of each of kIrregThreads threads (-1: none)
+  int colSlotsPerThread;
   const uint16_t *rowPtr, *tx;
   const int *syndLength;
   const uint8_t *invH, *crcBasis;
@@ -86,6 +98,8 @@ hipError_t launch_mode1_resolve(const PlaneDesc *planes, int n, int numCodes, in
                                 int slot_capacity, int *rerun_count, hipStream_t s);
 hipError_t launch_ce_llr(const PlaneDesc *d_planes, int n, int fixed_start, const Ladder &l1584,
                          const Ladder &l6336, hipStream_t s);
+// BP on an irregular ladder (bp_irregular_kernel, kIrregThreads per workgroup).
+hipError_t launch_bp_irregular(const BpArgs &a, int grid, hipStream_t s);
 hipError_t launch_invh(const PlaneDesc *d_planes, int n, const LadderDev &L, uint8_t *slots, int *slot_count,
                        int slot_capacity, hipStream_t s);
 // Copies bitplane p's accepted decisions (N bytes, N a multiple of 16) to

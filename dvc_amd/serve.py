@@ -141,10 +141,42 @@ def run(ctx, streams, rate, seed=1, max_inflight=64, lanes=0, duration_cap=120.0
     return res, outputs
 
 
+def decode_gop(eng, video, wz, begin, end, sid=0):
+    """Decodes the WZ frames of decodeGOP(begin, end) (DVC_Decoder.cpp:26-474)
+    through engine `eng`.  A frame is submitted as soon as both of its
+    references are decoded, so the frames of one level of the hierarchy run
+    together; the key frames are video[begin] and video[end].  Returns
+    {cur: (recons, stats)} and the reference's decode order."""
+    order = []
+    gop_order(begin, end, order)
+    decoded = {begin: video[begin], end: video[end]}
+    todo, tickets, out = list(order), {}, {}
+    while todo or tickets:
+        for item in list(todo):
+            cur, past, fut = item
+            if past in decoded and fut in decoded:
+                fr = dict(past=decoded[past], fut=decoded[fut], orig=video[cur], wz=wz[cur], idx=(past, cur, fut))
+                tickets[eng.submit(fr, key_id(sid, past), key_id(sid, fut))] = cur
+                todo.remove(item)
+        eng.step()
+        for tk, cur in list(tickets.items()):
+            if eng.done(tk):
+                rec, st = eng.result(tk)
+                decoded[cur] = rec
+                out[cur] = (rec, st)
+                del tickets[tk]
+    return out, order
+
+
+def references(stream, cur):
+    """(past, fut) of WZ frame `cur` of a stream."""
+    return [(p, f) for c, p, f in stream.order if c == cur][0]
+
+
 def verify_solo(ctx, sample=8, seed=0):
     """Checks a sample of served frames (all of them when sample <= 0)
-    against solo decodes of the same frame with the references the serving
-    loop used."""
+    against decodes of the same frame alone (an engine with one frame in
+    flight), with the references the serving loop used."""
     def check(outputs, streams):
         rng = np.random.default_rng(seed)
         keys = list(outputs)
@@ -153,13 +185,19 @@ def verify_solo(ctx, sample=8, seed=0):
         else:
             pick = [keys[i] for i in rng.choice(len(keys), min(sample, len(keys)), replace=False)]
         bad = 0
-        for sid, cur in pick:
-            s = streams[sid]
-            past, fut = [(p, f) for c, p, f in s.order if c == cur][0]
-            fr = dict(past=s.decoded[past], fut=s.decoded[fut], orig=s.video[cur], wz=s.wz[cur], idx=(past, cur, fut))
-            (rec, st), = ctx.decode_wz_frames([fr], batched=True)
-            srec, sst = outputs[(sid, cur)]
-            bad += int(not (np.array_equal(rec, srec) and st["levels"] == sst["levels"]))
+        eng = _lib.Engine(ctx, max_inflight=1, lanes=1)
+        try:
+            for sid, cur in pick:
+                s = streams[sid]
+                past, fut = references(s, cur)
+                fr = dict(past=s.decoded[past], fut=s.decoded[fut], orig=s.video[cur], wz=s.wz[cur],
+                          idx=(past, cur, fut))
+                (rec, st), = eng.decode_all([fr])
+                srec, sst = outputs[(sid, cur)]
+                bad += int(not (np.array_equal(rec, srec) and st["levels"] == sst["levels"]
+                                and st["psnr"] == sst["psnr"]))
+        finally:
+            eng.close()
         return dict(checked=len(pick), mismatches=bad)
     return check
 

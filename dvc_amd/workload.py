@@ -133,8 +133,11 @@ def crc8(data):
     return crc
 
 
-def load_ladder(n):
-    blob = (DATA / f"reg{n}.ladder").read_bytes()
+def load_ladder(n, kind="reg"):
+    """The ladder pack dvc_amd/data/<kind><n>.ladder: kind "reg" (the
+    regular ladders, N = 1584 or 6336) or "irreg" (the irregular QCIF ladder
+    of irReg1584.lad, N = 1584 only)."""
+    blob = (DATA / f"{kind}{n}.ladder").read_bytes()
     assert blob[:4] == b"DVCL"
     ver, N, E, period, codes = struct.unpack_from("<5I", blob, 4)
     off = 24
@@ -157,10 +160,16 @@ def load_ladder(n):
 _LADDERS = {}
 
 
-def ladder(n):
-    if n not in _LADDERS:
-        _LADDERS[n] = load_ladder(n)
-    return _LADDERS[n]
+def ladder(n, kind="reg"):
+    if (n, kind) not in _LADDERS:
+        _LADDERS[(n, kind)] = load_ladder(n, kind)
+    return _LADDERS[(n, kind)]
+
+
+def qcif_kind(n, kind):
+    """The ladder kind used for code length n: the irregular ladder exists for
+    QCIF (N = 1584) only; CIF always uses the regular one."""
+    return kind if n == 1584 else "reg"
 
 
 def ldpca_encode(L, bits):
@@ -224,10 +233,11 @@ class WZFrame:
     bitplanes: np.ndarray    # uint8[nbp, N] (encoder side, for checks)
 
 
-def encode_wz_frame(frame, qindex):
+def encode_wz_frame(frame, qindex, kind="reg"):
+    """kind: the QCIF ladder ("reg" or "irreg"; CIF always regular)."""
     h, w = frame.shape
     n = (h // 4) * (w // 4)
-    L = ladder(n)
+    L = ladder(n, qcif_kind(n, kind))
     coef = forward44_blocks(frame_to_blocks(frame))          # [bh, bw, 16]
     ac_range = np.abs(coef).reshape(-1, 16).max(0)
     ac_range[0] = 0
@@ -263,7 +273,7 @@ def encode_wz_frame(frame, qindex):
 
 
 # ----------------------------------------------------------------- LDPCA set
-def ldpca_bitplane_set(n_cif=384, n_qcif=128, seed=2, delta=4.0):
+def ldpca_bitplane_set(n_cif=384, n_qcif=128, seed=2, delta=4.0, kind="reg"):
     """Seeded bitplanes with Laplacian soft inputs.
 
     Per bitplane: p ~ U[0.02, 0.5], bits ~ Bernoulli(p); alpha ~ U[0.05, 2.0];
@@ -272,7 +282,8 @@ def ldpca_bitplane_set(n_cif=384, n_qcif=128, seed=2, delta=4.0):
     exp(alpha*(|y - delta| - |y|)) * (1-p)/p, computed in double, clipped to
     the range condBitProb produces (noiseModel.cpp:301-306) and stored as
     float32.  Returns a list of dicts (N, bits, LR, acc, crc, p, alpha) with
-    CIF and QCIF bitplanes interleaved.
+    CIF and QCIF bitplanes interleaved.  kind: the QCIF ladder the syndromes
+    are formed with ("reg" or "irreg").
     """
     rng = np.random.default_rng(seed)
     sizes = [6336] * n_cif + [1584] * n_qcif
@@ -288,6 +299,6 @@ def ldpca_bitplane_set(n_cif=384, n_qcif=128, seed=2, delta=4.0):
         llr = alpha * (np.abs(y - delta) - np.abs(y)) + np.log((1.0 - p) / p)
         lr = np.exp(np.clip(llr, -30, 30)).astype(np.float32)
         lr = np.clip(lr, lo, hi).astype(np.float32)
-        acc, crc = ldpca_encode(ladder(n), bits)
+        acc, crc = ldpca_encode(ladder(n, qcif_kind(n, kind)), bits)
         out.append(dict(N=n, bits=bits, LR=lr, acc=acc, crc=crc, p=p, alpha=alpha))
     return out

@@ -48,18 +48,21 @@ typedef struct {
   int rate_step;         /* rate search: 0 = every code from the start code up (LdpcaDecoder.cpp:674);
                             1 = the DYNAMICSTEP build's growing steps (:19-21, 669-672, 760-768): codes
                             start + 0, 1, 3, 6, 10, 14, ... then the full rate; needs parallel_mode 0 */
+  int qcif_ladder;       /* the N = 1584 ladder: 0 = the regular RegDecoder1584.lad (65 codes, column
+                            degree 3); 1 = the irregular irReg1584.lad (66 codes, column degrees 2..21),
+                            the reference's alternative at main.cpp:84-85.  N = 6336 is always regular. */
 } dvc_params;
 
 /* InitCUDA + LDPCA_CUDA_Init_Variable (LDPCA_cuda.cu:36-89, 546-763): selects the
- * device and makes both ladders (N=1584 and N=6336, 65 codes each) and their
- * full-rate inverses resident. */
+ * device and makes both ladders (N=1584 and N=6336) and their full-rate
+ * inverses resident. */
 int dvc_ctx_create(const dvc_params *params, dvc_ctx **out);
 /* LDPCA_CUDA_Free_Variable and the *_Free_Buffer calls. */
 int dvc_ctx_destroy(dvc_ctx *ctx);
 const char *dvc_last_error(void);
 /* Host-only: checks the ladder packs in data_dir (NULL = the library's own
- * data directory) and writes the cached full-rate inverses (reg<N>.invh) if
- * they are missing.  Needs no GPU. */
+ * data directory) and writes the cached full-rate inverses (<kind><N>.invh)
+ * if they are missing.  Needs no GPU. */
 int dvc_prepare_data(const char *data_dir);
 
 /* Kernel timing (no counterpart in the reference, which times whole
@@ -142,7 +145,8 @@ int dvc_set_tuning(dvc_ctx *ctx, const dvc_tuning *tuning);
 /* Verification entry (no counterpart in the reference): evaluates one of the
  * device's scalar routines on n inputs.  func 0..3: float logf, expf, tanhf,
  * the BP kernel phi(x) = -logf(tanhf((x+1e-9f)/2)) (float in, float out);
- * 4: double sqrt of int32 inputs; 5: double exp (double in, double out);
+ * 4: double sqrt of int32 inputs; 5: double exp as reconstruction evaluates it, dvc_exp
+ * (double in, double out);
  * 6: the branch-free phi for x >= 0 (float in, float out); 7: its shortened
  * form, which the BP kernel evaluates (float in, float out). */
 int dvc_selftest_unary(dvc_ctx *ctx, int func, const void *in, void *out, int n);
@@ -152,6 +156,14 @@ int dvc_selftest_unary(dvc_ctx *ctx, int func, const void *in, void *out, int n)
  * skipped).  counts[0] = patterns compared, counts[1] = mismatches,
  * counts[2] = lowest mismatching pattern (0xffffffff when none). */
 int dvc_selftest_phi_sweep(dvc_ctx *ctx, uint32_t lo, uint32_t hi, unsigned long long *counts);
+/* Calibration entry (no counterpart in the reference): times one launch of a
+ * VALU issue-rate kernel -- 8 independent chains per lane of one operation
+ * kind (0 fp32 fma, 1 fp32 add, 2 fp32 mul, 3 fp64 fma, 4 int32 add, 5
+ * v_sad_u8, 6 v_mqsad_u32_u8), built
+ * with the library's own flags -- over waves_per_cu (4..32, a multiple of 4)
+ * waves per CU and `iters` trips of 128 operations.  *ms = kernel time (HIP
+ * events), *wave_instr = VALU wave64 instructions the launch issues. */
+int dvc_selftest_valu_rate(dvc_ctx *ctx, int kind, int waves_per_cu, int iters, double *ms, double *wave_instr);
 
 /* Device buffers for callers without a HIP runtime of their own (no
  * counterpart in the reference): allocation on the context's device and
@@ -171,7 +183,6 @@ typedef struct {
   int used_inverse[128];
   float entropy[128];
   double psnr;                /* against orig when given (frame.cpp:115-124) */
-  int recon_host_fixups;      /* reconstruction entries recomputed with the C library's exp */
 } dvc_frame_stats;
 
 typedef struct {
@@ -214,6 +225,62 @@ typedef struct {
                                  positions of the bands, 0 elsewhere */
 } dvc_si_request;
 int dvc_side_info_batch(dvc_ctx *ctx, int n, const dvc_si_request *reqs);
+
+/* ---------------------------------------------------------------- per-band entry points
+ * The accelerator calls of a DvcDec that keeps its own decodeGOP loop
+ * (ParallelMode 2/3 at DVC_Decoder.cpp:91-95, 184-185, 218, 328-329, 380 and
+ * sideInfoCreation.cpp:503), on the reference's own data layouts.  Where the
+ * reference keeps implicit device state between calls (the band's noise
+ * distribution between noiseDistribution_CUDA and condBitProb_CUDA, the key
+ * frames uploaded by ME_CUDA_Init_Variable) the context keeps it.  Host
+ * pointers; every call returns when its outputs are written.  Skip and intra
+ * blocks are not supported: the SIR must list every 4x4 block in raster order
+ * (blkIdx i at entry i), as initSIR does when no block is skipped. */
+typedef struct {                 /* SMF, DvcDec/inc/global.h:146-152 */
+  float prob[81];
+  int searchTop, searchLeft, searchRight, searchBottom;
+  int searchWidth, blkIdx, probLen;
+} dvc_smf;
+typedef struct {                 /* SearchInfo, global.h:158-167 */
+  int searchTop, searchBottom, searchLeft, searchRight;
+  int top, left, searchWidth, searchRange, blkIdx;
+} dvc_search_info;
+typedef struct { double x, y; } dvc_point;                  /* Point, global.h:118-120 */
+typedef struct { dvc_point past, future, vector; } dvc_mv;  /* MV, global.h:122-126 */
+
+/* CNM_CUDA_Init_Variable (noiseModel_kernel.cu:61) + ME_CUDA_Init_Variable
+ * (updateSI_kernel.cu:90): the WZ frame the following per-band calls work on
+ * -- its size, Qindex, the decoded key frames past (frame past_idx) and fut
+ * (fut_idx), uploaded and filtered once (3x3 low-pass and half-pel FIR), and
+ * its SIR (sir may be NULL: every block). */
+int dvc_band_frame_init(dvc_ctx *ctx, int width, int height, int qindex, const uint8_t *past, const uint8_t *fut,
+                        int past_idx, int cur_idx, int fut_idx, const dvc_search_info *sir, int search_candidate);
+/* init_noiseDst_CUDA + noiseDistribution_CUDA (noiseModel_kernel.cu:84, 140;
+ * noiseDistribution, noiseModel.cpp:77-132): the band's noise distribution
+ * from alpha (W*H), the SMF (one per block) and the SI's over-complete
+ * transform (Trans oct[(H-3)*(W-3)], 16 ints each), kept in the context for
+ * dvc_cond_bit_prob.  dist (may be NULL) also receives it in noiseDistribution's
+ * layout: per block `range` (DC) or 2*range+1 (AC) floats. */
+int dvc_noise_distribution(dvc_ctx *ctx, int range, int index_in_blk, const float *alpha, const dvc_smf *smf,
+                           const int *oct, float *dist);
+/* condBitProb_CUDA (noiseModel_kernel.cu:346; condBitProb, noiseModel.cpp:191-310):
+ * the N likelihood ratios of bit `bit` of coefficient index_in_blk from the
+ * decoded indices so far (recons_trans, W*H) and the distribution of the last
+ * dvc_noise_distribution call for the same band (else DVC_ESTATE). */
+int dvc_cond_bit_prob(dvc_ctx *ctx, float *LR, int index_in_blk, int bit, int step_size, const int *recons_trans,
+                      int range);
+/* updateSI_CUDA (updateSI_kernel.cu:298; updateSI + subPelRefine,
+ * sideInfoCreation.cpp:598-736): refreshes side_info (W*H, in/out) from the
+ * partially reconstructed frame and the key frames of dvc_band_frame_init. */
+int dvc_update_si(dvc_ctx *ctx, uint8_t *side_info, const uint8_t *recons);
+/* updateSMF_CUDA (motionLearning_kernel.cu:91; updateSMF, motionLearning.cpp:
+ * 70-127): refreshes prob[0..probLen) of every SMF (in/out). */
+int dvc_update_smf(dvc_ctx *ctx, dvc_smf *smf, const uint8_t *recons, const uint8_t *side_info);
+/* forwardME_CUDA (forwardME_kernel.cu:187; forwardME, sideInfoCreation.cpp:
+ * 61-145): the macroblock MVs (W/16 * H/16, raster order) of the WZ frame
+ * cur_idx from the low-pass key frames.  Independent of dvc_band_frame_init. */
+int dvc_forward_me(dvc_ctx *ctx, int width, int height, const uint8_t *fut_lowpass, const uint8_t *past_lowpass,
+                   int past_idx, int cur_idx, int fut_idx, dvc_mv *mb_mv);
 
 /* ---------------------------------------------------------------- serving engine
  * Continuous batching (no counterpart in the reference, which decodes frames

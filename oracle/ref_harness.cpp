@@ -39,7 +39,12 @@ static int g_ent_n = 0;
 
 // The wrapped functions are C++ (mangled) symbols; the linker's --wrap works on
 // those names, so the wrapper and the real entry carry the mangled suffixes.
+#ifdef DVC_ACCEL
+// the accelerated build: LdpcaDecoder goes to the GPU library (ref_accel_stubs.cpp)
+#define REAL_LDPCA dvc_stub_LdpcaDecoder
+#else
 #define REAL_LDPCA __real__Z12LdpcaDecoderPfPhS0_Pihf
+#endif
 #define REAL_CE __real__Z9ComputeCEPf
 #define REAL_CE_OMP __real__Z16ComputeCE_OpenMPPf
 extern "C" {

@@ -20,6 +20,17 @@ Fixtures (all small JSON):
   ldpca_dynstep.json, frames_dynstep.json  the ldpca and frames fixtures from
                   the decoder built with DYNAMICSTEP defined in
                   LdpcaDecoder.cpp (DVC_GOLDEN_ONLY=dynstep writes just these)
+  ldpca_irreg.json, frames_irreg.json, frames_irreg_mode1.json  the irregular
+                  QCIF ladder (irReg1584.lad): reference LdpcaDecoder on
+                  seeded bitplanes and decodeGOP on QCIF frames, ParallelMode
+                  0 and 1 (DVC_GOLDEN_ONLY=irreg writes just these)
+  ldpca_nan.json, ldpca_nan_mode1.json  reference LdpcaDecoder, ParallelMode
+                  0 and 1, on seeded bitplanes with NaN likelihood ratios
+                  (DVC_GOLDEN_ONLY=nan writes just these)
+  frames_gop.json, frames_gop_mode1.json  reference decodeGOP(0, gop) for
+                  GOP 4 and 8 (decoded WZ frames are references of later
+                  ones), ParallelMode 0 and 1 (DVC_GOLDEN_ONLY=gop writes
+                  just these)
 Inputs are regenerated from their seeds by dvc_amd.workload, so a fixture
 holds only seeds, sizes and expected outputs.
 """
@@ -66,6 +77,9 @@ def ladder_fixture():
         # Row-degree histograms of the N=6336 decoder codes, as listed in the
         # reference's kernel comments (DvcDec/src/LDPCA_kernel.cu:321,364,465,584):
         # {code: {degree: rows}}
+        # the irregular QCIF ladder irReg1584.lad (main.cpp:84-85): decoder rows
+        # of its 66 codes; the reference ships no inverse for it
+        "irreg": irreg_ladder_facts(),
         "cif_row_degrees": {
             "1": {"48": 96, "50": 5, "51": 84, "52": 7, "98": 10, "99": 78, "100": 8},
             "3": {"23": 5, "24": 84, "25": 7, "27": 96, "48": 192, "50": 10, "51": 78, "52": 8},
@@ -73,6 +87,13 @@ def ladder_fixture():
             "63": {"2": 15, "3": 6114, "4": 15, "6": 96},
         },
     }
+
+
+def irreg_ladder_facts():
+    L = O.Ladder(lad_path=REF / "release/DvcDec/LDPCA_LadderFile/irReg1584.lad")
+    ir = np.stack([L.ir(c) for c in range(L.num_codes)]).astype(np.int32)
+    return {"N": 1584, "num_codes": int(L.num_codes), "E": int(L.E), "synd_length": L.synd_length().tolist(),
+            "decoder_ir_sha256": sha(ir)}
 
 
 def encoder_fixture():
@@ -171,8 +192,158 @@ def frames_fixture(mode=0, dynstep=False):
     return {"max_iter": 50, "cases": out} if not dynstep else {"max_iter": 50, "rate_step": 1, "cases": out}
 
 
+def ldpca_irreg_fixture():
+    """Reference LdpcaDecoder (ParallelMode 0) with the irregular QCIF ladder
+    irReg1584.lad (main.cpp:84-85) on seeded bitplanes whose syndromes come
+    from irRegEncoder1584.lad."""
+    R = O.ref_dec()
+    O.RefDecoder(176, 144, 4, kind="irreg")
+    out = []
+    for i, pl in enumerate(W.ldpca_bitplane_set(n_cif=0, n_qcif=24, seed=35, kind="irreg")):
+        lr = pl["LR"].copy()
+        ent = R.ref_compute_ce(O.P(lr))
+        acc_bits = np.unpackbits(pl["acc"]).astype(np.uint8)
+        dec = np.full(pl["N"], 0xAA, np.uint8)
+        rate = ctypes.c_int()
+        R.ref_ldpca(O.P(lr), O.P(acc_bits), ctypes.c_uint8(pl["crc"]), ctypes.c_float(ent), O.P(dec),
+                    ctypes.byref(rate))
+        out.append({"set": [0, 24, 35], "index": i, "N": pl["N"], "rate": rate.value,
+                    "level": (rate.value - 8) // (pl["N"] // 66) - 1, "entropy_bits": fbits(ent),
+                    "decoded_sha256": sha(dec), "source_ok": bool(np.array_equal(dec, pl["bits"]))})
+    return {"max_iter": 50, "mode": 0, "ladder": "irreg", "items": out}
+
+
+IRREG_FRAME_CASES = [(176, 144, 4, 1020, 3, 3, 1), (176, 144, 8, 1021, 3, 3, 1)]
+
+
+def frames_irreg_fixture(mode=0):
+    """Reference decodeGOP on QCIF WZ frames with the irregular ladder."""
+    out = []
+    for (w, h, q, seed, shift, nf, fi) in IRREG_FRAME_CASES:
+        vid = W.synthetic_video(w, h, nf, shift, seed)
+        wz = W.encode_wz_frame(vid[fi], q, kind="irreg")
+        R = O.RefDecoder(w, h, q, mode=mode, nthreads=1, kind="irreg")
+        rec, info = R.decode_wz_frame(vid[0], vid[2], vid[fi], wz)
+        out.append({"W": w, "H": h, "qindex": q, "seed": seed, "shift": shift, "frames": nf, "wz": fi,
+                    "mode": mode, "levels": info["levels"], "rates": info["rates"],
+                    "entropy_bits": [fbits(e) for e in info["entropy"]], "bitrate": info["bitrate"],
+                    "psnr": info["psnr"], "recons_sha256": sha(rec)})
+        print(f"irregular frame {w}x{h} q{q} mode{mode}: psnr {info['psnr']:.4f} bitrate {info['bitrate']}",
+              flush=True)
+    return {"max_iter": 50, "ladder": "irreg", "cases": out}
+
+
+NAN_BITS = 0xFFC00000  # x86's default NaN: what 0/0 yields on the reference's CPU path
+
+
+def nan_planes():
+    """Seeded bitplanes (workload.ldpca_bitplane_set) with 1 or 3 likelihood
+    ratios replaced by NaN, as condBitProb yields them when both of a block's
+    interval sums underflow to zero (p0/(p0+p1) = 0/0)."""
+    rng = np.random.default_rng(77)
+    out = []
+    for (n_cif, n_qcif, seed) in [(0, 12, 61), (3, 0, 62)]:
+        for i, pl in enumerate(W.ldpca_bitplane_set(n_cif=n_cif, n_qcif=n_qcif, seed=seed)):
+            nn = 1 if i % 2 == 0 else 3
+            pos = sorted(int(v) for v in rng.choice(pl["N"], nn, replace=False))
+            out.append(((n_cif, n_qcif, seed), i, pl, pos))
+    return out
+
+
+def with_nans(lr, pos):
+    lr = np.array(lr, np.float32)
+    lr.view(np.uint32)[pos] = NAN_BITS
+    return lr
+
+
+def ldpca_nan_fixture(mode):
+    """Reference LdpcaDecoder (ParallelMode `mode`, one thread) on bitplanes
+    with NaN likelihood ratios: accepted code, rate, entropy bits, decoded
+    hash, and (ParallelMode 1) the BP iteration count."""
+    R = O.ref_dec()
+    items = []
+    for key, i, pl, pos in nan_planes():
+        w, h = (352, 288) if pl["N"] == 6336 else (176, 144)
+        O.RefDecoder(w, h, 4, mode=mode, nthreads=1)
+        lr = with_nans(pl["LR"], pos)
+        ent = R.ref_compute_ce(O.P(lr.copy())) if mode == 0 else R.ref_compute_ce_omp(O.P(lr.copy()))
+        acc_bits = np.unpackbits(pl["acc"]).astype(np.uint8)
+        dec = np.full(pl["N"], 0xAA, np.uint8)
+        rate = ctypes.c_int()
+        it0 = R.ref_bp_iterations()
+        R.ref_ldpca(O.P(lr), O.P(acc_bits), ctypes.c_uint8(pl["crc"]), ctypes.c_float(ent), O.P(dec), ctypes.byref(rate))
+        items.append({"set": list(key), "index": i, "N": pl["N"], "nan_positions": pos, "rate": rate.value,
+                      "level": (rate.value - 8) // (pl["N"] // 66) - 2, "entropy_bits": fbits(ent),
+                      "decoded_sha256": sha(dec), "source_ok": bool(np.array_equal(dec, pl["bits"])),
+                      "bp_iterations": (R.ref_bp_iterations() - it0) if mode == 1 else None})
+    return {"max_iter": 50, "mode": mode, "nan_bits": NAN_BITS, "items": items}
+
+
+GOP_CASES = [
+    # (W, H, qindex, seed, shift, gop): frames 0..gop of a seeded video, key
+    # frames at 0 and gop, the WZ frames in between decoded hierarchically
+    (176, 144, 4, 1010, 3, 4),
+    (176, 144, 4, 1011, 3, 8),
+    (352, 288, 8, 1012, 6, 4),
+    (352, 288, 8, 1013, 6, 8),
+]
+GOP_CASES_MODE1 = [(176, 144, 4, 1011, 3, 8)]
+
+
+def frames_gop_fixture(mode=0):
+    """Reference decodeGOP(0, gop) for GOP 4 and 8: per WZ frame in the
+    reference's recursion order (DVC_Decoder.cpp:471-472), its references,
+    accepted code per bitplane, entropy bits, bitrate, PSNR and the hash of
+    its reconstruction (which later frames of the GOP take as a reference)."""
+    out = []
+    for (w, h, q, seed, shift, gop) in (GOP_CASES if mode == 0 else GOP_CASES_MODE1):
+        vid = W.synthetic_video(w, h, gop + 1, shift, seed)
+        wz = {k: W.encode_wz_frame(vid[k], q) for k in range(1, gop)}
+        R = O.RefDecoder(w, h, q, mode=mode, nthreads=1, nframes=gop + 1)
+        res, order = R.decode_gop(vid, wz, 0, gop)
+        frames = []
+        for cur, past, fut in order:
+            rec, info = res[cur]
+            frames.append({"cur": cur, "past": past, "fut": fut, "levels": info["levels"],
+                           "entropy_bits": [fbits(e) for e in info["entropy"]], "bitrate": info["bitrate"],
+                           "psnr": info["psnr"], "recons_sha256": sha(rec)})
+            print(f"gop case {w}x{h} q{q} gop{gop} mode{mode} frame {cur} ({past},{fut}): "
+                  f"psnr {info['psnr']:.4f} bitrate {info['bitrate']}", flush=True)
+        out.append({"W": w, "H": h, "qindex": q, "seed": seed, "shift": shift, "gop": gop, "mode": mode,
+                    "frames": frames})
+    return {"max_iter": 50, "cases": out}
+
+
 def main():
     assert O.have_reference_build(), "run `make -C oracle ref` first"
+    if os.environ.get("DVC_GOLDEN_ONLY") == "ladder":
+        (HERE / "ladder.json").write_text(json.dumps(ladder_fixture(), indent=1))
+        return
+    if os.environ.get("DVC_GOLDEN_ONLY") == "irreg":
+        (HERE / "ldpca_irreg.json").write_text(json.dumps(ldpca_irreg_fixture(), indent=1))
+        (HERE / "frames_irreg.json").write_text(json.dumps(frames_irreg_fixture(0), indent=1))
+        env = dict(os.environ, DVC_GOLDEN_ONLY="irreg_mode1", MALLOC_PERTURB_="85", OMP_NUM_THREADS="1")
+        subprocess.run([sys.executable, __file__], env=env, check=True)
+        return
+    if os.environ.get("DVC_GOLDEN_ONLY") == "irreg_mode1":
+        (HERE / "frames_irreg_mode1.json").write_text(json.dumps(frames_irreg_fixture(1), indent=1))
+        return
+    if os.environ.get("DVC_GOLDEN_ONLY") == "nan":
+        (HERE / "ldpca_nan.json").write_text(json.dumps(ldpca_nan_fixture(0), indent=1))
+        env = dict(os.environ, DVC_GOLDEN_ONLY="nan_mode1", MALLOC_PERTURB_="85", OMP_NUM_THREADS="1")
+        subprocess.run([sys.executable, __file__], env=env, check=True)
+        return
+    if os.environ.get("DVC_GOLDEN_ONLY") == "nan_mode1":
+        (HERE / "ldpca_nan_mode1.json").write_text(json.dumps(ldpca_nan_fixture(1), indent=1))
+        return
+    if os.environ.get("DVC_GOLDEN_ONLY") == "gop":
+        (HERE / "frames_gop.json").write_text(json.dumps(frames_gop_fixture(mode=0), indent=1))
+        env = dict(os.environ, DVC_GOLDEN_ONLY="gop_mode1", MALLOC_PERTURB_="85", OMP_NUM_THREADS="1")
+        subprocess.run([sys.executable, __file__], env=env, check=True)
+        return
+    if os.environ.get("DVC_GOLDEN_ONLY") == "gop_mode1":
+        (HERE / "frames_gop_mode1.json").write_text(json.dumps(frames_gop_fixture(mode=1), indent=1))
+        return
     if os.environ.get("DVC_GOLDEN_ONLY") == "dynstep":
         # the DYNAMICSTEP build's fixtures alone (the others are unchanged by it)
         (HERE / "ldpca_dynstep.json").write_text(json.dumps(ldpca_dynstep_fixture(), indent=1))

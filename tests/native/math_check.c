@@ -2,6 +2,7 @@
  * host C library over a range of float bit patterns.
  *   math_check <func> <first_bits_hex> <last_bits_hex> <stride>
  * func: logf | expf | tanhf | expm1f | phi | phi_nonneg | phi_fast
+ *   math_check exp <count> <seed_hex> 0   (double exp on pseudo-random arguments)
  * Prints "<func> checked=<n> mismatches=<m>" and the first few mismatches.
  * phi is -logf(tanhf((x + (float)1e-9)/2)), the check-node kernel of
  * DvcDec/src/LdpcaDecoder.cpp:513.  phi_nonneg is the branch-free form the BP
@@ -15,9 +16,41 @@
 
 static float ref_phi(float x) { return -logf(tanhf((x + (float)0.000000001) / 2)); }
 
+/* Double exp (reconstruction.cpp:211-277) on n pseudo-random arguments
+ * (xorshift64, seed): a third uniform in [-700, 700], a third in [-20, 20], a
+ * third the arguments reconstruction forms, +-alpha*k with alpha a float in
+ * (1e-4, 64) and k an integer in [0, 4096]. */
+static int check_exp(long long n, unsigned long long seed) {
+  long long mism = 0;
+#pragma omp parallel for reduction(+:mism) schedule(static)
+  for (long long i = 0; i < n; i++) {
+    unsigned long long s = seed ^ (0x9e3779b97f4a7c15ull * (unsigned long long)(i + 1));
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    s ^= s << 13; s ^= s >> 7; s ^= s << 17;
+    const double u = (double)(s >> 11) / 9007199254740992.0;
+    double x;
+    if (i % 3 == 0) x = u * 1400.0 - 700.0;
+    else if (i % 3 == 1) x = u * 40.0 - 20.0;
+    else {
+      const float a = (float)(1e-4 + u * 64.0);
+      const int k = (int)((s >> 3) % 4097);
+      x = ((s & 1) ? -(double)a : (double)a) * (double)k;
+    }
+    const double g = exp(x), m = dvc_exp(x);
+    if (dvcm_asu64(g) != dvcm_asu64(m)) {
+      mism++;
+#pragma omp critical
+      { static int shown = 0; if (shown++ < 5) fprintf(stderr, "mismatch exp x=%a libm=%a ours=%a\n", x, g, m); }
+    }
+  }
+  printf("exp checked=%lld mismatches=%lld\n", n, mism);
+  return mism ? 1 : 0;
+}
+
 int main(int argc, char **argv) {
   if (argc != 5) { fprintf(stderr, "usage: %s func first last stride\n", argv[0]); return 2; }
   const char *fn = argv[1];
+  if (!strcmp(fn, "exp")) return check_exp(strtoll(argv[2], 0, 10), strtoull(argv[3], 0, 16));
   unsigned long long first = strtoull(argv[2], 0, 16), last = strtoull(argv[3], 0, 16);
   unsigned long long stride = strtoull(argv[4], 0, 10);
   if (stride == 0) stride = 1;

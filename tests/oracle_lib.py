@@ -85,7 +85,7 @@ class Ladder:
     """Decoder ladder held by the oracle (derived from the committed pack or
     read from the reference's text files)."""
 
-    def __init__(self, n=None, lad_path=None, inv_path=None):
+    def __init__(self, n=None, lad_path=None, inv_path=None, kind="reg"):
         self.L = OrcLadder()
         lib = orc()
         if lad_path is not None:
@@ -95,7 +95,7 @@ class Ladder:
             self.enc = None
         else:
             from dvc_amd import workload
-            pk = workload.ladder(n)
+            pk = workload.ladder(n, kind)
             self.enc = OrcEncLadder()
             self._jc = np.ascontiguousarray(pk.jc, np.int32)
             self._ir = np.ascontiguousarray(pk.ir, np.int32)
@@ -132,10 +132,13 @@ class Ladder:
 _LADDERS = {}
 
 
-def ladder(n):
-    if n not in _LADDERS:
-        _LADDERS[n] = Ladder(n)
-    return _LADDERS[n]
+def ladder(n, kind="reg"):
+    """The oracle's decoder ladder derived from the pack <kind><n>.ladder
+    (kind "irreg": the irregular QCIF ladder; CIF is always regular)."""
+    kind = kind if n == 1584 else "reg"
+    if (n, kind) not in _LADDERS:
+        _LADDERS[(n, kind)] = Ladder(n, kind=kind)
+    return _LADDERS[(n, kind)]
 
 
 def compute_ce(lr):
@@ -143,12 +146,13 @@ def compute_ce(lr):
     return float(orc().orc_compute_ce(P(lr), lr.size))
 
 
-def ldpca_decode(lr, acc_bytes, crc, mode=0, max_iter=50, nthreads=1, decoded_init=None, entropy=None, rate_step=0):
-    """Oracle LdpcaDecoder (rate_step 1: the DYNAMICSTEP build's search):
-    returns (decoded, stats dict, entropy)."""
+def ldpca_decode(lr, acc_bytes, crc, mode=0, max_iter=50, nthreads=1, decoded_init=None, entropy=None, rate_step=0,
+                 kind="reg"):
+    """Oracle LdpcaDecoder (rate_step 1: the DYNAMICSTEP build's search; kind:
+    the QCIF ladder): returns (decoded, stats dict, entropy)."""
     lr = np.array(lr, np.float32)  # converted to LLR in place, keep the caller's copy
     n = lr.size
-    L = ladder(n)
+    L = ladder(n, kind)
     ent = compute_ce(lr) if entropy is None else entropy
     acc_bits = np.unpackbits(np.asarray(acc_bytes, np.uint8))[:n].copy()
     dec = np.full(n, 0xAA, np.uint8) if decoded_init is None else np.array(decoded_init, np.uint8)
@@ -209,12 +213,14 @@ def ref_enc():
 
 
 # ------------------------------------------------------------ whole frames
-def decode_wz_frame(past, fut, orig, wz, qindex, mode=0, nthreads=1, max_iter=50, idx=(0, 1, 2), rate_step=0):
-    """Oracle decodeGOP body for one WZ frame.  Returns (recons, result dict)."""
+def decode_wz_frame(past, fut, orig, wz, qindex, mode=0, nthreads=1, max_iter=50, idx=(0, 1, 2), rate_step=0,
+                    kind="reg"):
+    """Oracle decodeGOP body for one WZ frame (kind: the QCIF ladder).
+    Returns (recons, result dict)."""
     h, w = past.shape
     g = geom(w, h)
     n = (h // 4) * (w // 4)
-    L = ladder(n)
+    L = ladder(n, kind)
     past = np.ascontiguousarray(past, np.uint8)
     fut = np.ascontiguousarray(fut, np.uint8)
     orig = np.ascontiguousarray(orig, np.uint8)
@@ -257,18 +263,30 @@ class RefDecoder:
     for one frame geometry.  The reference keeps global state, so one geometry
     per process: call setup() again to switch."""
 
-    def __init__(self, w, h, qindex, mode=0, nthreads=1, nframes=3, tmpdir="/tmp", dynstep=False):
-        self.lib = ref_dec(dynstep)
+    def __init__(self, w, h, qindex, mode=0, nthreads=1, nframes=3, tmpdir="/tmp", dynstep=False, lib=None,
+                 kind="reg"):
+        """lib: another build of the reference decoder with the same harness
+        entry points (e.g. libdvcref_accel.so); its ladders are then written
+        from the oracle's (pinned equal to the reference's files), so nothing
+        under /root/reference is read.  kind "irreg": the QCIF decoder reads
+        the reference's irReg1584.lad (main.cpp:84-85) with the inverse the
+        oracle computes from irRegEncoder1584.lad (the reference ships none)."""
+        self.lib = lib if lib is not None else ref_dec(dynstep)
         self.w, self.h, self.q = w, h, qindex
         self.lib.ref_setup(w, h, qindex, mode, nthreads, nframes)
         n = (w // 4) * (h // 4)
-        if n == 1584:
+        if n == 1584 and kind == "irreg":
+            lad = REFERENCE / "release/DvcDec/LDPCA_LadderFile/irReg1584.lad"
+            inv = Path(tmpdir) / "Inverse_Matrix_H_irReg1584.bin"
+            if not inv.exists():
+                inv.write_bytes(ladder(1584, "irreg").inv().tobytes())
+        elif n == 1584 and lib is None:
             lad = REFERENCE / "release/DvcDec/LDPCA_LadderFile/RegDecoder1584.lad"
             inv = REFERENCE / "release/DvcDec/LDPCA_LadderFile/Inverse_Matrix_H_Reg1584.bin"
         else:
-            lad, inv = Path(tmpdir) / "RegDecoder6336.lad", Path(tmpdir) / "Inverse_Matrix_H_Reg6336.bin"
+            lad, inv = Path(tmpdir) / f"RegDecoder{n}.lad", Path(tmpdir) / f"Inverse_Matrix_H_Reg{n}.bin"
             if not (lad.exists() and inv.exists()):
-                write_text_ladder(ladder(6336), lad, inv)
+                write_text_ladder(ladder(n), lad, inv)
         self.lib.ref_load_ladder(str(lad).encode(), str(inv).encode())
 
     def decode_wz_frame(self, past, fut, orig, wz):
@@ -285,3 +303,45 @@ class RefDecoder:
         self.lib.ref_get_frame(1, P(rec))
         return rec, dict(levels=levels[:n].tolist(), rates=rates[:n].tolist(), entropy=ent[:n].tolist(),
                          psnr=self.lib.ref_get_psnr(1), bitrate=self.lib.ref_get_bitrate(1))
+
+    def decode_gop(self, video, wz, begin, end):
+        """decodeGOP(begin, end) (DVC_Decoder.cpp:26-474) over frames
+        begin..end of `video`: the two key frames are the originals, every
+        frame in between is a WZ frame (wz[cur]) decoded in the reference's
+        hierarchical order, decoded frames serving as references of the
+        later ones (:471-472).  The setup's nframes must exceed `end`.
+        Returns {cur: (recons, result dict)} and the decode order."""
+        h, w = video[0].shape
+        order = []
+        _gop_order(begin, end, order)
+        for k in range(begin, end + 1):
+            pix = video[k] if k in (begin, end) else np.zeros((h, w), np.uint8)
+            self.lib.ref_set_frame(k, P(np.ascontiguousarray(pix)), P(np.ascontiguousarray(video[k])))
+        for cur, _, _ in order:
+            f = wz[cur]
+            self.lib.ref_set_encoded(cur, P(np.ascontiguousarray(f.ac_range, np.int32)),
+                                     P(np.ascontiguousarray(f.crc, np.uint8)), P(np.ascontiguousarray(f.acc, np.uint8)))
+        n = self.lib.ref_decode_gop(begin, end)
+        rates = np.zeros(4096, np.int32); levels = np.zeros(4096, np.int32); ent = np.zeros(4096, np.float32)
+        self.lib.ref_get_log(P(rates), P(levels), P(ent))
+        out, at = {}, 0
+        for cur, _, _ in order:
+            nb = len(wz[cur].crc)
+            rec = np.zeros((h, w), np.uint8)
+            self.lib.ref_get_frame(cur, P(rec))
+            out[cur] = (rec, dict(levels=levels[at:at + nb].tolist(), rates=rates[at:at + nb].tolist(),
+                                  entropy=ent[at:at + nb].tolist(), psnr=self.lib.ref_get_psnr(cur),
+                                  bitrate=self.lib.ref_get_bitrate(cur)))
+            at += nb
+        assert at == n, (at, n)
+        return out, order
+
+
+def _gop_order(begin, end, out):
+    """The WZ frames of decodeGOP(begin, end) in its recursion order
+    (DVC_Decoder.cpp:27-28, 471-472) as (cur, past, fut)."""
+    mid = (begin + end) // 2
+    if mid > begin:
+        out.append((mid, begin, end))
+        _gop_order(begin, mid, out)
+        _gop_order(mid, end, out)

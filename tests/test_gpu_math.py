@@ -1,7 +1,7 @@
 """The device's scalar routines against the host C library, on the GPU:
 the glibc-identical float functions on the BP/CNM domains, double sqrt on
-every integer the motion costs can take, and the double exp used by the
-reconstruction (whose few-ulp differences stay inside the rounding margin)."""
+every integer the motion costs can take, and the double exp reconstruction
+uses (dvc_exp), bit for bit."""
 import ctypes
 
 import numpy as np
@@ -58,13 +58,24 @@ def test_double_sqrt_exact_on_integer_costs(ctx):
     assert np.array_equal(dev, np.sqrt(n.astype(np.float64)))
 
 
-def test_double_exp_within_margin(ctx):
+def test_double_exp_equals_host_libm(ctx):
+    """dvc_exp on the device (reconstruction's exp) against the host C
+    library's exp, bit for bit: uniform arguments over [-700, 700] and
+    [-20, 20], and reconstruction's own forms +-alpha*k (alpha a float,
+    k an integer step or distance)."""
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.exp.restype = ctypes.c_double
+    libm.exp.argtypes = [ctypes.c_double]
     rng = np.random.default_rng(5)
-    x = np.concatenate([-rng.uniform(0, 200, 100000), rng.uniform(0, 200, 100000)])
+    alpha = rng.uniform(1e-4, 64.0, 100000).astype(np.float32).astype(np.float64)
+    k = rng.integers(0, 4097, 100000).astype(np.float64)
+    x = np.concatenate([rng.uniform(-700, 700, 100000), rng.uniform(-20, 20, 100000), alpha * k, -alpha * k,
+                        np.array([0.0, -0.0, 1e-300, 512.0, -512.0, 709.7, -745.0, 1023.0, -1075.0])])
     dev = ctx.selftest(5, x)
-    ref = np.exp(x)
-    rel = np.abs(dev - ref) / ref
-    assert rel.max() < 1e-15
+    ref = np.array([libm.exp(float(v)) for v in x])
+    bad = np.flatnonzero(dev.view(np.uint64) != ref.view(np.uint64))
+    assert bad.size == 0, [(x[i], dev[i], ref[i]) for i in bad[:5]]
 
 
 def test_bp_phi_kernel_form_equals_restatement_on_every_argument(ctx):
