@@ -107,6 +107,7 @@ SIGNATURES = {
     "dvc_ldpca_decode": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_uint8,
                                     _c.c_int, _c.c_void_p, _c.POINTER(BpStats)]),
     "dvc_ldpca_decode_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(LdpcaItem)]),
+    "dvc_ldpca_decode_batch_noalloc": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(LdpcaItem)]),
     "dvc_selftest_unary": (_c.c_int, [_c.c_void_p, _c.c_int, _c.c_void_p, _c.c_void_p, _c.c_int]),
     "dvc_autotune": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(Tuning)]),
     "dvc_get_tuning": (_c.c_int, [_c.c_void_p, _c.POINTER(Tuning)]),

@@ -125,6 +125,10 @@ int dvc_ctx_create(const dvc_params *params, dvc_ctx **out) {
   }
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) { dvc_ctx_destroy(c); set_error("hipStreamCreate failed"); return DVC_EDEVICE; }
+  if ((rc = noalloc_attach(c))) {
+    dvc_ctx_destroy(c);
+    return rc;
+  }
   *out = c;
   return DVC_OK;
 }

@@ -84,6 +84,9 @@ int ensure_cost_tables(dvc_ctx *c);
 // Per-band entry points' device state (band.hip), freed with the context.
 struct BandState;
 void band_free(dvc_ctx *c);
+// Looks up the allocation-free entry point's scratch array on the context's
+// (current) device, once per device (noalloc.hip).
+int noalloc_attach(dvc_ctx *c);
 void free_ladder(Ladder &L);
 
 }  // namespace dvc
@@ -128,6 +131,7 @@ struct dvc_ctx {
   size_t ld_dev_bytes = 0;
   void *ld_host = nullptr;     // its pinned host staging (inputs up, results back)
   size_t ld_host_bytes = 0;
+  void *noalloc_scratch = nullptr;  // device address of noalloc.hip's scratch array on this device
   dvc::BandState *band = nullptr;   // per-band entry points' frame state (dvc_band_frame_init)
   dvc::Ladder *ladder(int N) { return N == 1584 ? &lad1584 : (N == 6336 ? &lad6336 : nullptr); }
 };

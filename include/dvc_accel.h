@@ -126,6 +126,19 @@ typedef struct {
  * sequential search selects.  Host pointers. */
 int dvc_ldpca_decode_batch(dvc_ctx *ctx, int n, dvc_ldpca_item *items);
 
+/* Allocation-free form of dvc_ldpca_decode_batch (no counterpart in the
+ * reference; for callers that cannot allocate: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same items and
+ * results as dvc_ldpca_decode_batch, except that LR, acc and decoded are
+ * device (HBM) pointers.  The call allocates nothing: its working memory is a
+ * fixed device array of DVC_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; a batch that needs more (about 150 bitplanes of N = 6336 or
+ * 600 of N = 1584 per code length) is rejected with DVC_EINVAL.  Calls that
+ * overlap on one GPU share that array and race; the call synchronises the
+ * context's stream before it returns.  parallel_mode 0 contexts only. */
+#define DVC_NOALLOC_SCRATCH_BYTES (64u << 20)
+int dvc_ldpca_decode_batch_noalloc(dvc_ctx *ctx, int n, dvc_ldpca_item *items);
+
 /* BP launch configuration (no counterpart in the reference; the project's
  * autotuner).  dvc_autotune decodes a synthetic calibration batch of
  * `planes` bitplanes per code length (0: 128) with each variant, checks that
