@@ -69,7 +69,8 @@ class WZFrameSpec(ctypes.Structure):
                 ("past_idx", ctypes.c_int), ("cur_idx", ctypes.c_int), ("fut_idx", ctypes.c_int),
                 ("past", ctypes.c_void_p), ("fut", ctypes.c_void_p), ("orig", ctypes.c_void_p),
                 ("ac_range", ctypes.c_void_p), ("crc", ctypes.c_void_p), ("acc", ctypes.c_void_p),
-                ("recons", ctypes.c_void_p), ("stats", ctypes.POINTER(FrameStats))]
+                ("recons", ctypes.c_void_p), ("stats", ctypes.POINTER(FrameStats)),
+                ("block_mode", ctypes.c_void_p), ("mode_bits", ctypes.c_int)]
 
 
 class Tuning(ctypes.Structure):
@@ -95,6 +96,22 @@ class EngineParams(ctypes.Structure):
 
 # Every symbol include/dvc_accel.h declares, with its ctypes signature.
 _c = ctypes
+class WzStreamInfo(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("qindex", ctypes.c_int), ("intra_qp", ctypes.c_int),
+                ("gop", ctypes.c_int), ("frames", ctypes.c_int), ("deblocking", ctypes.c_int),
+                ("wz_frames", ctypes.c_int), ("nbitplanes", ctypes.c_int), ("bytes_read", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class WzStreamFrame(ctypes.Structure):
+    _fields_ = [("past_idx", ctypes.c_int), ("cur_idx", ctypes.c_int), ("fut_idx", ctypes.c_int),
+                ("ac_range", ctypes.c_int * 16), ("skip_mode", ctypes.c_int), ("intra_mode", ctypes.c_int),
+                ("mode_bits", ctypes.c_int), ("block_mode", ctypes.c_void_p), ("crc", ctypes.c_void_p),
+                ("acc", ctypes.c_void_p)]
+
+
 SIGNATURES = {
     "dvc_ctx_create": (_c.c_int, [_c.POINTER(Params), _c.POINTER(_c.c_void_p)]),
     "dvc_ctx_destroy": (_c.c_int, [_c.c_void_p]),
@@ -117,6 +134,14 @@ SIGNATURES = {
                                           _c.POINTER(_c.c_double)]),
     "dvc_decode_wz_frames": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WZFrameSpec), _c.c_int]),
     "dvc_side_info_batch": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(SIRequest)]),
+    "dvc_wz_stream_open": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_void_p)]),
+    "dvc_wz_stream_info_get": (_c.c_int, [_c.c_void_p, _c.POINTER(WzStreamInfo)]),
+    "dvc_wz_stream_frame_get": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WzStreamFrame)]),
+    "dvc_wz_stream_close": (_c.c_int, [_c.c_void_p]),
+    "dvc_jbig_decode_map": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.c_int, _c.c_int, _c.c_void_p,
+                                       _c.POINTER(_c.c_size_t)]),
+    "dvc_jbig_decode": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_uint32),
+                                   _c.POINTER(_c.c_uint32), _c.POINTER(_c.c_size_t)]),
     "dvc_buffer_alloc": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_void_p)]),
     "dvc_buffer_free": (_c.c_int, [_c.c_void_p, _c.c_void_p]),
     "dvc_buffer_copy": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_void_p, _c.c_size_t, _c.c_int]),
@@ -315,13 +340,15 @@ class Context:
             crc = np.ascontiguousarray(wz.crc, np.uint8)
             acc = np.ascontiguousarray(wz.acc, np.uint8)
             rec = np.zeros_like(past)
-            keep += [past, fut, orig, acr, crc, acc]
+            mode = _block_mode(wz)
+            keep += [past, fut, orig, acr, crc, acc, mode]
             outs.append(rec)
             pi, ci, fi = fr.get("idx", (0, 1, 2))
             h, w = past.shape
             specs[i] = WZFrameSpec(w, h, wz.qindex, pi, ci, fi, past.ctypes.data, fut.ctypes.data,
                                    None if orig is None else orig.ctypes.data, acr.ctypes.data, crc.ctypes.data,
-                                   acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats[i]))
+                                   acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats[i]),
+                                   None if mode is None else mode.ctypes.data, int(getattr(wz, "mode_bits", 0)))
         _check(lib.dvc_decode_wz_frames(self.h, k, specs, 1 if batched else 0), "dvc_decode_wz_frames")
         return [(outs[i], stats[i].as_dict()) for i in range(k)]
 
@@ -446,10 +473,17 @@ def _frame_spec(fr, stats):
     rec = np.zeros_like(past)
     pi, ci, fi = fr.get("idx", (0, 1, 2))
     h, w = past.shape
+    mode = _block_mode(wz)
     spec = WZFrameSpec(w, h, wz.qindex, pi, ci, fi, past.ctypes.data, fut.ctypes.data,
                        None if orig is None else orig.ctypes.data, acr.ctypes.data, crc.ctypes.data,
-                       acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats))
-    return spec, [past, fut, orig, acr, crc, acc, rec], rec
+                       acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats),
+                       None if mode is None else mode.ctypes.data, int(getattr(wz, "mode_bits", 0)))
+    return spec, [past, fut, orig, acr, crc, acc, rec, mode], rec
+
+
+def _block_mode(wz):
+    m = getattr(wz, "block_mode", None)
+    return None if m is None else np.ascontiguousarray(m, np.uint8)
 
 
 class Engine:
@@ -483,11 +517,13 @@ class Engine:
         stats = FrameStats()
         acr = np.ascontiguousarray(wz.ac_range, np.int32)
         crc = np.ascontiguousarray(wz.crc, np.uint8)
+        mode = _block_mode(wz)
         spec = WZFrameSpec(width, height, wz.qindex, idx[0], idx[1], idx[2], past_ptr, fut_ptr, None,
-                           acr.ctypes.data, crc.ctypes.data, acc_ptr, recons_ptr, ctypes.pointer(stats))
+                           acr.ctypes.data, crc.ctypes.data, acc_ptr, recons_ptr, ctypes.pointer(stats),
+                           None if mode is None else mode.ctypes.data, int(getattr(wz, "mode_bits", 0)))
         t = ctypes.c_uint64()
         _check(lib.dvc_engine_submit(self.h, ctypes.byref(spec), past_key, fut_key, ctypes.byref(t)), "dvc_engine_submit")
-        self.live[t.value] = (spec, [acr, crc], None, stats)
+        self.live[t.value] = (spec, [acr, crc, mode], None, stats)
         return t.value
 
     def step(self):
@@ -534,3 +570,53 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+def jbig_decode_map(data, width, height):
+    """One block-mode map's JBIG data -> (uint8[height, width] of 0/1, bytes consumed)."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    out = np.zeros((height, width), np.uint8)
+    used = ctypes.c_size_t()
+    _check(lib.dvc_jbig_decode_map(buf.ctypes.data, len(buf), width, height, out.ctypes.data, ctypes.byref(used)),
+           "dvc_jbig_decode_map")
+    return out, used.value
+
+
+def jbig_decode(bie):
+    """A JBIG BIE (header + data) -> (uint8[yd, xd] image, bytes consumed)."""
+    buf = np.frombuffer(bytes(bie), np.uint8)
+    w, h, used = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_size_t()
+    _check(lib.dvc_jbig_decode(buf.ctypes.data, len(buf), None, 0, ctypes.byref(w), ctypes.byref(h), ctypes.byref(used)),
+           "dvc_jbig_decode")
+    out = np.zeros((h.value, w.value), np.uint8)
+    _check(lib.dvc_jbig_decode(buf.ctypes.data, len(buf), out.ctypes.data, out.size, ctypes.byref(w), ctypes.byref(h),
+                               ctypes.byref(used)), "dvc_jbig_decode")
+    return out, used.value
+
+
+def read_wz_stream(data):
+    """Parses a WZ bitstream (dvc_wz_stream_*): (info dict, list of per-WZ-frame dicts in decoding order with
+    past/cur/fut indices and a workload.WZFrame carrying ac_range, crc, acc, block_mode and mode_bits)."""
+    from .workload import WZFrame
+    buf = np.frombuffer(bytes(data), np.uint8)
+    h = ctypes.c_void_p()
+    _check(lib.dvc_wz_stream_open(buf.ctypes.data, len(buf), ctypes.byref(h)), "dvc_wz_stream_open")
+    try:
+        info = WzStreamInfo()
+        _check(lib.dvc_wz_stream_info_get(h, ctypes.byref(info)), "dvc_wz_stream_info_get")
+        n8 = (info.width // 8) * (info.height // 8)
+        synd = (info.width // 4) * (info.height // 4) // 8
+        frames = []
+        for k in range(info.wz_frames):
+            f = WzStreamFrame()
+            _check(lib.dvc_wz_stream_frame_get(h, k, ctypes.byref(f)), "dvc_wz_stream_frame_get")
+            mode = np.ctypeslib.as_array((ctypes.c_uint8 * n8).from_address(f.block_mode)).copy()
+            crc = np.ctypeslib.as_array((ctypes.c_uint8 * info.nbitplanes).from_address(f.crc)).copy()
+            acc = np.ctypeslib.as_array((ctypes.c_uint8 * (info.nbitplanes * synd)).from_address(f.acc)).copy()
+            wz = WZFrame(info.qindex, np.array(list(f.ac_range), np.int32), crc, acc.reshape(info.nbitplanes, synd),
+                         None, mode if mode.any() else None, f.mode_bits)
+            frames.append(dict(idx=(f.past_idx, f.cur_idx, f.fut_idx), skip_mode=f.skip_mode, intra_mode=f.intra_mode,
+                               wz=wz))
+        return info.as_dict(), frames
+    finally:
+        lib.dvc_wz_stream_close(h)

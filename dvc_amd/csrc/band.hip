@@ -106,7 +106,8 @@ extern "C" int dvc_band_frame_init(dvc_ctx *c, int W, int H, int qindex, const u
   auto take = [&off](size_t b) { const size_t o = off; off = al(off + b); return o; };
   const size_t oPast = take(FS), oFut = take(FS), oPlp = take(FS), oFlp = take(FS), oPh = take(4 * FS),
                oFh = take(4 * FS), oSi = take(FS), oRec = take(FS), oRt = take(4 * FS), oAl = take(4 * FS),
-               oOct = take((size_t)64 * (W - 3) * (H - 3)), oSmf = take((size_t)4 * 81 * N),
+               oOct = take((size_t)64 * (W - 3) * (H - 3)),
+               oOb = take((size_t)4 * (W - 3) * (H - 3)), oSmf = take((size_t)4 * 81 * N),
                oSmfRef = take(sizeof(dvc_smf) * (size_t)N), oLR = take(4 * (size_t)N), oFd = take(sizeof(FrameDev)),
                oList = take(sizeof(int));
   if ((rc = grow(&B->mem, &B->bytes, off))) return rc;
@@ -119,7 +120,7 @@ extern "C" int dvc_band_frame_init(dvc_ctx *c, int W, int H, int qindex, const u
   f.past = m + oPast; f.fut = m + oFut; f.past_lp = m + oPlp; f.fut_lp = m + oFlp;
   f.past_half = m + oPh; f.fut_half = m + oFh;
   f.si = m + oSi; f.recons = m + oRec; f.reconsTrans = (int *)(m + oRt); f.alpha = (float *)(m + oAl);
-  f.oct = (int *)(m + oOct); f.smf = (float *)(m + oSmf); f.LR = (float *)(m + oLR);
+  f.oct = (int *)(m + oOct); f.octBand = (int *)(m + oOb); f.smf = (float *)(m + oSmf); f.LR = (float *)(m + oLR);
   B->smfRef = (dvc_smf *)(m + oSmfRef);
   B->d_fd = (FrameDev *)(m + oFd);
   B->d_list = (int *)(m + oList);
@@ -158,6 +159,7 @@ extern "C" int dvc_noise_distribution(dvc_ctx *c, int range, int index, const fl
   DVC_HIP(hipGetLastError());
   f.index = index; f.range = range; f.distStride = stride; f.dist = B->dist;
   if ((rc = upload_fd(c, B))) return rc;
+  DVC_HIP(launch_oct_plane(B->d_fd, B->d_list, 1, (W - 3) * (H - 3), index, s));
   DVC_HIP(launch_noise_dist(B->d_fd, B->d_list, 1, N, stride, s));
   if (dist) DVC_HIP(hipMemcpyAsync(dist, B->dist, need, hipMemcpyDeviceToHost, s));
   DVC_HIP(hipStreamSynchronize(s));

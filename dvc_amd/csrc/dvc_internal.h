@@ -79,6 +79,9 @@ int bp_grid(const dvc_ctx *c, int N, int items);
 int upload_ladder(Ladder &L);
 // Quantisation table entry Qtable[q][index] (reconstruction.cpp:9-78), 0 outside 1..8 x 0..15.
 int qtable_entry(int q, int index);
+// QuantizationTableBits[q][index] (reconstruction.cpp:44-53) and zigzag[band] (global.cpp:36-41).
+int qbits_entry(int q, int index);
+int zigzag_entry(int band);
 // Motion-cost factor tables of the context (made on first use).
 int ensure_cost_tables(dvc_ctx *c);
 // Per-band entry points' device state (band.hip), freed with the context.

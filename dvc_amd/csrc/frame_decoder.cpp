@@ -54,6 +54,8 @@ static const int kQBits[9][16] = {
 static const int kZigzag[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
 
 int qtable_entry(int q, int index) { return (q >= 1 && q <= 8 && index >= 0 && index < 16) ? kQTable[q][index] : 0; }
+int qbits_entry(int q, int index) { return (q >= 1 && q <= 8 && index >= 0 && index < 16) ? kQBits[q][index] : 0; }
+int zigzag_entry(int band) { return (band >= 0 && band < 16) ? kZigzag[band] : 0; }
 
 #define ROUNDD(x) ((x) >= 0.0 ? (int)((x) + 0.5) : -(int)(-(x) + 0.5))
 
@@ -100,6 +102,7 @@ struct Job {
   dvc_wz_frame spec{};           // caller's request; pointers must stay valid until completion
   std::vector<int> acRange;      // owned copies of the bitstream fields
   std::vector<uint8_t> crc, acc;
+  std::vector<uint8_t> mode;     // 8x8 block modes when the frame has skip blocks, else empty
   uint64_t ticket = 0, pastKey = 0, futKey = 0;
   bool setup = false;
   size_t bytes = 0;
@@ -128,12 +131,12 @@ static size_t job_bytes(const Job &j) {
   for (auto &b : j.bands) maxStride = std::max(maxStride, b.stride);
   size_t t = 0;
   for (size_t b : {(size_t)FS, 4 * (size_t)FS, 4 * (size_t)FS, 4 * (size_t)FS, 4 * (size_t)FS,
-                   (size_t)64 * (W - 3) * (H - 3), (size_t)4 * 81 * N, (size_t)4 * N * std::max(maxStride, 1),
+                   (size_t)64 * (W - 3) * (H - 3), (size_t)4 * (W - 3) * (H - 3), (size_t)4 * 81 * N, (size_t)4 * N * std::max(maxStride, 1),
                    4 * (size_t)N, (size_t)N, (size_t)FS, sizeof(MVd) * (size_t)(FS / 256), sizeof(MVd) * (size_t)(FS / 64),
                    sizeof(MVd) * (size_t)(FS / 64), sizeof(MVd) * (size_t)(FS / 256), 16 * (size_t)(FS / 256),
                    8 * (size_t)(FS / 64), (size_t)j.nbp * (N / 8),
                    sizeof(PlaneState) * (size_t)j.nbp, sizeof(int4) * kMaxCodes * (size_t)j.nbp, 4 * (size_t)N,
-                   (size_t)N, 4 * kMaxCodes * (size_t)j.nbp})
+                   (size_t)N, 4 * kMaxCodes * (size_t)j.nbp, (size_t)FS / 64})
     t = align_up(t + b);
   return t;
 }
@@ -146,19 +149,21 @@ static void carve_job(Job &j, uint8_t *base) {
   size_t off = 0;
   auto take = [&](size_t bytes) { size_t o = off; off = align_up(off + bytes); return o; };
   const size_t o_si = take(FS), o_res = take(4 * (size_t)FS), o_st = take(4 * (size_t)FS), o_rt = take(4 * (size_t)FS),
-               o_al = take(4 * (size_t)FS), o_oct = take((size_t)64 * (W - 3) * (H - 3)), o_smf = take((size_t)4 * 81 * N),
+               o_al = take(4 * (size_t)FS), o_oct = take((size_t)64 * (W - 3) * (H - 3)),
+               o_ob = take((size_t)4 * (W - 3) * (H - 3)), o_smf = take((size_t)4 * 81 * N),
                o_dist = take((size_t)4 * N * std::max(maxStride, 1)), o_lr = take(4 * (size_t)N), o_dec = take(N),
                o_rec = take(FS), o_mb = take(sizeof(MVd) * (size_t)(FS / 256)), o_bmv = take(sizeof(MVd) * (size_t)(FS / 64)),
                o_bmvt = take(sizeof(MVd) * (size_t)(FS / 64)), o_fwd = take(sizeof(MVd) * (size_t)(FS / 256)),
                o_holes = take(16 * (size_t)(FS / 256)), o_mse = take(8 * (size_t)(FS / 64)),
                o_acc = take((size_t)j.nbp * (N / 8)), o_state = take(sizeof(PlaneState) * (size_t)j.nbp),
                o_recd = take(sizeof(int4) * kMaxCodes * (size_t)j.nbp), o_llr = take(4 * (size_t)N),
-               o_carry = take(N), o_bcase = take(4 * kMaxCodes * (size_t)j.nbp);
+               o_carry = take(N), o_bcase = take(4 * kMaxCodes * (size_t)j.nbp), o_mode = take((size_t)FS / 64);
   j.mem = base;
   j.bytes = off;
   FrameDev &f = j.fd;
   f.si = base + o_si; f.residue = (int *)(base + o_res); f.sideTrans = (int *)(base + o_st);
   f.reconsTrans = (int *)(base + o_rt); f.alpha = (float *)(base + o_al); f.oct = (int *)(base + o_oct);
+  f.octBand = (int *)(base + o_ob);
   f.smf = (float *)(base + o_smf); f.dist = (float *)(base + o_dist); f.LR = (float *)(base + o_lr);
   f.decoded = base + o_dec; f.recons = base + o_rec; f.mbmv = (MVd *)(base + o_mb); f.bmv = (MVd *)(base + o_bmv);
   f.bmv_tmp = (MVd *)(base + o_bmvt); f.fwd = (MVd *)(base + o_fwd); f.holes = (double *)(base + o_holes);
@@ -167,6 +172,7 @@ static void carve_job(Job &j, uint8_t *base) {
   j.d_LLR = (float *)(base + o_llr);
   j.d_carry = base + o_carry;
   j.d_bcase = (int *)(base + o_bcase);
+  f.mode = j.mode.empty() ? nullptr : base + o_mode;
 }
 
 }  // namespace dvc
@@ -491,6 +497,7 @@ static int admit(dvc_engine *e, Job *j, hipStream_t s, std::vector<JobInitDev> &
   f.past = kp.full; f.past_lp = kp.lp; f.past_half = kp.half;
   f.fut = kf.full; f.fut_lp = kf.lp; f.fut_half = kf.half;
   if (!e->deviceBuffers) DVC_HIP(hipMemcpyAsync(j->d_acc, j->acc.data(), j->acc.size(), hipMemcpyHostToDevice, s));
+  if (f.mode) DVC_HIP(hipMemcpyAsync((void *)f.mode, j->mode.data(), j->mode.size(), hipMemcpyHostToDevice, s));
   // k_job_init: no decoded bits; per-bitplane rate-search state with the
   // start level from the entropy (computed on the device), no accepted code
   // and no record of any code run; ParallelMode 1's carried decisions 0xAA
@@ -756,6 +763,11 @@ static int lane_step(dvc_engine *e, Lane &ln) {
   // gathered on the device and read back with one copy, then outputs
   if (!fin.empty()) {
     TIMED_LAUNCH(c, "inverse_transform", s, launch_inverse_transform(ln.d_frames, L4, (int)fin.size(), maxN, s));
+    // skip blocks take the past key frame's pixels (DVC_Decoder.cpp:155-157);
+    // no kernel of the band loop reads or writes them, so they are copied here
+    bool anySkip = false;
+    for (int i : fin) anySkip |= !ln.inflight[i]->mode.empty();
+    if (anySkip) DVC_HIP(launch_skip_copy(ln.d_frames, L4, (int)fin.size(), maxN / 4, s));
     DVC_HIP(launch_copy_segments(ln.d_segs, nseg, s));
     DVC_HIP(hipMemcpyAsync(ln.h_out, ln.d_out, outUsed, hipMemcpyDeviceToHost, s));
     DVC_HIP(hipStreamSynchronize(s));
@@ -866,6 +878,21 @@ static int engine_submit(dvc_engine *e, const dvc_wz_frame *sp, uint64_t pastKey
   j->W = sp->width; j->H = sp->height; j->N = j->W * j->H / 16;
   j->acRange.assign(sp->ac_range, sp->ac_range + 16);
   j->bands = band_plan(sp->qindex, j->acRange.data(), &j->bitrate0);
+  if (sp->block_mode) {
+    const int n8 = j->W * j->H / 64;
+    bool anyNonWz = false;
+    for (int k = 0; k < n8; k++) {
+      DVC_CHECK(sp->block_mode[k] == 0 || sp->block_mode[k] == 2, DVC_EINVAL,
+                sp->block_mode[k] == 1 ? "WZ frame: intra blocks are not supported (they need the intra bitstream)"
+                                       : "WZ frame: block modes must be 0 (WZ) or 2 (skip)");
+      anyNonWz |= sp->block_mode[k] != 0;
+    }
+    if (anyNonWz) j->mode.assign(sp->block_mode, sp->block_mode + n8);
+  }
+  if (sp->block_mode || sp->mode_bits) {
+    DVC_CHECK(sp->mode_bits >= 2, DVC_EINVAL, "WZ frame: mode_bits counts at least the two mode flags");
+    j->bitrate0 += sp->mode_bits - 2;  // band_plan counted one bit per mode flag
+  }
   for (auto &b : j->bands) j->nbp += b.qbits;
   j->crc.assign(sp->crc, sp->crc + j->nbp);
   if (!e->deviceBuffers) j->acc.assign(sp->acc, sp->acc + (size_t)j->nbp * (j->N / 8));

@@ -18,6 +18,16 @@
 
 namespace dvc {
 
+// BlockModeFlag value of a WZ block (defines.h:15)
+constexpr uint8_t kWzBlock = 0, kSkipBlock = 2;
+// Whether 4x4 block b of f is a WZ block (its 8x8 block's mode, as the
+// reference's blk88row + (j >> 1) lookup reads it).
+__device__ __forceinline__ bool wz_block(const FrameDev &f, int b) {
+  if (!f.mode) return true;
+  const int bw = f.W >> 2, i = b / bw, j = b - i * bw;
+  return f.mode[(i >> 1) * (f.W >> 3) + (j >> 1)] == kWzBlock;
+}
+
 #define DVC_K 0.05f  // defines.h:10, promoted to double where it is used
 
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
@@ -462,6 +472,10 @@ __global__ __launch_bounds__(1024) void k_noise_param(const FrameDev *F, const i
 // ------------------------------------------------------------ over-complete transform
 // overCompleteTransform (transform.cpp:275-298); with initSideTrans, also
 // init_si_ML (motionLearning.cpp:49-68) from the block-aligned positions.
+// Each position's coefficient of the band whose noise distribution comes next
+// (band 0 at the frame's start, else band + 1) also goes to the octBand plane,
+// so that kernel reads 4 contiguous bytes per window position instead of one
+// int out of each 64-byte record.
 __global__ __launch_bounds__(256) void k_oct(const FrameDev *F, const int *list, int initSideTrans) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int ow = f.W - 3, npos = ow * (f.H - 3);
@@ -479,9 +493,19 @@ __global__ __launch_bounds__(256) void k_oct(const FrameDev *F, const int *list,
   o[1] = make_int4(blk[4], blk[5], blk[6], blk[7]);
   o[2] = make_int4(blk[8], blk[9], blk[10], blk[11]);
   o[3] = make_int4(blk[12], blk[13], blk[14], blk[15]);
+  const int nb = initSideTrans ? 0 : f.band + 1;
+  if (nb < 16) f.octBand[p] = blk[c_zigzag[nb]];
   if (initSideTrans && (top & 3) == 0 && (left & 3) == 0 && top <= f.H - 4 && left <= f.W - 4)
     for (int m = 0; m < 4; m++)
       for (int n = 0; n < 4; n++) f.sideTrans[(top + m) * W + left + n] = blk[4 * m + n];
+}
+
+// octBand from a complete over-complete transform (coefficient `index`).
+__global__ __launch_bounds__(256) void k_oct_plane(const FrameDev *F, const int *list, int index) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  const int npos = (f.W - 3) * (f.H - 3);
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < npos) f.octBand[p] = f.oct[(size_t)16 * p + index];
 }
 
 // SMF window of WZ block b (initSMF, motionLearning.cpp:6-47).
@@ -521,14 +545,14 @@ __global__ void k_smf_init(const FrameDev *F, const int *list) {
 __global__ __launch_bounds__(64) void k_noise_dist(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int b = blockIdx.x;
-  if (b >= f.N) return;
+  if (b >= f.N || !wz_block(f, b)) return;  // only SIR candidates have a distribution
   const int tid = threadIdx.x, T = blockDim.x;
   const int ow = f.W - 3, index = f.index, range = f.range;
   const SmfWin s = smf_window(f, b);
   const float a = f.alpha[(s.top + (index >> 2)) * f.W + s.left + (index & 3)];
   float *out = f.dist + (size_t)b * f.distStride;
   if (index == 0) {
-    const int side = f.oct[(size_t)16 * (s.top * ow + s.left)];
+    const int side = f.octBand[s.top * ow + s.left];
     for (int k = tid; k < range; k += T) out[k] = dvc_expf(-a * (float)abs(k - side));
     return;
   }
@@ -551,7 +575,7 @@ __global__ __launch_bounds__(64) void k_noise_dist(const FrameDev *F, const int 
       if (keep) {
         const int pos = n + __popcll(mask & ((1ull << tid) - 1ull));
         const int m = s.sT + p / s.w, q = s.sL + p % s.w;
-        const int sd = f.oct[(size_t)16 * (m * ow + q) + index];
+        const int sd = f.octBand[m * ow + q];
         s_ps[pos] = make_int2(__float_as_int(pr), sd);
         mx = imax(mx, abs(sd));
       }
@@ -624,8 +648,9 @@ __global__ __launch_bounds__(64) void k_condbit(const FrameDev *F, const int *li
   __shared__ int s_lo[64], s_hi[64];
   const int b = b0 + lane;
   const bool valid = b < f.N;
+  const bool wz = valid && wz_block(f, b);  // condBitProb walks the SIR's WZ blocks
   int v0l = 0, v0u = 0, v1l = 0, v1u = 0;
-  if (valid) {
+  if (wz) {
     const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4;
     const int index = f.index, bit = f.bit, step = f.step, range = f.range;
     const int iq = (1 << (bit - 1)) - 1;
@@ -692,6 +717,12 @@ __global__ __launch_bounds__(64) void k_condbit(const FrameDev *F, const int *li
     __syncthreads();
   }
   if (!valid) return;
+  if (!wz) {
+    // the frame's LR are first set to 0.9999f / 0.0001f when it has skip or
+    // intra blocks (DVC_Decoder.cpp:203-207); WZ blocks overwrite theirs
+    f.LR[b] = 0.9999f / 0.0001f;
+    return;
+  }
   p0 = p0 / (p0 + p1);
   p1 = 1 - p0;
   float lr;
@@ -716,6 +747,7 @@ __global__ void k_put_bitplane(const FrameDev *F, const int *list, const BpFrame
   const uint8_t bitv = lk.slots[(size_t)slot * f.N + b];
   f.decoded[b] = bitv;
   if (lk.carry) lk.carry[b] = bitv;  // ParallelMode 1: the next bitplane starts from these decisions
+  if (!wz_block(f, b)) return;      // putBitplaneToCoeff (bitplane.cpp:4-22) skips other blocks
   const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4, index = f.index;
   int *p = &f.reconsTrans[(top + (index >> 2)) * f.W + left + (index & 3)];
   *p = (*p << 1) + (int)bitv;
@@ -729,7 +761,7 @@ __global__ void k_put_bitplane(const FrameDev *F, const int *list, const BpFrame
 __global__ void k_reconstruct(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= f.N) return;
+  if (b >= f.N || !wz_block(f, b)) return;
   const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4, index = f.index, step = f.step;
   const int ci = (top + (index >> 2)) * f.W + left + (index & 3);
   const int side = f.sideTrans[ci];
@@ -763,7 +795,7 @@ __global__ void k_reconstruct(const FrameDev *F, const int *list) {
 __global__ void k_inverse_transform(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= f.N) return;
+  if (b >= f.N || !wz_block(f, b)) return;  // the SIR lists WZ blocks only
   const int bw = f.W / 4, top = (b / bw) * 4, left = (b % bw) * 4, W = f.W;
   int blk[16];
   for (int m = 0; m < 4; m++)
@@ -828,7 +860,7 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
   }
   __syncthreads();
   const int b = b0 + wave;
-  if (b >= f.N) return;
+  if (b >= f.N || !wz_block(f, b)) return;  // updateSI walks the SIR's WZ blocks
   const int top = row0, left = (b % bw) * 4;
   const int sT = imax(0, top - 32), sB = imin(H - 4, top + 32), sL = imax(0, left - 32), sR = imin(W - 4, left + 32);
   const int ww = sR - sL + 1, nr = sB - sT + 1;
@@ -925,7 +957,7 @@ __global__ __launch_bounds__(256) void k_update_smf(const FrameDev *F, const int
   const FrameDev &f = F[list[blockIdx.y]];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int b = blockIdx.x * 4 + wave;
-  if (b >= f.N) return;
+  if (b >= f.N || !wz_block(f, b)) return;  // updateSMF walks the SIR's WZ blocks
   const int W = f.W;
   const SmfWin s = smf_window(f, b);
   float blk[16];
@@ -973,14 +1005,23 @@ __global__ __launch_bounds__(256) void k_update_smf(const FrameDev *F, const int
 // a band not yet decoded): the 16 - (band + 1) lanes of a block read the same
 // 64-byte coefficient record per window position.  A window row's (at most 9)
 // weights and coefficients are loaded before they are added in order.
-__global__ __launch_bounds__(256) void k_update_si_ml(const FrameDev *F, const int *list, int) {
-  const FrameDev &f = F[list[blockIdx.y]];
+// Workgroups are dispatched to the 8 XCDs in turn, each XCD with its own L2;
+// the 1-D grid is renumbered so that XCD k runs the k-th eighth of the
+// (frame, row segment) list in order.  Vertically adjacent blocks, whose
+// windows share 5 of 9 rows, then run on one XCD close together in time and
+// find each other's records in its L2.
+__global__ __launch_bounds__(256) void k_update_si_ml(const FrameDev *F, const int *list, int perFrame, int total) {
+  const int per = gridDim.x >> 3;  // the grid is a multiple of 8
+  const int logical = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (logical >= total) return;
+  const int frame = logical / perFrame, seg = logical - frame * perFrame;
+  const FrameDev &f = F[list[frame]];
   const int bandStart = f.band + 1;  // update_si_ML(..., band+1, ...) (DVC_Decoder.cpp:369)
   const int cnt = 16 - bandStart;    // coefficients still to predict per block
   if (cnt <= 0) return;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = seg * blockDim.x + threadIdx.x;
   const int b = t / cnt;
-  if (b >= f.N) return;
+  if (b >= f.N || !wz_block(f, b)) return;  // update_si_ML walks the SIR's WZ blocks
   const int index = c_zigzag[bandStart + t % cnt];
   const int W = f.W, ow = W - 3;
   const SmfWin s = smf_window(f, b);
@@ -1032,6 +1073,27 @@ hipError_t launch_oct(const FrameDev *F, const int *list, int n, int maxPos, int
   hipLaunchKernelGGL(k_oct, dim3((maxPos + 255) / 256, n), dim3(256), 0, s, F, list, initSideTrans);
   return hipGetLastError();
 }
+// copySkipBlkToRecons (skipBlock.cpp:3-22): one thread per 8x8 block.
+__global__ void k_skip_copy(const FrameDev *F, const int *list) {
+  const FrameDev &f = F[list[blockIdx.y]];
+  if (!f.mode) return;
+  const int w8 = f.W >> 3, n8 = w8 * (f.H >> 3);
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n8 || f.mode[k] != kSkipBlock) return;
+  const int top = (k / w8) * 8, left = (k % w8) * 8;
+  for (int m = 0; m < 8; m++) {
+    const size_t o = (size_t)(top + m) * f.W + left;
+    for (int n = 0; n < 8; n++) f.recons[o + n] = f.past[o + n];  // caller buffers need not be aligned
+  }
+}
+hipError_t launch_skip_copy(const FrameDev *F, const int *list, int n, int maxBlk88, hipStream_t s) {
+  hipLaunchKernelGGL(k_skip_copy, dim3((maxBlk88 + 255) / 256, n), dim3(256), 0, s, F, list);
+  return hipGetLastError();
+}
+hipError_t launch_oct_plane(const FrameDev *F, const int *list, int n, int maxPos, int index, hipStream_t s) {
+  hipLaunchKernelGGL(k_oct_plane, dim3((maxPos + 255) / 256, n), dim3(256), 0, s, F, list, index);
+  return hipGetLastError();
+}
 hipError_t launch_noise_dist(const FrameDev *F, const int *list, int n, int maxN, int maxStride, hipStream_t s) {
   (void)maxStride;  // one wave per block whatever the band's value count
   hipLaunchKernelGGL(k_noise_dist, dim3(maxN, n), dim3(64), 0, s, F, list);
@@ -1067,7 +1129,9 @@ hipError_t launch_update_smf(const FrameDev *F, const int *list, int n, int maxN
   return hipGetLastError();
 }
 hipError_t launch_update_si_ml(const FrameDev *F, const int *list, int n, int maxN, int bandStart, hipStream_t s) {
-  hipLaunchKernelGGL(k_update_si_ml, dim3((16 * maxN + 255) / 256, n), dim3(256), 0, s, F, list, bandStart);
+  (void)bandStart;  // each frame's own band (FrameDev::band) sets its coefficient count
+  const int perFrame = (16 * maxN + 255) / 256, total = perFrame * n;
+  hipLaunchKernelGGL(k_update_si_ml, dim3((total + 7) / 8 * 8), dim3(256), 0, s, F, list, perFrame, total);
   return hipGetLastError();
 }
 

@@ -63,7 +63,8 @@ __global__ void k_phi_sweep(uint32_t lo, uint32_t hi, unsigned long long *counts
 // VALU issue-rate calibration: 8 independent chains per lane, 128 operations
 // of one kind per trip (kind 0: v_fma_f32, 1: v_add_f32, 2: v_mul_f32, 3:
 // v_fma_f64, 4: v_add_u32, 5: v_sad_u8 (4 absolute differences), 6:
-// v_mqsad_u32_u8 (16)), built with the library's flags (packed FP32 off).
+// v_mqsad_u32_u8 (16), 7: v_qsad_pk_u16_u8 (16), 8: v_rcp_f32, a
+// transcendental), built with the library's flags (packed FP32 off).
 // The loop's own counter is scalar, so a trip issues 128 VALU instructions
 // per wave.
 constexpr int VALU_CHAINS = 8, VALU_UNROLL = 16;
@@ -116,6 +117,36 @@ __global__ __launch_bounds__(256) void k_valu_rate(const float *in, float *out, 
 #pragma unroll
     for (int q = 0; q < VALU_CHAINS; q++) t ^= acc[q].x ^ acc[q].y ^ acc[q].z ^ acc[q].w;
     out[i] = __uint_as_float(t);
+  } else if constexpr (KIND == 7) {
+    uint64_t acc[VALU_CHAINS];
+    const uint64_t src = ((uint64_t)__float_as_uint(b) << 32) | __float_as_uint(in[i & 1023]);
+    const uint32_t k = __float_as_uint(a);
+#pragma unroll
+    for (int q = 0; q < VALU_CHAINS; q++) acc[q] = (uint64_t)q;
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+      for (int u = 0; u < VALU_UNROLL; u++)
+#pragma unroll
+        for (int q = 0; q < VALU_CHAINS; q++) acc[q] = __builtin_amdgcn_qsad_pk_u16_u8(src, k, acc[q]);
+    }
+    uint64_t t = 0;
+#pragma unroll
+    for (int q = 0; q < VALU_CHAINS; q++) t ^= acc[q];
+    out[i] = __uint_as_float((uint32_t)t ^ (uint32_t)(t >> 32));
+  } else if constexpr (KIND == 8) {
+    float x[VALU_CHAINS];
+#pragma unroll
+    for (int q = 0; q < VALU_CHAINS; q++) x[q] = in[i & 1023] + (float)q;
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+      for (int u = 0; u < VALU_UNROLL; u++)
+#pragma unroll
+        for (int q = 0; q < VALU_CHAINS; q++) x[q] = __builtin_amdgcn_rcpf(x[q]);
+    }
+    float t = 0.0f;
+#pragma unroll
+    for (int q = 0; q < VALU_CHAINS; q++) t += x[q];
+    out[i] = t;
   } else if constexpr (KIND == 4) {
     uint32_t x[VALU_CHAINS];
 #pragma unroll
@@ -155,12 +186,13 @@ __global__ __launch_bounds__(256) void k_valu_rate(const float *in, float *out, 
 
 // Verification entry (no counterpart in the reference): times one launch of
 // the VALU calibration kernel of `kind` (0 fp32 fma, 1 fp32 add, 2 fp32 mul,
-// 3 fp64 fma, 4 int32 add, 5 sad_u8, 6 mqsad_u32_u8) over `waves_per_cu` waves per CU and `iters`
+// 3 fp64 fma, 4 int32 add, 5 sad_u8, 6 mqsad_u32_u8, 7 qsad_pk_u16_u8, 8 rcp_f32)
+// over `waves_per_cu` waves per CU and `iters`
 // trips; writes the kernel time (HIP events on the context stream) and the
 // VALU wave-instructions it issues (128 per trip per wave).
 extern "C" int dvc_selftest_valu_rate(dvc_ctx *c, int kind, int waves_per_cu, int iters, double *ms,
                                       double *wave_instr) {
-  DVC_CHECK(c && ms && wave_instr && kind >= 0 && kind <= 6 && waves_per_cu >= 4 && waves_per_cu % 4 == 0 &&
+  DVC_CHECK(c && ms && wave_instr && kind >= 0 && kind <= 8 && waves_per_cu >= 4 && waves_per_cu % 4 == 0 &&
                 waves_per_cu <= 32 && iters > 0,
             DVC_EINVAL, "dvc_selftest_valu_rate: bad argument");
   DVC_HIP(hipSetDevice(c->device));
@@ -184,7 +216,9 @@ extern "C" int dvc_selftest_valu_rate(dvc_ctx *c, int kind, int waves_per_cu, in
       case 3: hipLaunchKernelGGL(dvc::k_valu_rate<3>, dim3(blocks), dim3(threads), 0, c->stream, din, dout, iters, a, b); break;
       case 4: hipLaunchKernelGGL(dvc::k_valu_rate<4>, dim3(blocks), dim3(threads), 0, c->stream, din, dout, iters, a, b); break;
       case 5: hipLaunchKernelGGL(dvc::k_valu_rate<5>, dim3(blocks), dim3(threads), 0, c->stream, din, dout, iters, a, b); break;
-      default: hipLaunchKernelGGL(dvc::k_valu_rate<6>, dim3(blocks), dim3(threads), 0, c->stream, din, dout, iters, a, b); break;
+      case 6: hipLaunchKernelGGL(dvc::k_valu_rate<6>, dim3(blocks), dim3(threads), 0, c->stream, din, dout, iters, a, b); break;
+      case 7: hipLaunchKernelGGL(dvc::k_valu_rate<7>, dim3(blocks), dim3(threads), 0, c->stream, din, dout, iters, a, b); break;
+      default: hipLaunchKernelGGL(dvc::k_valu_rate<8>, dim3(blocks), dim3(threads), 0, c->stream, din, dout, iters, a, b); break;
     }
     e = hipGetLastError();
     if (e == hipSuccess && rep == 1) e = hipEventRecord(e1, c->stream);

@@ -10,12 +10,17 @@ frame.cpp:91-111).
 
 Encoder: the WZ part of the reference encoder (DvcEnc: forwardTransform with
 the AC range, transform.cpp:229-276; quantize, quantization.cpp:95-140;
-extractBitplane, bitplane.cpp:4-38; LdpcaEncoder, LdpcaEncoder.cpp:64-95)
-with skip and intra blocks off, vectorised with numpy.  Integer arithmetic and
-the double-precision post-scaling round exactly as the C code does.
+extractBitplane, bitplane.cpp:4-38; LdpcaEncoder, LdpcaEncoder.cpp:64-95),
+vectorised with numpy, with skip blocks given as a block-mode map (no AC
+range over them, zero bits for them) and intra blocks off.  Integer
+arithmetic and the double-precision post-scaling round exactly as the C code
+does.  skip_candidates restates findSkipBlock's MSE test (skipBlock.cpp:7-43)
+without its map-size rule, which needs the JBIG encoder; write_wz_bitstream
+lays the fields out as writeWZbitstream does (DvcEnc wzbitstream.cpp:6-96).
 """
 import struct
 from dataclasses import dataclass, field
+from typing import Optional
 from pathlib import Path
 
 import numpy as np
@@ -231,15 +236,35 @@ class WZFrame:
     crc: np.ndarray          # uint8[nbp]
     acc: np.ndarray          # uint8[nbp, N/8]
     bitplanes: np.ndarray    # uint8[nbp, N] (encoder side, for checks)
+    block_mode: Optional[np.ndarray] = None  # uint8[W/8 * H/8] BlockModeFlag (0 WZ, 2 skip), None: all WZ
+    mode_bits: int = 0       # skipBlockBits + intraBlockBits (0: both modes off, i.e. 2)
 
 
-def encode_wz_frame(frame, qindex, kind="reg"):
-    """kind: the QCIF ladder ("reg" or "irreg"; CIF always regular)."""
+SKIP_BLOCK = 2  # BlockModeFlag value (defines.h:17)
+
+
+def skip_candidates(frame, past, threshold=10.0):
+    """8x8 blocks whose MSE against the past key frame is below threshold
+    (SKIPTHRESHOLD, defines.h:10): uint8[H/8 * W/8] of 0 (WZ) / 2 (skip)."""
+    h, w = frame.shape
+    d = frame.astype(np.float64) - past.astype(np.float64)
+    mse = (d * d).reshape(h // 8, 8, w // 8, 8).sum(axis=(1, 3)) / 64.0
+    return np.where(mse < threshold, SKIP_BLOCK, 0).astype(np.uint8).reshape(-1)
+
+
+def encode_wz_frame(frame, qindex, kind="reg", block_mode=None, mode_bits=0):
+    """kind: the QCIF ladder ("reg" or "irreg"; CIF always regular).
+    block_mode: uint8[H/8 * W/8] BlockModeFlag (0 WZ, 2 skip) or None."""
     h, w = frame.shape
     n = (h // 4) * (w // 4)
     L = ladder(n, qcif_kind(n, kind))
     coef = forward44_blocks(frame_to_blocks(frame))          # [bh, bw, 16]
-    ac_range = np.abs(coef).reshape(-1, 16).max(0)
+    wzmask = np.ones((h // 4, w // 4), bool)
+    if block_mode is not None:
+        m8 = np.asarray(block_mode, np.uint8).reshape(h // 8, w // 8)
+        wzmask = np.repeat(np.repeat(m8 == 0, 2, 0), 2, 1)
+    # the AC range skips skip blocks (transform.cpp:239); no intra blocks here
+    ac_range = np.abs(coef[wzmask]).reshape(-1, 16).max(0) if wzmask.any() else np.zeros(16, np.int64)
     ac_range[0] = 0
     qt, qb = QTABLE[qindex], QBITS[qindex]
     step = np.zeros(16, np.int64)
@@ -256,6 +281,7 @@ def encode_wz_frame(frame, qindex, kind="reg"):
             c = coef[..., i]
             q[..., i] = np.sign(c) * (np.abs(c) // step[i]) + shift[i]  # C division truncates
     planes = []
+    q = np.where(wzmask[..., None], q, 0)  # extractBitplane: zero bits outside WZ blocks
     qf = q.reshape(n, 16)
     for band in range(16):
         idx = ZIGZAG[band]
@@ -269,7 +295,42 @@ def encode_wz_frame(frame, qindex, kind="reg"):
         a, c = ldpca_encode(L, p)
         accs.append(a)
         crcs.append(c)
-    return WZFrame(qindex, ac_range.astype(np.int32), np.array(crcs, np.uint8), np.array(accs, np.uint8), planes)
+    mode = None if block_mode is None else np.asarray(block_mode, np.uint8).reshape(-1)
+    return WZFrame(qindex, ac_range.astype(np.int32), np.array(crcs, np.uint8), np.array(accs, np.uint8), planes,
+                   mode, mode_bits)
+
+
+def write_wz_bitstream(width, height, qindex, gop, frames, wz, deblocking=0, intra_qp=0, skip_jbig=None):
+    """A WZ bitstream (writeWZbitstream's layout): wz[k] is the WZFrame of
+    every WZ frame k of the hierarchical GOPs; skip_jbig[k] the JBIG data of
+    its skip map when the frame is in skip mode."""
+    qt = QTABLE[qindex]
+    out = bytearray([0 if width == 176 else 1, qindex, intra_qp, gop, frames >> 8, frames & 0xff, deblocking])
+
+    def gop_out(begin, end):
+        cur = (begin + end) // 2
+        if cur <= begin:
+            return
+        f = wz[cur]
+        for j in range(1, 16):
+            idx = ZIGZAG[j]
+            if qt[idx] <= 0:
+                break
+            out.extend([(int(f.ac_range[idx]) >> 8) & 0xff, int(f.ac_range[idx]) & 0xff])
+        jb = (skip_jbig or {}).get(cur)
+        out.append(1 if jb is not None else 0)
+        if jb is not None:
+            out.extend(jb)
+        out.append(0)  # intra mode off
+        for b in range(len(f.crc)):
+            out.append(int(f.crc[b]))
+            out.extend(np.asarray(f.acc[b], np.uint8).tobytes())
+        gop_out(begin, cur)
+        gop_out(cur, end)
+
+    for i in range(0, frames, gop):
+        gop_out(i, min(i + gop, frames - 1))
+    return bytes(out)
 
 
 # ----------------------------------------------------------------- LDPCA set

@@ -172,7 +172,7 @@ int dvc_selftest_phi_sweep(dvc_ctx *ctx, uint32_t lo, uint32_t hi, unsigned long
 /* Calibration entry (no counterpart in the reference): times one launch of a
  * VALU issue-rate kernel -- 8 independent chains per lane of one operation
  * kind (0 fp32 fma, 1 fp32 add, 2 fp32 mul, 3 fp64 fma, 4 int32 add, 5
- * v_sad_u8, 6 v_mqsad_u32_u8), built
+ * v_sad_u8, 6 v_mqsad_u32_u8, 7 v_qsad_pk_u16_u8, 8 v_rcp_f32), built
  * with the library's own flags -- over waves_per_cu (4..32, a multiple of 4)
  * waves per CU and `iters` trips of 128 operations.  *ms = kernel time (HIP
  * events), *wave_instr = VALU wave64 instructions the launch issues. */
@@ -209,15 +209,68 @@ typedef struct {
   const uint8_t *acc;         /* [nbitplanes * N/8] accumulated syndromes */
   uint8_t *recons;            /* out: W*H decoded frame */
   dvc_frame_stats *stats;     /* out, may be NULL */
+  /* Block modes (EncodedFrame::BlockModeFlag, wzbitstream.cpp:34-100): one
+   * byte per 8x8 block in raster order, 0 = WZ block, 2 = skip block (copied
+   * from the past key frame, copySkipBlkToRecons); NULL = every block is a WZ
+   * block.  Intra blocks (1) are rejected.  mode_bits = skipBlockBits +
+   * intraBlockBits of the frame (the bitrate's starting value,
+   * DVC_Decoder.cpp:131); 0 means 2 (both modes off). */
+  const uint8_t *block_mode;
+  int mode_bits;
 } dvc_wz_frame;
 
-/* decodeGOP's per-WZ-frame body (DVC_Decoder.cpp:26-474, skip/intra/deblock
- * off) for n frames.  batched = 0: frames one after another with the
+/* decodeGOP's per-WZ-frame body (DVC_Decoder.cpp:26-474; skip blocks per
+ * block_mode, intra blocks and deblocking off) for n frames.  batched = 0: frames one after another with the
  * sequential rate search per bitplane (the plain drop-in of ParallelMode 2);
  * batched = 1: all frames advance together, one bitplane per step, with the
  * speculative persistent LDPCA launch.  Outputs are identical in both modes
  * and equal the CPU path's (ParallelMode 0). */
 int dvc_decode_wz_frames(dvc_ctx *ctx, int n, const dvc_wz_frame *frames, int batched);
+
+/* ---------------------------------------------------------------- WZ bitstream
+ * readWZBitstream (wzbitstream.cpp:128-210) and readGOP (wzbitstream.cpp:13-126)
+ * over a bitstream held in memory.  The skip and intra block-mode maps are
+ * decoded from their JBIG images (replacing simple_JBIG_dec, sJBIG.cpp:49-103,
+ * and ByteToBitmap, sJBIG.cpp:24-47, over the vendored JBIG-KIT) by the
+ * library's own T.82 decoder. */
+typedef struct dvc_wz_stream dvc_wz_stream;
+typedef struct {
+  int width, height;          /* resolution byte 0: 176x144, 1: 352x288 */
+  int qindex, intra_qp, gop, frames, deblocking;
+  int wz_frames;              /* WZ frames in the stream */
+  int nbitplanes;             /* QbitsNumPerBlk of qindex */
+  size_t bytes_read;          /* bytes consumed by the reader */
+} dvc_wz_stream_info;
+typedef struct {
+  int past_idx, cur_idx, fut_idx;  /* begin, (begin + end) / 2, end of the frame's readGOP call */
+  int ac_range[16];
+  int skip_mode, intra_mode;
+  int mode_bits;              /* skipBlockBits + intraBlockBits as readGOP leaves them */
+  const uint8_t *block_mode;  /* W/8 * H/8 BlockModeFlag bytes (0 WZ, 1 intra, 2 skip), owned by the stream */
+  const uint8_t *crc;         /* [nbitplanes] */
+  const uint8_t *acc;         /* [nbitplanes * N/8] accumulated syndromes */
+} dvc_wz_stream_frame;
+/* Parses the whole stream (header, then every GOP's WZ frames in readGOP's
+ * hierarchical order, which is decodeGOP's decoding order). */
+int dvc_wz_stream_open(const uint8_t *data, size_t len, dvc_wz_stream **out);
+int dvc_wz_stream_info_get(const dvc_wz_stream *s, dvc_wz_stream_info *info);
+/* The k-th WZ frame of the stream (0 <= k < wz_frames); its pointers stay
+ * valid until dvc_wz_stream_close. */
+int dvc_wz_stream_frame_get(const dvc_wz_stream *s, int k, dvc_wz_stream_frame *frame);
+int dvc_wz_stream_close(dvc_wz_stream *s);
+
+/* One block-mode map as readGOP decodes it (simple_JBIG_dec + ByteToBitmap,
+ * sJBIG.cpp:24-103): `data` holds the map's JBIG data (no header: the header
+ * is the one wzbitstream.cpp:43-52 rebuilds -- width x height, one stripe,
+ * TPDON|DPON, MX 8); writes width*height bytes 0/1 to `map` and the data
+ * bytes the decoder consumed (through the stripe's end marker) to *consumed. */
+int dvc_jbig_decode_map(const uint8_t *data, size_t len, int width, int height, uint8_t *map, size_t *consumed);
+/* A whole JBIG bi-level image entity (20-byte header + data) of one plane
+ * and the lowest resolution layer (no counterpart in the reference; the
+ * general form of the above): the image (one byte 0/1 per pixel, row-major)
+ * into `image` when it is not NULL and holds capacity >= width*height bytes. */
+int dvc_jbig_decode(const uint8_t *bie, size_t len, uint8_t *image, size_t capacity, uint32_t *width,
+                    uint32_t *height, size_t *consumed);
 
 /* ---------------------------------------------------------------- side information
  * createSideInfo (sideInfoCreation.cpp:438-551: key-frame low-pass and

@@ -12,6 +12,8 @@
 #include "bitplane.h"
 #include "LdpcaEncoder.h"
 #include "ReadLdpcaLadderFile_LdpcaEncoder.h"
+#include "skipBlock.h"
+#include "sJBIG.h"
 
 extern "C" {
 int refenc_setup(int W, int H, int qindex, const char *ladder) {
@@ -57,5 +59,51 @@ int refenc_encode(const unsigned char *frame, int *acRange, unsigned char *crc, 
   memcpy(acRange, e.ACRange, sizeof(int) * 16);
   free(bp); free(trans); free(e.skipBlockFlag); free(e.intraBlockFlag);
   return QbitsNumPerBlk;
+}
+
+// The same with skip blocks (DVC_Encoder.cpp:24-28 with EnableSkipBlock):
+// findSkipBlock against the past key frame `ref`, then the frame's fields as
+// above; flags[Block88Num] receives skipBlockFlag, and when the frame keeps
+// skip mode its map's JBIG data as the encoder writes it to the bitstream
+// (wzbitstream.cpp:36-38) goes to jbig (cap bytes), its length to *jbig_len.
+// Returns skipMode.
+int refenc_encode_skip(const unsigned char *frame, const unsigned char *ref, int *acRange, unsigned char *crc,
+                       unsigned char *acc, unsigned char *flags, unsigned char *jbig, int cap, int *jbig_len) {
+  EncodedFrame e;
+  memset(&e, 0, sizeof(e));
+  e.skipBlockFlag = (unsigned char *)calloc(Block88Num, 1);
+  e.intraBlockFlag = (unsigned char *)calloc(Block88Num, 1);
+  findSkipBlock((unsigned char *)frame, (unsigned char *)ref, &e);
+  int *trans = (int *)malloc(sizeof(int) * FrameSize);
+  forwardTransform((unsigned char *)frame, trans, &e);
+  quantize(trans, &e);
+  unsigned char **bp = (unsigned char **)malloc(sizeof(unsigned char *) * QbitsNumPerBlk);
+  for (int i = 0; i < QbitsNumPerBlk; i++) bp[i] = (unsigned char *)malloc(Block44Num);
+  extractBitplane(trans, bp, &e);
+  int nb = Block44Num / 8;
+  for (int i = 0; i < QbitsNumPerBlk; i++) {
+    int len = 0;
+    unsigned char *a = LdpcaEncoder(bp[i], &len, &crc[i]);
+    memcpy(acc + (size_t)i * nb, a, nb);
+    free(a);
+    free(bp[i]);
+  }
+  memcpy(acRange, e.ACRange, sizeof(int) * 16);
+  memcpy(flags, e.skipBlockFlag, Block88Num);
+  *jbig_len = 0;
+  if (e.skipMode == 1) {
+    int nbytes = 0;
+    unsigned char *bits = BitToBitmap(e.skipBlockFlag, Block88Width, Block88Height, &nbytes);
+    FILE *fp = tmpfile();
+    unsigned char jbgh[20];
+    const int n = simple_JBIG_enc(fp, bits, Block88Width, Block88Height, jbgh, Block88Height, 1);
+    rewind(fp);
+    *jbig_len = n <= cap ? (int)fread(jbig, 1, n, fp) : -1;
+    fclose(fp);
+    free(bits);
+  }
+  const int mode = e.skipMode;
+  free(bp); free(trans); free(e.skipBlockFlag); free(e.intraBlockFlag);
+  return mode;
 }
 }

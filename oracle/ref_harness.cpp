@@ -25,6 +25,10 @@
 #include "bitplane.h"
 #include "frame.h"
 #include "DVC_Decoder.h"
+#include "jbig.h"
+#include "sJBIG.h"
+#include "wzbitstream.h"
+#include <vector>
 
 // Iteration counter that the reference's beliefPropagation_OpenMP increments
 // (declared extern at LdpcaDecoder.cpp:30 and frame.cpp:10); the harness owns
@@ -265,4 +269,107 @@ void ref_reconstruction(int *reconsTrans, int *sideTrans, int index, float *alph
   free(db);
 }
 double ref_psnr(const unsigned char *a, const unsigned char *b) { return psnr((unsigned char *)a, (unsigned char *)b); }
+
+// ---- JBIG block-mode maps and the WZ bitstream reader (fixture generation)
+static std::vector<unsigned char> g_bie;
+static void harness_bie_out(unsigned char *start, size_t len, void *) { g_bie.insert(g_bie.end(), start, start + len); }
+
+// The reference's JBIG-KIT encoder on a 0/1 map (rows packed MSB first):
+// the 20-byte header followed by the data into out; returns the length, or
+// -1 when cap is too small.
+int ref_jbig_encode(const unsigned char *map, int w, int h, int order, int options, int l0, int mx,
+                    unsigned char *out, int cap) {
+  const int bpl = (w + 7) / 8;
+  std::vector<unsigned char> packed((size_t)bpl * h, 0);
+  for (int y = 0; y < h; y++)
+    for (int x = 0; x < w; x++)
+      if (map[y * w + x]) packed[(size_t)y * bpl + x / 8] |= (unsigned char)(0x80 >> (x & 7));
+  unsigned char *planes[1] = {packed.data()};
+  struct jbg_enc_state se;
+  g_bie.clear();
+  jbg_enc_init(&se, w, h, 1, planes, harness_bie_out, NULL);
+  jbg_enc_options(&se, order, options, l0, mx, 0);
+  unsigned char hdr[20];
+  jbg_enc_out(&se, hdr);
+  jbg_enc_free(&se);
+  if ((int)(20 + g_bie.size()) > cap) return -1;
+  memcpy(out, hdr, 20);
+  memcpy(out + 20, g_bie.data(), g_bie.size());
+  return (int)(20 + g_bie.size());
+}
+
+// The reference's JBIG-KIT decoder on a whole BIE: image (w*h bytes 0/1)
+// and the bytes consumed; returns 0, or the JBIG-KIT error code.
+int ref_jbig_decode_bie(const unsigned char *bie, int len, unsigned char *img, int cap, int *w, int *h, int *consumed) {
+  struct jbg_dec_state sd;
+  jbg_dec_init(&sd);
+  size_t cnt = 0;
+  int r = jbg_dec_in(&sd, (unsigned char *)bie, len, &cnt);
+  if (r != JBG_EOK && r != JBG_EOK_INTR) { jbg_dec_free(&sd); return r; }
+  *w = (int)jbg_dec_getwidth(&sd);
+  *h = (int)jbg_dec_getheight(&sd);
+  *consumed = (int)cnt;
+  const unsigned char *p = jbg_dec_getimage(&sd, 0);
+  const int bpl = (*w + 7) / 8;
+  if (*w * *h <= cap)
+    for (int y = 0; y < *h; y++)
+      for (int x = 0; x < *w; x++) img[y * *w + x] = (p[(size_t)y * bpl + x / 8] >> (7 - (x & 7))) & 1;
+  jbg_dec_free(&sd);
+  return 0;
+}
+
+// One block-mode map as readGOP decodes it (wzbitstream.cpp:40-66): the
+// rebuilt header, simple_JBIG_dec over `bufsize` bytes, ByteToBitmap.
+// Returns the decoded byte count (decode_byte).
+int ref_jbig_map_decode(const unsigned char *bid, int len, int w, int h, unsigned char *map) {
+  const int bufsize = ((((w + 4) >> 3) << 3) * h >> 3) * 2;
+  struct jbg_enc_state se;
+  jbg_enc_init(&se, w, h, 1, NULL, NULL, NULL);
+  jbg_enc_options(&se, JBG_ILEAVE | JBG_SMID, JBG_TPDON | JBG_DPON, h, 8, 0);
+  unsigned char jbgh[20];
+  jbg_set_header(&se, jbgh);
+  std::vector<unsigned char> enc((size_t)bufsize, 0), dec((size_t)bufsize, 0);
+  memcpy(enc.data(), bid, len < bufsize ? len : bufsize);
+  const int used = simple_JBIG_dec(enc.data(), bufsize, jbgh, bufsize, dec.data());
+  unsigned char *bits = ByteToBitmap(dec.data(), w, h);
+  memcpy(map, bits, (size_t)w * h);
+  free(bits);
+  return used;
+}
+
+// A frame's skip map as readGOP sets it from the map's JBIG data: skip
+// blocks in BlockModeFlag, skipMode 1, skipBlockBits = decoded bytes * 8 + 1.
+int ref_set_skip_map(int idx, const unsigned char *bid, int len) {
+  EncodedFrame *e = &encodedFrames[idx];
+  std::vector<unsigned char> map((size_t)Block88Num);
+  const int used = ref_jbig_map_decode(bid, len, Block88Width, Block88Height, map.data());
+  for (int i = 0; i < Block88Num; i++)
+    if (map[i] == 1) e->BlockModeFlag[i] = SKIP_B;
+  e->skipMode = 1;
+  e->skipBlockBits = (used << 3) + 1;
+  return used;
+}
+
+// readWZBitstream on a file: the header fields and, per WZ frame index,
+// ACRange, the modes and mode bits, BlockModeFlag, CRCs and syndromes
+// (fetched with ref_wz_frame).
+int ref_read_wz_bitstream(const char *path, int *hdr) {
+  memset(&codingOption, 0, sizeof(codingOption));  // its file-name fields are appended to
+  QbitsNumPerBlk = 0;                              // readWZBitstream accumulates into it
+  readWZBitstream((char *)path);
+  hdr[0] = codingOption.FrameWidth; hdr[1] = codingOption.FrameHeight; hdr[2] = codingOption.Qindex;
+  hdr[3] = codingOption.IntraQP; hdr[4] = codingOption.GOP; hdr[5] = codingOption.FramesToBeDecoded;
+  hdr[6] = codingOption.DeblockingMode; hdr[7] = QbitsNumPerBlk;
+  return 0;
+}
+int ref_wz_frame(int idx, int *acRange, int *modes, unsigned char *flags, unsigned char *crc, unsigned char *acc) {
+  EncodedFrame *e = &encodedFrames[idx];  // a WZ frame's index (key frames' entries are not set)
+  memcpy(acRange, e->ACRange, sizeof(int) * 16);
+  modes[0] = e->skipMode; modes[1] = e->intraMode; modes[2] = e->skipBlockBits; modes[3] = e->intraBlockBits;
+  memcpy(flags, e->BlockModeFlag, Block88Num);
+  memcpy(crc, e->CRC, QbitsNumPerBlk);
+  const int nb = Block44Num / 8;
+  for (int j = 0; j < QbitsNumPerBlk; j++) memcpy(acc + (size_t)j * nb, e->AccumulatedSyndrome[j], nb);
+  return 0;
+}
 }  // extern "C"

@@ -31,8 +31,21 @@ Fixtures (all small JSON):
                   GOP 4 and 8 (decoded WZ frames are references of later
                   ones), ParallelMode 0 and 1 (DVC_GOLDEN_ONLY=gop writes
                   just these)
+  jbig.json       the reference's JBIG-KIT (DvcDec/src/jbig.cpp) on seeded
+                  bi-level maps: each map's BIE as JBIG-KIT's encoder writes
+                  it under several option sets (the block-mode maps' own,
+                  typical prediction, two-line template, several stripes,
+                  SDRST, adaptive template), and what JBIG-KIT's decoder and
+                  readGOP's simple_JBIG_dec read back (image, bytes used)
+  frames_skip.json  skip blocks: the reference encoder with skip blocks on
+                  (findSkipBlock + the map's JBIG data) and decodeGOP on the
+                  map readGOP decodes, QCIF and CIF, ParallelMode 0
+  wz_stream.json  a WZ bitstream with skip frames (GOP 4) as the reference's
+                  readWZBitstream parses it, and decodeGOP over its GOPs
+  (DVC_GOLDEN_ONLY=skip writes just these three)
 Inputs are regenerated from their seeds by dvc_amd.workload, so a fixture
-holds only seeds, sizes and expected outputs.
+holds only seeds, sizes and expected outputs (and the small JBIG images,
+which take JBIG-KIT's encoder to make).
 """
 import ctypes
 import hashlib
@@ -314,8 +327,201 @@ def frames_gop_fixture(mode=0):
     return {"max_iter": 50, "cases": out}
 
 
+# ------------------------------------------------------------ skip blocks, JBIG, WZ bitstream
+JBIG_ILEAVE, JBIG_SMID, JBIG_TPBON, JBIG_TPDON, JBIG_DPON, JBIG_LRLTWO, JBIG_SDRST = 0x02, 0x01, 0x08, 0x10, 0x04, 0x40, 0x200
+
+
+def jbig_encode(m, order, options, l0, mx):
+    lib = O.ref_dec()
+    h, w = m.shape
+    out = np.zeros(65536, np.uint8)
+    n = lib.ref_jbig_encode(O.P(np.ascontiguousarray(m, np.uint8)), w, h, order, options, l0, mx, O.P(out), len(out))
+    assert n > 0
+    return out[:n].tobytes()
+
+
+def jbig_decode_ref(bie):
+    lib = O.ref_dec()
+    buf = np.frombuffer(bie, np.uint8)
+    img = np.zeros(1 << 20, np.uint8)
+    w, h, used = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rc = lib.ref_jbig_decode_bie(O.P(buf), len(buf), O.P(img), len(img), ctypes.byref(w), ctypes.byref(h),
+                                 ctypes.byref(used))
+    assert rc == 0, rc
+    return img[:w.value * h.value].reshape(h.value, w.value), used.value
+
+
+def jbig_maps(seed):
+    """Seeded bi-level maps: random at several densities, blocky regions (as
+    skip maps are), and periodic rows that make JBIG-KIT move its adaptive
+    template pixel."""
+    rng = np.random.default_rng(seed)
+    maps = []
+    for (w, h) in [(22, 18), (44, 36), (37, 23), (64, 40)]:
+        for d in (0.02, 0.3, 0.7, 0.98):
+            maps.append(("random", (rng.random((h, w)) < d).astype(np.uint8)))
+        blk = np.zeros((h, w), np.uint8)
+        for _ in range(6):
+            y0, x0 = rng.integers(0, h), rng.integers(0, w)
+            blk[y0:y0 + rng.integers(2, 12), x0:x0 + rng.integers(2, 16)] = 1
+        maps.append(("blocks", blk))
+        per = int(rng.integers(3, 9))
+        row = (rng.random(per) < 0.5).astype(np.uint8)
+        pm = np.tile(row, (h, w // per + 1))[:, :w].copy()
+        pm ^= (rng.random((h, w)) < 0.03).astype(np.uint8)
+        maps.append((f"periodic{per}", pm))
+    maps.append(("zeros", np.zeros((36, 44), np.uint8)))
+    maps.append(("ones", np.ones((36, 44), np.uint8)))
+    return maps
+
+
+JBIG_OPTION_SETS = [
+    # (name, order, options, l0 (0: the image height), mx)
+    ("block_mode_map", JBIG_ILEAVE | JBIG_SMID, JBIG_TPDON | JBIG_DPON, 0, 8),
+    ("tpbon", JBIG_ILEAVE | JBIG_SMID, JBIG_TPBON | JBIG_TPDON | JBIG_DPON, 0, 8),
+    ("two_line", JBIG_ILEAVE | JBIG_SMID, JBIG_LRLTWO | JBIG_TPBON, 0, 8),
+    ("stripes4", JBIG_ILEAVE | JBIG_SMID, JBIG_TPBON, 4, 8),
+    ("sdrst_stripes5", JBIG_ILEAVE | JBIG_SMID, JBIG_TPBON | JBIG_SDRST, 5, 8),
+    ("no_at", JBIG_ILEAVE | JBIG_SMID, JBIG_TPDON | JBIG_DPON, 0, 0),
+    ("at_mx16", JBIG_ILEAVE | JBIG_SMID, JBIG_TPBON, 7, 16),
+]
+
+
+def pack_rows(m):
+    return np.packbits(np.ascontiguousarray(m, np.uint8), axis=1).tobytes().hex()
+
+
+def jbig_fixture():
+    cases = []
+    for k, (kind, m) in enumerate(jbig_maps(2024)):
+        h, w = m.shape
+        for (name, order, options, l0, mx) in JBIG_OPTION_SETS:
+            bie = jbig_encode(m, order, options, l0 or h, mx)
+            img, used = jbig_decode_ref(bie)
+            assert np.array_equal(img, m)
+            case = {"map": k, "kind": kind, "W": w, "H": h, "options": name, "bie": bie.hex(),
+                    "image": pack_rows(img), "consumed": used}
+            if name == "block_mode_map":
+                # readGOP's own path: the data only, header rebuilt, bytes used
+                mp = np.zeros(w * h, np.uint8)
+                data = np.frombuffer(bie[20:] + bytes(64), np.uint8)  # the bitstream goes on after the map
+                case["map_decode_consumed"] = O.ref_dec().ref_jbig_map_decode(O.P(data), len(data), w, h, O.P(mp))
+                assert np.array_equal(mp.reshape(h, w), m)
+            cases.append(case)
+    print(f"jbig: {len(cases)} cases", flush=True)
+    return {"cases": cases}
+
+
+SKIP_CASES = [
+    # (W, H, qindex, seed, shift, frames, wz frame index)
+    (176, 144, 4, 1101, 0, 3, 1),
+    (176, 144, 8, 1102, 0, 3, 1),
+    (352, 288, 8, 1103, 0, 3, 1),
+    (352, 288, 4, 1104, 1, 3, 1),
+]
+
+
+def ref_encode_skip(frame, past, q):
+    """The reference encoder with skip blocks on: (WZFrame, skip_jbig or None)."""
+    enc = O.ref_enc()
+    h, w = frame.shape
+    n = (w // 4) * (h // 4)
+    lad = REF / f"release/DvcEnc/LDPCA_LadderFile/RegEncoder{n}.lad"
+    nbp = enc.refenc_setup(w, h, q, str(lad).encode())
+    acr = np.zeros(16, np.int32)
+    crc = np.zeros(nbp, np.uint8)
+    acc = np.zeros((nbp, n // 8), np.uint8)
+    flags = np.zeros((h // 8) * (w // 8), np.uint8)
+    jb = np.zeros(4096, np.uint8)
+    jl = ctypes.c_int()
+    mode = enc.refenc_encode_skip(O.P(np.ascontiguousarray(frame)), O.P(np.ascontiguousarray(past)), O.P(acr), O.P(crc),
+                                  O.P(acc), O.P(flags), O.P(jb), len(jb), ctypes.byref(jl))
+    block_mode = (flags * W.SKIP_BLOCK).astype(np.uint8)
+    mine = W.encode_wz_frame(frame, q, block_mode=block_mode)
+    assert np.array_equal(mine.ac_range, acr) and np.array_equal(mine.crc, crc) and np.array_equal(mine.acc, acc), \
+        "numpy encoder differs from the reference encoder with skip blocks"
+    assert mode == 0 or jl.value > 0
+    return mine, (jb[:jl.value].tobytes() if mode == 1 else None), block_mode
+
+
+def frames_skip_fixture():
+    out = []
+    for (w, h, q, seed, shift, nf, fi) in SKIP_CASES:
+        vid = W.synthetic_video(w, h, nf, shift, seed)
+        wz, jbig, block_mode = ref_encode_skip(vid[fi], vid[0], q)
+        assert jbig is not None, "case without skip mode"
+        R = O.RefDecoder(w, h, q, mode=0, nthreads=1)
+        rec, info = R.decode_wz_frame(vid[0], vid[2], vid[fi], wz, skip_jbig=jbig)
+        out.append({"W": w, "H": h, "qindex": q, "seed": seed, "shift": shift, "frames": nf, "wz": fi,
+                    "skip_blocks": int((block_mode == W.SKIP_BLOCK).sum()), "skip_jbig": jbig.hex(),
+                    "block_mode_sha256": sha(block_mode), "levels": info["levels"],
+                    "entropy_bits": [fbits(e) for e in info["entropy"]], "bitrate": info["bitrate"],
+                    "psnr": info["psnr"], "recons_sha256": sha(rec)})
+        print(f"skip case {w}x{h} q{q}: {out[-1]['skip_blocks']} skip blocks, jbig {len(jbig)} B, "
+              f"psnr {info['psnr']:.4f} bitrate {info['bitrate']}", flush=True)
+    return {"max_iter": 50, "cases": out}
+
+
+STREAM_CASE = dict(W=176, H=144, qindex=4, seed=1201, shift=0, gop=4, frames=9)
+
+
+def wz_stream_fixture():
+    """A WZ bitstream (GOPs of 4 over 9 frames, skip frames as the reference
+    encoder decides) parsed by the reference's readWZBitstream, and decoded
+    GOP by GOP by the reference's decodeGOP."""
+    c = STREAM_CASE
+    w, h, q, gop, nf = c["W"], c["H"], c["qindex"], c["gop"], c["frames"]
+    vid = W.synthetic_video(w, h, nf, c["shift"], c["seed"])
+    wz, jbig = {}, {}
+    for i in range(0, nf, gop):
+        order = []
+        O._gop_order(i, min(i + gop, nf - 1), order)
+        for cur, past, fut in order:
+            f, jb, _ = ref_encode_skip(vid[cur], vid[past], q)
+            wz[cur] = f
+            if jb is not None:
+                jbig[cur] = jb
+    stream = W.write_wz_bitstream(w, h, q, gop, nf, wz, skip_jbig=jbig)
+    path = Path("/tmp/dvc_wz_stream_fixture.bin")
+    path.write_bytes(stream)
+    lib = O.ref_dec()
+    O.RefDecoder(w, h, q)  # geometry and ladder
+    hdr = np.zeros(8, np.int32)
+    lib.ref_read_wz_bitstream(str(path).encode(), O.P(hdr))
+    n8, nbp, n = (w // 8) * (h // 8), int(hdr[7]), (w // 4) * (h // 4)
+    frames = []
+    for cur in sorted(wz):
+        acr = np.zeros(16, np.int32); modes = np.zeros(4, np.int32); flags = np.zeros(n8, np.uint8)
+        crc = np.zeros(nbp, np.uint8); acc = np.zeros((nbp, n // 8), np.uint8)
+        lib.ref_wz_frame(cur, O.P(acr), O.P(modes), O.P(flags), O.P(crc), O.P(acc))
+        frames.append({"cur": cur, "ac_range": acr.tolist(), "skip_mode": int(modes[0]), "intra_mode": int(modes[1]),
+                       "skip_bits": int(modes[2]), "intra_bits": int(modes[3]), "block_mode_sha256": sha(flags),
+                       "crc_sha256": sha(crc), "acc_sha256": sha(acc)})
+    # decodeGOP over each GOP on what readGOP read (skip maps from the JBIG data)
+    decoded = {}
+    for i in range(0, nf, gop):
+        end = min(i + gop, nf - 1)
+        R = O.RefDecoder(w, h, q, mode=0, nthreads=1, nframes=nf)
+        res, order = R.decode_gop(vid, wz, i, end, skip_jbig=jbig)
+        for cur, past, fut in order:
+            rec, info = res[cur]
+            decoded[cur] = {"cur": cur, "past": past, "fut": fut, "levels": info["levels"],
+                            "entropy_bits": [fbits(e) for e in info["entropy"]], "bitrate": info["bitrate"],
+                            "psnr": info["psnr"], "recons_sha256": sha(rec)}
+            print(f"stream frame {cur} ({past},{fut}) skip={cur in jbig}: psnr {info['psnr']:.4f} "
+                  f"bitrate {info['bitrate']}", flush=True)
+    return {"case": c, "header": hdr.tolist(), "stream_sha256": hashlib.sha256(stream).hexdigest(),
+            "stream_bytes": len(stream), "skip_jbig": {str(k): v.hex() for k, v in jbig.items()},
+            "frames": frames, "decoded": [decoded[k] for k in sorted(decoded)], "max_iter": 50}
+
+
 def main():
     assert O.have_reference_build(), "run `make -C oracle ref` first"
+    if os.environ.get("DVC_GOLDEN_ONLY") == "skip":
+        (HERE / "jbig.json").write_text(json.dumps(jbig_fixture(), indent=1))
+        (HERE / "frames_skip.json").write_text(json.dumps(frames_skip_fixture(), indent=1))
+        (HERE / "wz_stream.json").write_text(json.dumps(wz_stream_fixture(), indent=1))
+        return
     if os.environ.get("DVC_GOLDEN_ONLY") == "ladder":
         (HERE / "ladder.json").write_text(json.dumps(ladder_fixture(), indent=1))
         return

@@ -289,13 +289,21 @@ class RefDecoder:
                 write_text_ladder(ladder(n), lad, inv)
         self.lib.ref_load_ladder(str(lad).encode(), str(inv).encode())
 
-    def decode_wz_frame(self, past, fut, orig, wz):
+    def set_skip_map(self, idx, jbig):
+        """readGOP's skip branch on the map's JBIG data (the reference's own
+        JBIG-KIT decoder); returns the decoded byte count."""
+        buf = np.frombuffer(bytes(jbig), np.uint8)
+        return self.lib.ref_set_skip_map(idx, P(buf), len(buf))
+
+    def decode_wz_frame(self, past, fut, orig, wz, skip_jbig=None):
         h, w = past.shape
         self.lib.ref_set_frame(0, P(np.ascontiguousarray(past)), P(np.ascontiguousarray(orig)))
         self.lib.ref_set_frame(2, P(np.ascontiguousarray(fut)), P(np.ascontiguousarray(orig)))
         self.lib.ref_set_frame(1, P(np.ascontiguousarray(orig)), P(np.ascontiguousarray(orig)))
         self.lib.ref_set_encoded(1, P(np.ascontiguousarray(wz.ac_range, np.int32)),
                                  P(np.ascontiguousarray(wz.crc, np.uint8)), P(np.ascontiguousarray(wz.acc, np.uint8)))
+        if skip_jbig is not None:
+            self.set_skip_map(1, skip_jbig)
         n = self.lib.ref_decode_gop(0, 2)
         rates = np.zeros(4096, np.int32); levels = np.zeros(4096, np.int32); ent = np.zeros(4096, np.float32)
         self.lib.ref_get_log(P(rates), P(levels), P(ent))
@@ -304,7 +312,7 @@ class RefDecoder:
         return rec, dict(levels=levels[:n].tolist(), rates=rates[:n].tolist(), entropy=ent[:n].tolist(),
                          psnr=self.lib.ref_get_psnr(1), bitrate=self.lib.ref_get_bitrate(1))
 
-    def decode_gop(self, video, wz, begin, end):
+    def decode_gop(self, video, wz, begin, end, skip_jbig=None):
         """decodeGOP(begin, end) (DVC_Decoder.cpp:26-474) over frames
         begin..end of `video`: the two key frames are the originals, every
         frame in between is a WZ frame (wz[cur]) decoded in the reference's
@@ -321,6 +329,8 @@ class RefDecoder:
             f = wz[cur]
             self.lib.ref_set_encoded(cur, P(np.ascontiguousarray(f.ac_range, np.int32)),
                                      P(np.ascontiguousarray(f.crc, np.uint8)), P(np.ascontiguousarray(f.acc, np.uint8)))
+            if skip_jbig and skip_jbig.get(cur) is not None:
+                self.set_skip_map(cur, skip_jbig[cur])
         n = self.lib.ref_decode_gop(begin, end)
         rates = np.zeros(4096, np.int32); levels = np.zeros(4096, np.int32); ent = np.zeros(4096, np.float32)
         self.lib.ref_get_log(P(rates), P(levels), P(ent))
