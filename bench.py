@@ -463,6 +463,31 @@ def profiled_value(kernel, workload, key):
     return None
 
 
+def profiled_mix(kernel, workload):
+    """The VALU instruction classes per launch of `kernel` (the committed
+    rocprofv3 SQ_INSTS_VALU_* passes of this workload): dict class -> count."""
+    f = ROOT / "profiles" / f"{kernel}_{workload}.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith(f"valu_mix_per_launch {kernel} "):
+            return {k: float(v) for k, v in (t.split("=") for t in line.split()[2:])}
+    return None
+
+
+def mix_peak(mix, valu):
+    """The issue rate the instruction mix allows: total / sum(n_k / rate_k),
+    each class at the rate valu_calibration measured for it (fp32 for the
+    classes it has no kernel for)."""
+    k = valu
+    fp32 = max(k["fp32_fma"]["wave_instr_per_s"], k["fp32_add"]["wave_instr_per_s"], k["fp32_mul"]["wave_instr_per_s"])
+    rate = {"int32": k["int32_add"]["wave_instr_per_s"], "fp32": fp32, "f64": k["fp64_fma"]["wave_instr_per_s"],
+            "trans": k["trans_rcp_f32"]["wave_instr_per_s"] if "trans_rcp_f32" in k else fp32 / 2,
+            "cvt": fp32, "int64": fp32, "other": fp32}
+    total = sum(mix.values())
+    return total / sum(n / rate[c] for c, n in mix.items()), rate
+
+
 def bp_roofline(kernel_ms_avg, hbm_bytes_per_launch, edge_updates_per_launch, workload, valu=None):
     """Roofline of the BP kernel.  Its bound is the VALU issue port: the exact
     glibc-identical transcendentals (two phi per edge-update) are about 200
@@ -486,11 +511,20 @@ def bp_roofline(kernel_ms_avg, hbm_bytes_per_launch, edge_updates_per_launch, wo
            "lds": {"achieved": lds, "peak": LDS_PEAK_GBS, "unit": "GB/s", "frac": lds / LDS_PEAK_GBS,
                    "bytes_per_launch": 16.0 * edge_updates_per_launch},
            "profile": f"profiles/bp_6336_{workload}.md"}
+    mix = profiled_mix("bp_6336", workload)
     if vi and valu:
-        peak = valu["peak_wave_instr_per_s"]
+        if mix:
+            peak, rates = mix_peak(mix, valu)
+            src = ("the instruction mix of the committed SQ_INSTS_VALU_* passes, each class at the issue rate "
+                   "valu_calibration measured in this run")
+        else:
+            peak, rates = valu["peak_wave_instr_per_s"], None
+            src = "valu_calibration (measured in this run), highest rate"
         res.update({"bound": "valu", "achieved": vi / t, "peak": peak, "unit": "VALU wave64-instr/s",
-                    "frac": vi / t / peak, "wave_instr_per_launch": vi,
-                    "peak_source": "valu_calibration (measured in this run)"})
+                    "frac": vi / t / peak, "wave_instr_per_launch": vi, "instruction_mix_per_launch": mix,
+                    "class_rates": rates, "peak_source": src,
+                    "peak_single_rate": valu["peak_wave_instr_per_s"],
+                    "frac_of_single_rate": vi / t / valu["peak_wave_instr_per_s"]})
     else:
         res.update({"bound": "hbm", "achieved": hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": hbm / HBM_PEAK_GBS})

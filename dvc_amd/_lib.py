@@ -70,7 +70,9 @@ class WZFrameSpec(ctypes.Structure):
                 ("past", ctypes.c_void_p), ("fut", ctypes.c_void_p), ("orig", ctypes.c_void_p),
                 ("ac_range", ctypes.c_void_p), ("crc", ctypes.c_void_p), ("acc", ctypes.c_void_p),
                 ("recons", ctypes.c_void_p), ("stats", ctypes.POINTER(FrameStats)),
-                ("block_mode", ctypes.c_void_p), ("mode_bits", ctypes.c_int)]
+                ("block_mode", ctypes.c_void_p), ("mode_bits", ctypes.c_int), ("intra_coeffs", ctypes.c_void_p),
+                ("deblocking", ctypes.c_int), ("intra_qp", ctypes.c_int), ("past_is_wz", ctypes.c_int),
+                ("past_block_mode", ctypes.c_void_p)]
 
 
 class Tuning(ctypes.Structure):
@@ -109,7 +111,8 @@ class WzStreamFrame(ctypes.Structure):
     _fields_ = [("past_idx", ctypes.c_int), ("cur_idx", ctypes.c_int), ("fut_idx", ctypes.c_int),
                 ("ac_range", ctypes.c_int * 16), ("skip_mode", ctypes.c_int), ("intra_mode", ctypes.c_int),
                 ("mode_bits", ctypes.c_int), ("block_mode", ctypes.c_void_p), ("crc", ctypes.c_void_p),
-                ("acc", ctypes.c_void_p)]
+                ("acc", ctypes.c_void_p), ("intra_coeffs", ctypes.c_void_p), ("past_is_wz", ctypes.c_int),
+                ("past_block_mode", ctypes.c_void_p)]
 
 
 SIGNATURES = {
@@ -138,6 +141,9 @@ SIGNATURES = {
     "dvc_wz_stream_info_get": (_c.c_int, [_c.c_void_p, _c.POINTER(WzStreamInfo)]),
     "dvc_wz_stream_frame_get": (_c.c_int, [_c.c_void_p, _c.c_int, _c.POINTER(WzStreamFrame)]),
     "dvc_wz_stream_close": (_c.c_int, [_c.c_void_p]),
+    "dvc_wz_stream_attach_intra": (_c.c_int, [_c.c_void_p, _c.c_void_p, _c.c_size_t]),
+    "dvc_intra_parse": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_size_t), _c.c_void_p, _c.c_int, _c.c_int,
+                                   _c.c_int, _c.c_void_p]),
     "dvc_jbig_decode_map": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.c_int, _c.c_int, _c.c_void_p,
                                        _c.POINTER(_c.c_size_t)]),
     "dvc_jbig_decode": (_c.c_int, [_c.c_void_p, _c.c_size_t, _c.c_void_p, _c.c_size_t, _c.POINTER(_c.c_uint32),
@@ -340,15 +346,14 @@ class Context:
             crc = np.ascontiguousarray(wz.crc, np.uint8)
             acc = np.ascontiguousarray(wz.acc, np.uint8)
             rec = np.zeros_like(past)
-            mode = _block_mode(wz)
-            keep += [past, fut, orig, acr, crc, acc, mode]
+            mf, mkeep = _mode_fields(wz)
+            keep += [past, fut, orig, acr, crc, acc] + mkeep
             outs.append(rec)
             pi, ci, fi = fr.get("idx", (0, 1, 2))
             h, w = past.shape
             specs[i] = WZFrameSpec(w, h, wz.qindex, pi, ci, fi, past.ctypes.data, fut.ctypes.data,
                                    None if orig is None else orig.ctypes.data, acr.ctypes.data, crc.ctypes.data,
-                                   acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats[i]),
-                                   None if mode is None else mode.ctypes.data, int(getattr(wz, "mode_bits", 0)))
+                                   acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats[i]), *mf)
         _check(lib.dvc_decode_wz_frames(self.h, k, specs, 1 if batched else 0), "dvc_decode_wz_frames")
         return [(outs[i], stats[i].as_dict()) for i in range(k)]
 
@@ -473,17 +478,29 @@ def _frame_spec(fr, stats):
     rec = np.zeros_like(past)
     pi, ci, fi = fr.get("idx", (0, 1, 2))
     h, w = past.shape
-    mode = _block_mode(wz)
+    mf, mkeep = _mode_fields(wz)
     spec = WZFrameSpec(w, h, wz.qindex, pi, ci, fi, past.ctypes.data, fut.ctypes.data,
                        None if orig is None else orig.ctypes.data, acr.ctypes.data, crc.ctypes.data,
-                       acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats),
-                       None if mode is None else mode.ctypes.data, int(getattr(wz, "mode_bits", 0)))
-    return spec, [past, fut, orig, acr, crc, acc, rec, mode], rec
+                       acc.ctypes.data, rec.ctypes.data, ctypes.pointer(stats), *mf)
+    return spec, [past, fut, orig, acr, crc, acc, rec] + mkeep, rec
 
 
 def _block_mode(wz):
     m = getattr(wz, "block_mode", None)
     return None if m is None else np.ascontiguousarray(m, np.uint8)
+
+
+def _mode_fields(wz):
+    """(ctypes field values from block_mode on, arrays to keep alive) of a WZFrame."""
+    mode = _block_mode(wz)
+    ic = getattr(wz, "intra_coeffs", None)
+    ic = None if ic is None else np.ascontiguousarray(ic, np.int32)
+    pm = getattr(wz, "past_block_mode", None)
+    pm = None if pm is None else np.ascontiguousarray(pm, np.uint8)
+    vals = (None if mode is None else mode.ctypes.data, int(getattr(wz, "mode_bits", 0)),
+            None if ic is None else ic.ctypes.data, int(getattr(wz, "deblocking", 0)), int(getattr(wz, "intra_qp", 0)),
+            int(getattr(wz, "past_is_wz", 0)), None if pm is None else pm.ctypes.data)
+    return vals, [mode, ic, pm]
 
 
 class Engine:
@@ -517,13 +534,12 @@ class Engine:
         stats = FrameStats()
         acr = np.ascontiguousarray(wz.ac_range, np.int32)
         crc = np.ascontiguousarray(wz.crc, np.uint8)
-        mode = _block_mode(wz)
+        mf, mkeep = _mode_fields(wz)
         spec = WZFrameSpec(width, height, wz.qindex, idx[0], idx[1], idx[2], past_ptr, fut_ptr, None,
-                           acr.ctypes.data, crc.ctypes.data, acc_ptr, recons_ptr, ctypes.pointer(stats),
-                           None if mode is None else mode.ctypes.data, int(getattr(wz, "mode_bits", 0)))
+                           acr.ctypes.data, crc.ctypes.data, acc_ptr, recons_ptr, ctypes.pointer(stats), *mf)
         t = ctypes.c_uint64()
         _check(lib.dvc_engine_submit(self.h, ctypes.byref(spec), past_key, fut_key, ctypes.byref(t)), "dvc_engine_submit")
-        self.live[t.value] = (spec, [acr, crc, mode], None, stats)
+        self.live[t.value] = (spec, [acr, crc] + mkeep, None, stats)
         return t.value
 
     def step(self):
@@ -594,14 +610,18 @@ def jbig_decode(bie):
     return out, used.value
 
 
-def read_wz_stream(data):
-    """Parses a WZ bitstream (dvc_wz_stream_*): (info dict, list of per-WZ-frame dicts in decoding order with
-    past/cur/fut indices and a workload.WZFrame carrying ac_range, crc, acc, block_mode and mode_bits)."""
+def read_wz_stream(data, intra=None):
+    """Parses a WZ bitstream (dvc_wz_stream_*) and, when given, its intra bitstream: (info dict, list of
+    per-WZ-frame dicts in decoding order with past/cur/fut indices and a workload.WZFrame carrying every field
+    dvc_wz_frame takes from the stream)."""
     from .workload import WZFrame
     buf = np.frombuffer(bytes(data), np.uint8)
     h = ctypes.c_void_p()
     _check(lib.dvc_wz_stream_open(buf.ctypes.data, len(buf), ctypes.byref(h)), "dvc_wz_stream_open")
     try:
+        if intra is not None:
+            ib = np.frombuffer(bytes(intra) or b"\0", np.uint8)
+            _check(lib.dvc_wz_stream_attach_intra(h, ib.ctypes.data, len(bytes(intra))), "dvc_wz_stream_attach_intra")
         info = WzStreamInfo()
         _check(lib.dvc_wz_stream_info_get(h, ctypes.byref(info)), "dvc_wz_stream_info_get")
         n8 = (info.width // 8) * (info.height // 8)
@@ -613,8 +633,14 @@ def read_wz_stream(data):
             mode = np.ctypeslib.as_array((ctypes.c_uint8 * n8).from_address(f.block_mode)).copy()
             crc = np.ctypeslib.as_array((ctypes.c_uint8 * info.nbitplanes).from_address(f.crc)).copy()
             acc = np.ctypeslib.as_array((ctypes.c_uint8 * (info.nbitplanes * synd)).from_address(f.acc)).copy()
+            nblk = (info.width // 4) * (info.height // 4)
+            ic = (np.ctypeslib.as_array((ctypes.c_int32 * (16 * nblk)).from_address(f.intra_coeffs)).copy()
+                  .reshape(nblk, 16) if f.intra_coeffs else None)
+            pm = (np.ctypeslib.as_array((ctypes.c_uint8 * n8).from_address(f.past_block_mode)).copy()
+                  if f.past_block_mode else None)
             wz = WZFrame(info.qindex, np.array(list(f.ac_range), np.int32), crc, acc.reshape(info.nbitplanes, synd),
-                         None, mode if mode.any() else None, f.mode_bits)
+                         None, mode if mode.any() else None, f.mode_bits, ic, info.deblocking, info.intra_qp,
+                         f.past_is_wz, pm)
             frames.append(dict(idx=(f.past_idx, f.cur_idx, f.fut_idx), skip_mode=f.skip_mode, intra_mode=f.intra_mode,
                                wz=wz))
         return info.as_dict(), frames

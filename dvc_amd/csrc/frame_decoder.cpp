@@ -102,7 +102,10 @@ struct Job {
   dvc_wz_frame spec{};           // caller's request; pointers must stay valid until completion
   std::vector<int> acRange;      // owned copies of the bitstream fields
   std::vector<uint8_t> crc, acc;
-  std::vector<uint8_t> mode;     // 8x8 block modes when the frame has skip blocks, else empty
+  std::vector<uint8_t> mode;     // 8x8 block modes when the frame has skip or intra blocks, else empty
+  std::vector<int32_t> intra;    // intra blocks' dequantised coefficients (16 per 4x4 block), else empty
+  std::vector<uint8_t> pastMode; // deblocking: the past reference's block modes (a WZ frame's), else empty
+  bool deblock = false;
   uint64_t ticket = 0, pastKey = 0, futKey = 0;
   bool setup = false;
   size_t bytes = 0;
@@ -136,7 +139,9 @@ static size_t job_bytes(const Job &j) {
                    sizeof(MVd) * (size_t)(FS / 64), sizeof(MVd) * (size_t)(FS / 256), 16 * (size_t)(FS / 256),
                    8 * (size_t)(FS / 64), (size_t)j.nbp * (N / 8),
                    sizeof(PlaneState) * (size_t)j.nbp, sizeof(int4) * kMaxCodes * (size_t)j.nbp, 4 * (size_t)N,
-                   (size_t)N, 4 * kMaxCodes * (size_t)j.nbp, (size_t)FS / 64})
+                   (size_t)N, 4 * kMaxCodes * (size_t)j.nbp, (size_t)FS / 64, j.intra.size() * sizeof(int32_t),
+                   j.deblock ? (size_t)FS : 0, j.deblock ? 2 * (size_t)N : 0, j.deblock ? 16 * (size_t)N : 0,
+                   j.pastMode.size()})
     t = align_up(t + b);
   return t;
 }
@@ -157,7 +162,10 @@ static void carve_job(Job &j, uint8_t *base) {
                o_holes = take(16 * (size_t)(FS / 256)), o_mse = take(8 * (size_t)(FS / 64)),
                o_acc = take((size_t)j.nbp * (N / 8)), o_state = take(sizeof(PlaneState) * (size_t)j.nbp),
                o_recd = take(sizeof(int4) * kMaxCodes * (size_t)j.nbp), o_llr = take(4 * (size_t)N),
-               o_carry = take(N), o_bcase = take(4 * kMaxCodes * (size_t)j.nbp), o_mode = take((size_t)FS / 64);
+               o_carry = take(N), o_bcase = take(4 * kMaxCodes * (size_t)j.nbp), o_mode = take((size_t)FS / 64),
+               o_intra = take(j.intra.size() * sizeof(int32_t)), o_dbf = take(j.deblock ? (size_t)FS : 0),
+               o_dbm = take(j.deblock ? 2 * (size_t)N : 0), o_dbb = take(j.deblock ? 16 * (size_t)N : 0),
+               o_pm = take(j.pastMode.size());
   j.mem = base;
   j.bytes = off;
   FrameDev &f = j.fd;
@@ -173,6 +181,11 @@ static void carve_job(Job &j, uint8_t *base) {
   j.d_carry = base + o_carry;
   j.d_bcase = (int *)(base + o_bcase);
   f.mode = j.mode.empty() ? nullptr : base + o_mode;
+  f.intraCoeffs = j.intra.empty() ? nullptr : (int *)(base + o_intra);
+  f.dbFlag = j.deblock ? base + o_dbf : nullptr;
+  f.dbEdgeMode = j.deblock ? base + o_dbm : nullptr;
+  f.dbEdgeBs = j.deblock ? (double *)(base + o_dbb) : nullptr;
+  f.pastMode = j.pastMode.empty() ? nullptr : base + o_pm;
 }
 
 }  // namespace dvc
@@ -498,6 +511,21 @@ static int admit(dvc_engine *e, Job *j, hipStream_t s, std::vector<JobInitDev> &
   f.fut = kf.full; f.fut_lp = kf.lp; f.fut_half = kf.half;
   if (!e->deviceBuffers) DVC_HIP(hipMemcpyAsync(j->d_acc, j->acc.data(), j->acc.size(), hipMemcpyHostToDevice, s));
   if (f.mode) DVC_HIP(hipMemcpyAsync((void *)f.mode, j->mode.data(), j->mode.size(), hipMemcpyHostToDevice, s));
+  if (f.intraCoeffs)
+    DVC_HIP(hipMemcpyAsync((void *)f.intraCoeffs, j->intra.data(), j->intra.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  if (f.pastMode)
+    DVC_HIP(hipMemcpyAsync((void *)f.pastMode, j->pastMode.data(), j->pastMode.size(), hipMemcpyHostToDevice, s));
+  f.deblock = j->deblock ? 1 : 0;
+  if (j->deblock) {
+    // deblock.cpp:361-365 and 56-58: coefficient weights Qtable[i] / sum, the
+    // filter's QP (fixed at its first call to the stream's IntraQP) and 2^(QP/6)
+    int sum = 0;
+    for (int i = 0; i < 16; i++) sum += qtable_entry(sp.qindex, i);
+    for (int i = 0; i < 16; i++) f.dbWeight[i] = (double)qtable_entry(sp.qindex, i) / (double)sum;
+    f.intraQp = sp.intra_qp;
+    f.pow2qp = pow(2, ((double)sp.intra_qp / 6.0));
+    f.pastIsWz = sp.past_is_wz ? 1 : 0;
+  }
   // k_job_init: no decoded bits; per-bitplane rate-search state with the
   // start level from the entropy (computed on the device), no accepted code
   // and no record of any code run; ParallelMode 1's carried decisions 0xAA
@@ -683,6 +711,11 @@ static int lane_step(dvc_engine *e, Lane &ln) {
   if (!setup.empty()) {
     const int nf = (int)setup.size();
     TIMED_LAUNCH(c, "create_si", s, launch_create_si(ln.d_frames, L3, nf, maxMB, ct, s));
+    // intra blocks (intraRecon, DVC_Decoder.cpp:43-47) and their copy into the
+    // side information (:56), before the over-complete transform reads it
+    bool anyIntra = false;
+    for (int i : setup) anyIntra |= !ln.inflight[i]->intra.empty();
+    if (anyIntra) TIMED_LAUNCH(c, "intra", s, launch_intra(ln.d_frames, L3, nf, maxN / 4, s));
     TIMED_LAUNCH(c, "noise_param", s, launch_noise_param(ln.d_frames, L3, nf, maxN, s));
     TIMED_LAUNCH(c, "oct", s, launch_oct(ln.d_frames, L3, nf, maxPos, 1, s));
     TIMED_LAUNCH(c, "smf_init", s, launch_smf_init(ln.d_frames, L3, nf, maxN, s));
@@ -768,6 +801,17 @@ static int lane_step(dvc_engine *e, Lane &ln) {
     bool anySkip = false;
     for (int i : fin) anySkip |= !ln.inflight[i]->mode.empty();
     if (anySkip) DVC_HIP(launch_skip_copy(ln.d_frames, L4, (int)fin.size(), maxN / 4, s));
+    // the deblocking filter (DVC_Decoder.cpp:429-435) on the finished frame
+    bool anyDeblock = false;
+    int dw = 0, dh = 0;
+    for (int i : fin) {
+      const Job &j = *ln.inflight[i];
+      if (!j.deblock) continue;
+      anyDeblock = true;
+      dw = std::max(dw, j.W);
+      dh = std::max(dh, j.H);
+    }
+    if (anyDeblock) TIMED_LAUNCH(c, "deblock", s, launch_deblock(ln.d_frames, L4, (int)fin.size(), dw, dh, s));
     DVC_HIP(launch_copy_segments(ln.d_segs, nseg, s));
     DVC_HIP(hipMemcpyAsync(ln.h_out, ln.d_out, outUsed, hipMemcpyDeviceToHost, s));
     DVC_HIP(hipStreamSynchronize(s));
@@ -878,16 +922,24 @@ static int engine_submit(dvc_engine *e, const dvc_wz_frame *sp, uint64_t pastKey
   j->W = sp->width; j->H = sp->height; j->N = j->W * j->H / 16;
   j->acRange.assign(sp->ac_range, sp->ac_range + 16);
   j->bands = band_plan(sp->qindex, j->acRange.data(), &j->bitrate0);
+  const int n8 = j->W * j->H / 64;
   if (sp->block_mode) {
-    const int n8 = j->W * j->H / 64;
-    bool anyNonWz = false;
+    bool anyNonWz = false, anyIntra = false;
     for (int k = 0; k < n8; k++) {
-      DVC_CHECK(sp->block_mode[k] == 0 || sp->block_mode[k] == 2, DVC_EINVAL,
-                sp->block_mode[k] == 1 ? "WZ frame: intra blocks are not supported (they need the intra bitstream)"
-                                       : "WZ frame: block modes must be 0 (WZ) or 2 (skip)");
+      DVC_CHECK(sp->block_mode[k] <= 2, DVC_EINVAL, "WZ frame: block modes must be 0 (WZ), 1 (intra) or 2 (skip)");
       anyNonWz |= sp->block_mode[k] != 0;
+      anyIntra |= sp->block_mode[k] == 1;
     }
     if (anyNonWz) j->mode.assign(sp->block_mode, sp->block_mode + n8);
+    if (anyIntra) {
+      DVC_CHECK(sp->intra_coeffs, DVC_EINVAL, "WZ frame: intra blocks need their coefficients (intra_coeffs)");
+      j->intra.assign(sp->intra_coeffs, sp->intra_coeffs + (size_t)16 * j->N);
+    }
+  }
+  if (sp->deblocking) {
+    DVC_CHECK(sp->intra_qp >= 0 && sp->intra_qp <= 51, DVC_EINVAL, "WZ frame: intra_qp must be 0..51");
+    j->deblock = true;
+    if (sp->past_is_wz && sp->past_block_mode) j->pastMode.assign(sp->past_block_mode, sp->past_block_mode + n8);
   }
   if (sp->block_mode || sp->mode_bits) {
     DVC_CHECK(sp->mode_bits >= 2, DVC_EINVAL, "WZ frame: mode_bits counts at least the two mode flags");

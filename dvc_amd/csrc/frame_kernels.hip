@@ -789,6 +789,7 @@ __global__ void k_reconstruct(const FrameDev *F, const int *list) {
     v = side + round_d(((r + 1.0 / a) * er - (d + 1.0 / a) * ed) / (2.0 - (er + ed)));
   }
   f.sideTrans[ci] = v;
+  if (f.dbFlag) f.dbFlag[ci] = v != side;  // deblockFlag: reconstruction moved the coefficient
 }
 
 // inverseTransform (transform.cpp:234-273).

@@ -282,3 +282,13 @@ int intra_parse_frame(const uint8_t *data, size_t len, size_t *bitpos, const uin
 }
 
 }  // namespace dvc
+
+extern "C" int dvc_intra_parse(const uint8_t *data, size_t len, size_t *bit_offset, const uint8_t *block_mode,
+                               int width, int height, int intra_qp, int32_t *coeffs) {
+  using namespace dvc;
+  DVC_CHECK((data || len == 0) && bit_offset && block_mode && coeffs, DVC_EINVAL, "dvc_intra_parse: NULL argument");
+  DVC_CHECK((width == 176 && height == 144) || (width == 352 && height == 288), DVC_EINVAL,
+            "dvc_intra_parse: size must be QCIF or CIF");
+  DVC_CHECK(intra_qp >= 0 && intra_qp <= 51, DVC_EINVAL, "dvc_intra_parse: intra_qp must be 0..51");
+  return intra_parse_frame(data, len, bit_offset, block_mode, width, height, intra_qp, coeffs);
+}

@@ -13,6 +13,8 @@
 namespace dvc {
 int jbig_decode_bie(const uint8_t *bie, size_t len, std::vector<uint8_t> &img, uint32_t &xd, uint32_t &yd,
                     size_t &consumed);
+int intra_parse_frame(const uint8_t *data, size_t len, size_t *bitpos, const uint8_t *mode, int W, int H, int qp,
+                      int32_t *coeffs);
 }
 
 struct dvc_wz_stream {
@@ -22,7 +24,9 @@ struct dvc_wz_stream {
     int acRange[16];
     int skipMode, intraMode, modeBits;
     std::vector<uint8_t> mode, crc, acc;
+    std::vector<int32_t> intra;  // parsed intra coefficients (dvc_wz_stream_attach_intra)
   };
+  bool intraAttached = false;
   std::vector<Frame> frames;
 };
 
@@ -166,9 +170,40 @@ extern "C" int dvc_wz_stream_info_get(const dvc_wz_stream *s, dvc_wz_stream_info
   return DVC_OK;
 }
 
+extern "C" int dvc_wz_stream_attach_intra(dvc_wz_stream *s, const uint8_t *data, size_t len) {
+  DVC_CHECK(s && (data || len == 0), DVC_EINVAL, "dvc_wz_stream_attach_intra: bad argument");
+  DVC_CHECK(!s->intraAttached, DVC_ESTATE, "dvc_wz_stream_attach_intra: already attached");
+  const dvc_wz_stream_info &in = s->info;
+  size_t bit = 0;  // IntraBS.frame_bitoffset runs on from frame to frame
+  for (auto &f : s->frames) {
+    if (f.intraMode != 1) continue;
+    f.intra.assign((size_t)(in.width / 4) * (in.height / 4) * 16, 0);
+    const size_t before = bit;
+    const int rc = intra_parse_frame(data, len, &bit, f.mode.data(), in.width, in.height, in.intra_qp, f.intra.data());
+    if (rc) return rc;
+    DVC_CHECK(bit <= len * 8, DVC_EINVAL, "WZ bitstream: the intra bitstream ends inside a frame's blocks");
+    f.modeBits += (int)(bit - before);
+  }
+  s->intraAttached = true;
+  return DVC_OK;
+}
+
 extern "C" int dvc_wz_stream_frame_get(const dvc_wz_stream *s, int k, dvc_wz_stream_frame *f) {
   DVC_CHECK(s && f && k >= 0 && k < (int)s->frames.size(), DVC_EINVAL, "dvc_wz_stream_frame_get: bad argument");
   const dvc_wz_stream::Frame &x = s->frames[k];
+  bool intraBlocks = false;
+  for (uint8_t m : x.mode) intraBlocks |= m == kIntraB;
+  DVC_CHECK(!intraBlocks || s->intraAttached, DVC_ESTATE,
+            "dvc_wz_stream_frame_get: the frame has intra blocks; attach the intra bitstream first");
+  f->intra_coeffs = intraBlocks ? x.intra.data() : nullptr;
+  // the past reference: a WZ frame of the stream when one has that index
+  f->past_is_wz = 0;
+  f->past_block_mode = nullptr;
+  for (const auto &y : s->frames)
+    if (y.cur == x.past) {
+      f->past_is_wz = 1;
+      f->past_block_mode = y.mode.data();
+    }
   f->past_idx = x.past; f->cur_idx = x.cur; f->fut_idx = x.fut;
   memcpy(f->ac_range, x.acRange, sizeof(f->ac_range));
   f->skip_mode = x.skipMode;

@@ -236,8 +236,13 @@ class WZFrame:
     crc: np.ndarray          # uint8[nbp]
     acc: np.ndarray          # uint8[nbp, N/8]
     bitplanes: np.ndarray    # uint8[nbp, N] (encoder side, for checks)
-    block_mode: Optional[np.ndarray] = None  # uint8[W/8 * H/8] BlockModeFlag (0 WZ, 2 skip), None: all WZ
+    block_mode: Optional[np.ndarray] = None  # uint8[W/8 * H/8] BlockModeFlag (0 WZ, 1 intra, 2 skip), None: all WZ
     mode_bits: int = 0       # skipBlockBits + intraBlockBits (0: both modes off, i.e. 2)
+    intra_coeffs: Optional[np.ndarray] = None  # int32[N, 16] parsed intra coefficients (dvc_wz_stream_attach_intra)
+    deblocking: int = 0      # DeblockingMode
+    intra_qp: int = 0        # IntraQP (the deblocking filter's QP)
+    past_is_wz: int = 0      # the past reference is a WZ frame
+    past_block_mode: Optional[np.ndarray] = None  # its block modes
 
 
 SKIP_BLOCK = 2  # BlockModeFlag value (defines.h:17)

@@ -217,10 +217,22 @@ typedef struct {
    * DVC_Decoder.cpp:131); 0 means 2 (both modes off). */
   const uint8_t *block_mode;
   int mode_bits;
+  /* Intra blocks (block_mode 1; intraRecon, intra.cpp:240-258): their
+   * dequantised coefficients as dvc_wz_stream_frame gives them, 16 per 4x4
+   * block (row-major, W/4*H/4 blocks in raster order, zero outside intra
+   * blocks); required when block_mode has intra blocks, else NULL. */
+  const int32_t *intra_coeffs;
+  /* Deblocking (DeblockingMode, DVC_Decoder.cpp:429-435): on/off, the
+   * filter's QP (IntraQP), and the past reference's isWZFrame and
+   * BlockModeFlag (NULL: all WZ), which set the strength at skip blocks. */
+  int deblocking;
+  int intra_qp;
+  int past_is_wz;
+  const uint8_t *past_block_mode;
 } dvc_wz_frame;
 
-/* decodeGOP's per-WZ-frame body (DVC_Decoder.cpp:26-474; skip blocks per
- * block_mode, intra blocks and deblocking off) for n frames.  batched = 0: frames one after another with the
+/* decodeGOP's per-WZ-frame body (DVC_Decoder.cpp:26-474, with skip and
+ * intra blocks per block_mode and deblocking per `deblocking`) for n frames.  batched = 0: frames one after another with the
  * sequential rate search per bitplane (the plain drop-in of ParallelMode 2);
  * batched = 1: all frames advance together, one bitplane per step, with the
  * speculative persistent LDPCA launch.  Outputs are identical in both modes
@@ -249,6 +261,9 @@ typedef struct {
   const uint8_t *block_mode;  /* W/8 * H/8 BlockModeFlag bytes (0 WZ, 1 intra, 2 skip), owned by the stream */
   const uint8_t *crc;         /* [nbitplanes] */
   const uint8_t *acc;         /* [nbitplanes * N/8] accumulated syndromes */
+  const int32_t *intra_coeffs;  /* dvc_wz_frame::intra_coeffs once the intra bitstream is attached, else NULL */
+  int past_is_wz;             /* the past reference is itself a WZ frame of the stream */
+  const uint8_t *past_block_mode;  /* its block modes, or NULL */
 } dvc_wz_stream_frame;
 /* Parses the whole stream (header, then every GOP's WZ frames in readGOP's
  * hierarchical order, which is decodeGOP's decoding order). */
@@ -258,6 +273,20 @@ int dvc_wz_stream_info_get(const dvc_wz_stream *s, dvc_wz_stream_info *info);
  * valid until dvc_wz_stream_close. */
 int dvc_wz_stream_frame_get(const dvc_wz_stream *s, int k, dvc_wz_stream_frame *frame);
 int dvc_wz_stream_close(dvc_wz_stream *s);
+/* The intra bitstream of the stream's frames with intra blocks
+ * (readIntraBitstream, intraBitstream.cpp:10-35): parses every such frame's
+ * CAVLC blocks in decoding order (read_one_block, intra.cpp:185-215, and
+ * readCoeff4x4_CAVLC, vlc.cpp:613-761), which fills their intra_coeffs and
+ * adds the bits each frame used to its mode_bits (DVC_Decoder.cpp:43-47). */
+int dvc_wz_stream_attach_intra(dvc_wz_stream *s, const uint8_t *data, size_t len);
+/* One frame's intra blocks from the intra bitstream, as intraRecon's parsing
+ * half does it (read_one_block, intra.cpp:185-215, over readCoeff4x4_CAVLC,
+ * vlc.cpp:613-761): the CAVLC blocks of every 4x4 block whose 8x8 block is
+ * intra in `block_mode`, in raster order, from bit *bit_offset (advanced
+ * past them, as IntraBS.frame_bitoffset is), dequantised with intra_qp into
+ * coeffs (16 ints per 4x4 block, W/4*H/4 blocks; zero elsewhere). */
+int dvc_intra_parse(const uint8_t *data, size_t len, size_t *bit_offset, const uint8_t *block_mode, int width,
+                    int height, int intra_qp, int32_t *coeffs);
 
 /* One block-mode map as readGOP decodes it (simple_JBIG_dec + ByteToBitmap,
  * sJBIG.cpp:24-103): `data` holds the map's JBIG data (no header: the header

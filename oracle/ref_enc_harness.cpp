@@ -14,6 +14,11 @@
 #include "ReadLdpcaLadderFile_LdpcaEncoder.h"
 #include "skipBlock.h"
 #include "sJBIG.h"
+#include "DVC_Encoder.h"
+#include "frame.h"
+#include "wzbitstream.h"
+#include "intraBitstream.h"
+#include "modeSelection.h"
 
 extern "C" {
 int refenc_setup(int W, int H, int qindex, const char *ladder) {
@@ -105,5 +110,46 @@ int refenc_encode_skip(const unsigned char *frame, const unsigned char *ref, int
   const int mode = e.skipMode;
   free(bp); free(trans); free(e.skipBlockFlag); free(e.intraBlockFlag);
   return mode;
+}
+
+// The reference encoder's main loop (DvcEnc main.cpp:20-100) on frames in
+// memory: every GOP through encodeGOP with the given options, the key frames'
+// reconstructions being the originals; writes the WZ bitstream to wz_path and,
+// with intra mode on, the intra bitstream to intra_path.  Returns the number
+// of WZ frames.
+int refenc_run(const unsigned char *frames, int nframes, int W, int H, int q, int gop, int skip, int intra,
+               int deblock, int intra_qp, const char *ladder, const char *wz_path, const char *intra_path) {
+  refenc_setup(W, H, q, ladder);  // geometry, tables, ladder, transform
+  codingOption.FramesToBeEncoded = nframes;
+  codingOption.GOP = (unsigned char)gop;
+  codingOption.EnableSkipBlock = skip;
+  codingOption.EnableIntraMode = intra;
+  codingOption.IntraQP = (unsigned char)intra_qp;
+  codingOption.DeblockingMode = (unsigned char)deblock;
+  Block44Num = W * H / 16; Block88Num = W * H / 64;
+  initTransform();  // again, now that the intra options are set (it builds the intra tables only then)
+  initFrameBuffer();
+  for (int i = 0; i < nframes; i++) {
+    memcpy(oriFrames[i], frames + (size_t)i * FrameSize, FrameSize);
+    memcpy(reconsFrames[i], frames + (size_t)i * FrameSize, FrameSize);
+  }
+  if (intra) {
+    IntraFp = fopen(intra_path, "wb+");
+    if (!IntraFp) return -1;
+    intrap.thita = 0.0;
+    intrap.kappa = 0.0;
+    init_IntraBuffer();
+  }
+  int wzCount = 0;
+  for (int i = 0; i < nframes; i += gop) {
+    int end = i + gop;
+    if (end >= nframes) end = nframes - 1;
+    encodeGOP(i, end, &wzCount);
+  }
+  writeWZbitstream((char *)wz_path);
+  if (intra) free_IntraBuffer();  // flushes the last partial byte and closes IntraFp
+  freeFrameBuffer();
+  Kmatrix = NULL;  // freeFrameBuffer frees it, but only intra-mode runs allocate it again
+  return wzCount;
 }
 }

@@ -28,6 +28,7 @@
 #include "jbig.h"
 #include "sJBIG.h"
 #include "wzbitstream.h"
+#include "intraBitstream.h"
 #include <vector>
 
 // Iteration counter that the reference's beliefPropagation_OpenMP increments
@@ -361,6 +362,18 @@ int ref_read_wz_bitstream(const char *path, int *hdr) {
   hdr[3] = codingOption.IntraQP; hdr[4] = codingOption.GOP; hdr[5] = codingOption.FramesToBeDecoded;
   hdr[6] = codingOption.DeblockingMode; hdr[7] = QbitsNumPerBlk;
   return 0;
+}
+// A whole encoded sequence as the reference's main.cpp loads it
+// (main.cpp:36-42): the WZ bitstream, then the intra bitstream when a frame
+// uses intra blocks.  Keeps the ParallelMode and thread count ref_setup set.
+// Returns IntraModeExist.
+int ref_load_stream(const char *wz_path, const char *intra_path, int *hdr) {
+  const int mode = codingOption.ParallelMode, threads = codingOption.UseCoreNum;
+  ref_read_wz_bitstream(wz_path, hdr);
+  codingOption.ParallelMode = mode;
+  codingOption.UseCoreNum = threads;
+  if (codingOption.IntraModeExist) readIntraBitstream((char *)intra_path);
+  return codingOption.IntraModeExist;
 }
 int ref_wz_frame(int idx, int *acRange, int *modes, unsigned char *flags, unsigned char *crc, unsigned char *acc) {
   EncodedFrame *e = &encodedFrames[idx];  // a WZ frame's index (key frames' entries are not set)

@@ -116,3 +116,23 @@ def test_wz_stream_rejects_bad_input(lib):
         lib.read_wz_stream(bytes([2, 4, 0, 4, 0, 9, 0]))  # resolution byte 2
     with pytest.raises(RuntimeError):
         lib.read_wz_stream(bytes([0, 4, 0, 4, 0, 9, 0, 1, 2]))  # truncated first frame
+
+
+def test_intra_stream_mode_bits_match_reference(lib):
+    """The CAVLC parse of every intra frame uses exactly the bits the
+    reference's readCoeff4x4_CAVLC used: each frame's mode bits (skip map +
+    intra map + CAVLC bits) equal skipBlockBits + intraBlockBits after the
+    reference's decodeGOP (tests/golden/intra_streams.json)."""
+    import base64
+    d = json.loads((GOLD / "intra_streams.json").read_text())
+    assert any(f["intra_blocks"] for c in d["cases"] for f in c["decoded"])
+    for c in d["cases"]:
+        info, frames = lib.read_wz_stream(base64.b64decode(c["wz_stream"]), base64.b64decode(c["intra_stream"]))
+        assert (info["deblocking"], info["intra_qp"]) == (c["header"][6], c["header"][3])
+        gold = {f["cur"]: f for f in c["decoded"]}
+        for f in frames:
+            g = gold[f["idx"][1]]
+            wz = f["wz"]
+            assert wz.mode_bits == g["skip_block_bits"] + g["intra_block_bits"], (c["name"], f["idx"])
+            n_intra = 0 if wz.block_mode is None else int((wz.block_mode == 1).sum())
+            assert n_intra == g["intra_blocks"] and (wz.intra_coeffs is not None) == (n_intra > 0)
