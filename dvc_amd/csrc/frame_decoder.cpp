@@ -617,12 +617,14 @@ static int lane_step(dvc_engine *e, Lane &ln) {
   int rc;
   const int n = (int)ln.inflight.size();
   if (n == 0) return DVC_OK;
-  int maxN = 0, maxMB = 0, maxPos = 0;
+  int maxN = 0, maxMB = 0, maxPos = 0, maxBw = 0, maxBh = 0;
   for (int i = 0; i < n; i++) {
     Job *j = ln.inflight[i];
     maxN = std::max(maxN, j->N);
     maxMB = std::max(maxMB, j->W * j->H / 256);
     maxPos = std::max(maxPos, (j->W - 3) * (j->H - 3));
+    maxBw = std::max(maxBw, j->W / 4);
+    maxBh = std::max(maxBh, j->H / 4);
   }
   CostTables ct{c->d_costFme, c->d_costUsi};
   // ---- the step's frame lists (all known before any kernel runs)
@@ -784,7 +786,7 @@ static int lane_step(dvc_engine *e, Lane &ln) {
       TIMED_LAUNCH(c, "update_si", s, launch_update_si(ln.d_frames, L2, ne, maxN, ct, s));
       TIMED_LAUNCH(c, "oct", s, launch_oct(ln.d_frames, L2, ne, maxPos, 0, s));
       TIMED_LAUNCH(c, "update_smf", s, launch_update_smf(ln.d_frames, L2, ne, maxN, s));
-      TIMED_LAUNCH(c, "update_si_ml", s, launch_update_si_ml(ln.d_frames, L2, ne, maxN, 0, s));
+      TIMED_LAUNCH(c, "update_si_ml", s, launch_update_si_ml(ln.d_frames, L2, ne, maxBw, maxBh, s));
       for (int i : ending) {
         Job &j = *ln.inflight[i];
         j.bandPos++;

@@ -101,6 +101,6 @@ hipError_t launch_inverse_transform(const FrameDev *F, const int *list, int n, i
 hipError_t launch_update_si(const FrameDev *F, const int *list, int n, int maxN, CostTables ct, hipStream_t s);
 hipError_t launch_smf_init(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s);
 hipError_t launch_update_smf(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s);
-hipError_t launch_update_si_ml(const FrameDev *F, const int *list, int n, int maxN, int bandStart, hipStream_t s);
+hipError_t launch_update_si_ml(const FrameDev *F, const int *list, int n, int maxBw, int maxBh, hipStream_t s);
 
 }  // namespace dvc

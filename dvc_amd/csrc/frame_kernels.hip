@@ -122,40 +122,64 @@ __global__ void k_keyframe_batch(const KeyFrameJob *jobs) {
 // forwardME (sideInfoCreation.cpp:61-118): per macroblock of the future
 // low-pass frame, exhaustive search of the past low-pass frame over the
 // clipped +-32 window; cost SAD*(1 + K*|v|) in double, first minimum in raster
-// order.  One workgroup per macroblock; the window is staged in LDS.
+// order.  One workgroup per macroblock.  The window (up to 80 x 80 bytes) is
+// staged in LDS as 32-bit words and the macroblock's 64 words sit in scalar
+// registers; lane l of every wave takes candidate column l, the four waves
+// take every fourth candidate row, and each candidate's SAD is 64 v_sad_u8 on
+// window words byte-aligned with v_alignbyte (exact integer SADs, as the
+// reference's integer sums are).  The 65th column of a full window is spread
+// over the threads afterwards; ties go to the least raster index.
+#define FME_LW 21  // staged words per window row: 20 + one zero word
 __global__ __launch_bounds__(256) void k_forward_me(const FrameDev *F, const int *list, CostTables ct) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int bw = f.W / 16, nmb = bw * (f.H / 16), mb = blockIdx.x;
   if (mb >= nmb) return;
   const int W = f.W, top = (mb / bw) * 16, left = (mb % bw) * 16;
   const int sT = imax(0, top - 32), sB = imin(f.H - 16, top + 32);
-  const int sL = imax(0, left - 32), sR = imin(W - 16, left + 32);
-  const int ww = sR - sL + 1, wh = sB - sT + 1, winW = ww + 15, winH = wh + 15;
-  __shared__ uint8_t fb[256];
-  __shared__ uint8_t win[80 * 80];
+  const int sL = imax(0, left - 32), sR = imin(W - 16, left + 32);  // sL is a multiple of 16
+  const int ww = sR - sL + 1, wh = sB - sT + 1, winH = wh + 15, nw = (ww + 15) >> 2;  // nw <= 20
+  __shared__ uint32_t win[80 * FME_LW];
+  __shared__ uint32_t s_fb[64];
   __shared__ double s_cost[4];
   __shared__ int s_idx[4];
-  const int tid = threadIdx.x;
-  fb[tid] = f.fut_lp[(top + (tid >> 4)) * W + left + (tid & 15)];
-  for (int i = tid; i < winW * winH; i += 256) win[i] = f.past_lp[(sT + i / winW) * W + sL + i % winW];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  if (tid < 64) s_fb[tid] = *(const uint32_t *)(f.fut_lp + (size_t)(top + (tid >> 2)) * W + left + 4 * (tid & 3));
+  for (int i = tid; i < winH * FME_LW; i += 256) {
+    const int r = i / FME_LW, w = i - r * FME_LW;
+    win[i] = w < nw ? *(const uint32_t *)(f.past_lp + (size_t)(sT + r) * W + sL + 4 * w) : 0u;
+  }
   __syncthreads();
+  uint32_t fb[64];
+#pragma unroll
+  for (int k = 0; k < 64; k++) fb[k] = __builtin_amdgcn_readfirstlane(s_fb[k]);
+  auto sad_at = [&](int oy, int ox) -> uint32_t {
+    const uint32_t *wr = win + oy * FME_LW + (ox >> 2);
+    const int sh = ox & 3;
+    uint32_t sad = 0;
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        sad = __builtin_amdgcn_sad_u8(fb[4 * m + q],
+                                      __builtin_amdgcn_alignbyte(wr[m * FME_LW + q + 1], wr[m * FME_LW + q], sh), sad);
+    return sad;
+  };
   double best = DBL_MAX;
   int bidx = 0x7fffffff;
-  for (int c = tid; c < ww * wh; c += 256) {
-    const int oy = c / ww, ox = c % ww;
-    int sad = 0;
-    for (int m = 0; m < 16; m++) {
-      const uint8_t *wr = win + (oy + m) * winW + ox;
-      const uint8_t *fr = fb + m * 16;
-#pragma unroll
-      for (int q = 0; q < 16; q++) sad += abs((int)fr[q] - (int)wr[q]);
+  if (lane < ww) {
+    const double *fac = ct.fme + (sL + lane - left + 32);
+    for (int oy = wave; oy < wh; oy += 4) {
+      const double cost = (double)sad_at(oy, lane) * fac[(sT + oy - top + 32) * 65];
+      if (cost < best) { best = cost; bidx = oy * ww + lane; }
     }
-    const int dx = sL + ox - left, dy = sT + oy - top;
-    const double cost = (double)sad * ct.fme[(dy + 32) * 65 + dx + 32];
-    if (cost < best) { best = cost; bidx = c; }
   }
+  if (ww > 64)
+    for (int oy = tid; oy < wh; oy += 256) {
+      const double cost = (double)sad_at(oy, 64) * ct.fme[(sT + oy - top + 32) * 65 + sL + 64 - left + 32];
+      if (cost < best || (cost == best && oy * ww + 64 < bidx)) { best = cost; bidx = oy * ww + 64; }
+    }
   wave_argmin(best, bidx);
-  if ((tid & 63) == 0) { s_cost[tid >> 6] = best; s_idx[tid >> 6] = bidx; }
+  if (lane == 0) { s_cost[wave] = best; s_idx[wave] = bidx; }
   __syncthreads();
   if (tid == 0) {
     for (int w = 1; w < 4; w++)
@@ -542,11 +566,31 @@ __global__ void k_smf_init(const FrameDev *F, const int *list) {
 // Qindex 8, and the small workgroup with a 4 KB table lets a CU hold many
 // blocks at once to hide the LDS and global latencies of each one's phases.
 #define ND_TAB 1024  // table entries (4 KB of LDS); larger distances call expf directly
+#define ND_T 64      // threads per block distribution (one wave)
+// V values k + i*ND_T (i < V) of one block's distribution: sum over the window
+// terms m in order of p_m * tab[value + off_m].  Values at or beyond rmax are
+// computed from the table's padding (never from outside it) and not stored.
+template <int V>
+__device__ __forceinline__ void nd_values(const float *tab, const int2 *ps, int n, int k, int rmax, float *out) {
+  float acc[V];
+#pragma unroll
+  for (int i = 0; i < V; i++) acc[i] = 0.0f;
+  for (int m = 0; m < n; m++) {
+    const int2 t = ps[m];
+    const float pr = __int_as_float(t.x);
+    const float *tk = tab + k + t.y;
+#pragma unroll
+    for (int i = 0; i < V; i++) acc[i] += pr * tk[i * ND_T];
+  }
+#pragma unroll
+  for (int i = 0; i < V; i++)
+    if (k + i * ND_T < rmax) out[k + i * ND_T] = acc[i];
+}
 __global__ __launch_bounds__(64) void k_noise_dist(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int b = blockIdx.x;
   if (b >= f.N || !wz_block(f, b)) return;  // only SIR candidates have a distribution
-  const int tid = threadIdx.x, T = blockDim.x;
+  const int tid = threadIdx.x, T = ND_T;
   const int ow = f.W - 3, index = f.index, range = f.range;
   const SmfWin s = smf_window(f, b);
   const float a = f.alpha[(s.top + (index >> 2)) * f.W + s.left + (index & 3)];
@@ -558,7 +602,9 @@ __global__ __launch_bounds__(64) void k_noise_dist(const FrameDev *F, const int 
   }
   __shared__ int2 s_ps[81];  // (probability bits, SI coefficient) of the kept window terms
   __shared__ int s_n, s_maxabs;
-  __shared__ float s_tab[ND_TAB];
+  // table entries [0, 2D] with 2D + 1 <= ND_TAB, and 3 * ND_T of padding that a
+  // lane's values beyond rmax (at most 3 * ND_T further on) may read
+  __shared__ float s_tab[ND_TAB + 3 * ND_T];
   if (tid < 64) {
     // wave 0 compacts the window's non-zero terms in raster order
     const bool fin = isfinite(a);
@@ -599,21 +645,14 @@ __global__ __launch_bounds__(64) void k_noise_dist(const FrameDev *F, const int 
     }
     for (int m = tid; m < n; m += T) s_ps[m].y = D - range - s_ps[m].y;
     __syncthreads();
-    // two values per lane (k0 and k0 + T) share each term's read and carry
-    // independent sums; each value still adds its terms in window order
-    for (int k0 = tid; k0 < rmax; k0 += 2 * T) {
-      const bool two = k0 + T < rmax;
-      const int k1 = two ? k0 + T : k0;  // a lane without a second value repeats the first
-      float s0 = 0.0f, s1 = 0.0f;
-      for (int m = 0; m < n; m++) {
-        const int2 ps = s_ps[m];
-        const float pr = __int_as_float(ps.x);
-        s0 += pr * s_tab[k0 + ps.y];
-        s1 += pr * s_tab[k1 + ps.y];
-      }
-      out[k0] = s0;
-      if (two) out[k1] = s1;
-    }
+    // a lane's values k, k + T, ... share each term's offset and carry
+    // independent sums; each value still adds its terms in window order.  The
+    // number of values per lane is the same on every lane (groups of up to 4).
+    int k = tid, left = (rmax + T - 1) / T;
+    for (; left >= 4; left -= 4, k += 4 * T) nd_values<4>(s_tab, s_ps, n, k, rmax, out);
+    if (left == 3) nd_values<3>(s_tab, s_ps, n, k, rmax, out);
+    else if (left == 2) nd_values<2>(s_tab, s_ps, n, k, rmax, out);
+    else if (left == 1) nd_values<1>(s_tab, s_ps, n, k, rmax, out);
   } else {
     const int need = D + 1;
     const int tl = imin(need, ND_TAB);
@@ -817,15 +856,22 @@ __global__ void k_inverse_transform(const FrameDev *F, const int *list) {
 // One wave per block, 4 horizontally adjacent blocks per workgroup (the frame
 // width is a multiple of 16, so the 4 share a block row).  The union of their
 // search windows (up to 68 rows x 80 bytes of each key frame) is staged in LDS
-// as 32-bit words; lane l sweeps candidate column sL+l down the window,
-// forming each candidate's 4-pixel rows with v_alignbyte and its SAD with
-// v_sad_u8 (exact integer SADs, as the reference's double sums of integer
-// differences are).  The 65th column of a full window is spread over the lanes
-// afterwards.  Ties resolve to the first candidate in raster order.
+// as 32-bit words, each wave staging every fourth row of both frames, the two
+// frames' rows interleaved so that one address reaches both; lane l
+// sweeps candidate column sL+l down the window, forming each candidate's
+// 4-pixel rows with v_alignbyte and its SAD with v_sad_u8 (exact integer SADs,
+// as the reference's double sums of integer differences are).  A lane keeps
+// its best cost and the row it was found on (the row is wave-uniform, so the
+// select takes it from a scalar register); the rows above the block and the
+// rows from the block down run as two loops, so the cost factor's LDS address
+// steps by a constant.  The 65th column of a full window is spread over the
+// lanes afterwards.  Ties resolve to the first candidate in raster order: the
+// wave's minimum cost first, then the least (row, column) key among the lanes
+// that hold it.
 #define USI_LW 21   // staged words per window row: 20 + one zero word
 #define USI_ROWS 68
 __device__ __forceinline__ uint32_t usi_word(const uint32_t *R, int r, int j, int sh) {
-  return __builtin_amdgcn_alignbyte(R[r * USI_LW + j + 1], R[r * USI_LW + j], sh);
+  return __builtin_amdgcn_alignbyte(R[r * 2 * USI_LW + j + 1], R[r * 2 * USI_LW + j], sh);
 }
 __device__ __forceinline__ uint32_t sad4(const uint32_t *bk, uint32_t r0, uint32_t r1, uint32_t r2, uint32_t r3) {
   uint32_t s = __builtin_amdgcn_sad_u8(bk[0], r0, 0u);
@@ -836,28 +882,40 @@ __device__ __forceinline__ uint32_t sad4(const uint32_t *bk, uint32_t r0, uint32
 __device__ __forceinline__ void argmin_take(double c, int i, double &best, int &bi) {
   if (c < best || (c == best && i < bi)) { best = c; bi = i; }
 }
+// (cost, key) -> the wave's least cost and, among the lanes holding it, the
+// least key; every lane gets both.  Costs are finite or DBL_MAX, never NaN.
+__device__ __forceinline__ void wave_argmin_key(double &cost, int &key) {
+  double m = cost;
+  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o));
+  int k = cost == m ? key : 0x7fffffff;
+  for (int o = 32; o > 0; o >>= 1) k = imin(k, __shfl_xor(k, o));
+  cost = m;
+  key = k;
+}
 __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int *list, CostTables ct) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int b0 = blockIdx.x * 4;
   if (b0 >= f.N) return;  // uniform over the workgroup
   const int W = f.W, H = f.H, W2 = 2 * W, H2 = 2 * H, bw = W / 4;
-  __shared__ uint32_t s_win[2][USI_ROWS * USI_LW];
+  __shared__ uint32_t s_win[USI_ROWS * 2 * USI_LW];  // [row][past, future][word]
   __shared__ double s_fac[33 * 33];
   // union window of the 4 blocks: rows [wT, wB], byte columns [cL, cR]
   const int row0 = (b0 / bw) * 4, left0 = (b0 % bw) * 4;
   const int wT = imax(0, row0 - 32), wB = imin(H - 4, row0 + 32) + 3;
   const int cL = imax(0, left0 - 32) & ~3, cR = imin(W - 4, left0 + 12 + 32) + 3;
-  const int nrows = wB - wT + 1, nw = (cR + 1 - cL) >> 2;
-  for (int t = threadIdx.x; t < 2 * nrows * USI_LW; t += 256) {
-    const int ref = t / (nrows * USI_LW), rem = t - ref * nrows * USI_LW;
-    const int r = rem / USI_LW, w = rem - r * USI_LW;
-    const uint8_t *src = ref ? f.fut : f.past;
-    s_win[ref][r * USI_LW + w] = w < nw ? *(const uint32_t *)(src + (size_t)(wT + r) * W + cL + 4 * w) : 0u;
-  }
-  for (int t = threadIdx.x; t < 33 * 33; t += 256) {
-    const int dy = t / 33, dx = t - dy * 33;
-    s_fac[t] = ct.usi[(dy + 32) * 65 + dx + 32];
+  const int nrows = wB - wT + 1, nw = (cR + 1 - cL) >> 2;  // nrows <= 68, nw <= 20
+  {
+    // lanes 0..20 stage a past-frame row's words, lanes 32..52 the future's
+    const int half = lane >> 5, w = lane & 31;
+    if (w < USI_LW) {
+      const uint8_t *src = half ? f.fut : f.past;
+      uint32_t *dst = s_win + half * USI_LW;
+      for (int r = wave; r < nrows; r += 4)
+        dst[r * 2 * USI_LW + w] = w < nw ? *(const uint32_t *)(src + (size_t)(wT + r) * W + cL + 4 * w) : 0u;
+    }
+    for (int dy = wave; dy < 33; dy += 4)
+      if (lane < 33) s_fac[dy * 33 + lane] = ct.usi[(dy + 32) * 65 + lane + 32];
   }
   __syncthreads();
   const int b = b0 + wave;
@@ -868,31 +926,37 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
   uint32_t bk[4];
 #pragma unroll
   for (int m = 0; m < 4; m++) bk[m] = *(const uint32_t *)(f.recons + (size_t)(top + m) * W + left);
-  const uint32_t *P = s_win[0], *Fu = s_win[1];
+  const uint32_t *P = s_win, *Fu = s_win + USI_LW;
   double pBest = DBL_MAX, fBest = DBL_MAX;
-  int pIdx = 0x7fffffff, fIdx = 0x7fffffff;
+  int pKey = 0x7fffffff, fKey = 0x7fffffff;  // (row - sT) * 128 + column - sL
   if (lane < ww) {
     const int x = sL + lane, xo = x - cL, j = xo >> 2, sh = xo & 3;
-    const double *facRow = s_fac + abs(x - left);
-    int r = sT - wT;
+    const double *facCol = s_fac + abs(x - left);
+    int r = sT - wT, pRow = 0, fRow = 0;
     uint32_t p0 = usi_word(P, r, j, sh), p1 = usi_word(P, r + 1, j, sh), p2 = usi_word(P, r + 2, j, sh);
     uint32_t f0 = usi_word(Fu, r, j, sh), f1 = usi_word(Fu, r + 1, j, sh), f2 = usi_word(Fu, r + 2, j, sh);
-    int idx = lane;
-#pragma unroll 4
-    for (int y = sT; y <= sB; y++, r++, idx += ww) {
+    auto step = [&](int yr, double fac) {
       const uint32_t p3 = usi_word(P, r + 3, j, sh), f3 = usi_word(Fu, r + 3, j, sh);
       const uint32_t ps = sad4(bk, p0, p1, p2, p3), fs = sad4(bk, f0, f1, f2, f3);
-      const double fac = facRow[33 * abs(y - top)];
       const double pc = (double)ps * fac, fc = (double)fs * fac;
       // costs are positive and never NaN: the running minimum is fmin, and
-      // only the index needs the strict comparison
-      pIdx = pc < pBest ? idx : pIdx;
-      fIdx = fc < fBest ? idx : fIdx;
+      // only the row needs the strict comparison
+      pRow = pc < pBest ? yr : pRow;
+      fRow = fc < fBest ? yr : fRow;
       pBest = fmin(pBest, pc);
       fBest = fmin(fBest, fc);
       p0 = p1; p1 = p2; p2 = p3;
       f0 = f1; f1 = f2; f2 = f3;
-    }
+      r++;
+    };
+    int y = sT;
+    const double *fp = facCol + 33 * (top - sT);  // rows above the block: |y - top| falls
+#pragma unroll 4
+    for (; y < top; y++, fp -= 33) step(y - sT, *fp);
+#pragma unroll 4
+    for (; y <= sB; y++, fp += 33) step(y - sT, *fp);  // fp == facCol at y == top
+    pKey = pRow * 128 + lane;
+    fKey = fRow * 128 + lane;
   }
   if (ww > 64) {
     // column sL+64: one candidate row per lane
@@ -904,18 +968,13 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
       const uint32_t fs = sad4(bk, usi_word(Fu, r, j, sh), usi_word(Fu, r + 1, j, sh), usi_word(Fu, r + 2, j, sh),
                                usi_word(Fu, r + 3, j, sh));
       const double fac = s_fac[33 * abs(y - top) + abs(x - left)];
-      argmin_take((double)ps * fac, i * ww + 64, pBest, pIdx);
-      argmin_take((double)fs * fac, i * ww + 64, fBest, fIdx);
+      argmin_take((double)ps * fac, i * 128 + 64, pBest, pKey);
+      argmin_take((double)fs * fac, i * 128 + 64, fBest, fKey);
     }
   }
-  wave_argmin(pBest, pIdx);
-  wave_argmin(fBest, fIdx);
-  int blk[16];
-#pragma unroll
-  for (int m = 0; m < 4; m++)
-#pragma unroll
-    for (int n = 0; n < 4; n++) blk[4 * m + n] = (bk[m] >> (8 * n)) & 0xff;
-  int pbx = sL + pIdx % ww, pby = sT + pIdx / ww, fbx = sL + fIdx % ww, fby = sT + fIdx / ww;
+  wave_argmin_key(pBest, pKey);
+  wave_argmin_key(fBest, fKey);
+  const int pbx = sL + (pKey & 127), pby = sT + (pKey >> 7), fbx = sL + (fKey & 127), fby = sT + (fKey >> 7);
   // subPelRefine: lanes 0-8 evaluate the 9 positions of the past frame, 16-24 the future
   const int which = lane >> 4, pos = lane & 15;
   double cost = DBL_MAX;
@@ -925,9 +984,14 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
     const uint8_t *ref = which ? f.fut_half : f.past_half;
     const int t = 2 * by + pos / 3 - 1, l = 2 * bx + pos % 3 - 1;
     if (t >= 0 && t <= H2 - 8 && l >= 0 && l <= W2 - 8) {
-      int s = 0;
-      for (int m = 0; m < 4; m++)
-        for (int n = 0; n < 4; n++) s += abs(blk[4 * m + n] - (int)ref[(t + 2 * m) * W2 + l + 2 * n]);
+      // the 4 samples of block row m sit 2 bytes apart in the half-pel frame
+      uint32_t s = 0;
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        const uint8_t *rp = ref + (size_t)(t + 2 * m) * W2 + l;
+        const uint32_t row = (uint32_t)rp[0] | (uint32_t)rp[2] << 8 | (uint32_t)rp[4] << 16 | (uint32_t)rp[6] << 24;
+        s = __builtin_amdgcn_sad_u8(bk[m], row, s);
+      }
       cost = (double)s;
       cidx = pos;
     }
@@ -952,98 +1016,152 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
 // ------------------------------------------------------------ motion learning
 // updateSMF (motionLearning.cpp:70-127): p = expf(-MU*SSE) over the +-4
 // window, normalised by the float sum in raster order.  One wave per 4x4
-// block: lane l evaluates window positions l and l+64; the raster-order sum
-// is then formed identically on every lane from the lanes' values.
+// block.  Every pixel difference is an integer of magnitude at most 255 and
+// every partial SSE stays below 2^24, so the reference's float SSE is exact in
+// any order and equals the integer sum of squares; it is formed here as
+// sum(a^2) + sum(b^2) - 2 sum(a*b) with v_dot4_u32_u8 on 4-pixel rows (a the
+// reconstructed block, b the side information under the window position),
+// reading b from the block's side-information window (at most 12 rows x 12
+// bytes) staged in LDS as words.  Lane l evaluates window positions l and
+// l+64; the wave then adds the probabilities in raster order from LDS.
+#define SMF_WORDS 5  // staged words per window row: 4 + one for the byte alignment
 __global__ __launch_bounds__(256) void k_update_smf(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int b = blockIdx.x * 4 + wave;
-  if (b >= f.N || !wz_block(f, b)) return;  // updateSMF walks the SIR's WZ blocks
+  __shared__ uint32_t s_si[4][12 * SMF_WORDS];
+  __shared__ __align__(16) float s_v[4][84];  // rows of 336 bytes: float4 reads stay aligned
+  // updateSMF walks the SIR's WZ blocks; the barrier below is reached by all
+  const bool on = b < f.N && wz_block(f, b);
   const int W = f.W;
-  const SmfWin s = smf_window(f, b);
-  float blk[16];
+  const SmfWin s = smf_window(f, on ? b : 0);
+  const int cL = s.sL & ~3;
+  if (on && lane < 12 * SMF_WORDS) {
+    const int r = lane / SMF_WORDS, w = lane - r * SMF_WORDS, x = cL + 4 * w;
+    s_si[wave][lane] = s.sT + r <= s.sB + 3 && x + 3 < W ? *(const uint32_t *)(f.si + (size_t)(s.sT + r) * W + x) : 0u;
+  }
+  uint32_t aw[4];
+  uint32_t a2 = 0;
 #pragma unroll
-  for (int m = 0; m < 4; m++)
-#pragma unroll
-    for (int n = 0; n < 4; n++) blk[4 * m + n] = (float)f.recons[(s.top + m) * W + s.left + n];
+  for (int m = 0; m < 4; m++) {
+    aw[m] = on ? *(const uint32_t *)(f.recons + (size_t)(s.top + m) * W + s.left) : 0u;
+    a2 = __builtin_amdgcn_udot4(aw[m], aw[m], a2, false);
+  }
+  __syncthreads();
   float v[2];
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const int p = lane + 64 * h;
     v[h] = 0.0f;
-    if (p < s.len) {
-      const int m = s.sT + p / s.w, q = s.sL + p % s.w;
-      float sse = 0.0f;
+    if (on && p < s.len) {
+      const int pr = p / s.w, xo = s.sL + (p - pr * s.w) - cL, j = xo >> 2, sh = xo & 3;
+      const uint32_t *R = s_si[wave] + pr * SMF_WORDS + j;
+      uint32_t bb = 0, ab = 0;
 #pragma unroll
-      for (int ii = 0; ii < 4; ii++)
-#pragma unroll
-        for (int jj = 0; jj < 4; jj++) {
-          const float d = blk[4 * ii + jj] - (float)f.si[(m + ii) * W + q + jj];
-          sse += d * d;
-        }
+      for (int ii = 0; ii < 4; ii++) {
+        const uint32_t bw = __builtin_amdgcn_alignbyte(R[ii * SMF_WORDS + 1], R[ii * SMF_WORDS], sh);
+        bb = __builtin_amdgcn_udot4(bw, bw, bb, false);
+        ab = __builtin_amdgcn_udot4(aw[ii], bw, ab, false);
+      }
+      const float sse = (float)(a2 + bb - 2u * ab);
       v[h] = dvc_expf(-0.01f * sse);
+      s_v[wave][p] = v[h];
     }
   }
-  // the window is the wave's own (uniform): each term is read from its lane
-  // into a scalar register, with no LDS round trip between the ordered adds
-  const int len = __builtin_amdgcn_readfirstlane(s.len);
-  const int len0 = len < 64 ? len : 64;
+  __syncthreads();
+  if (!on) return;
+  // the sum in raster order, the same on every lane (LDS broadcast reads)
+  const int len = s.len;
+  const float *V = s_v[wave];
   float sum = 0.0f;
-  for (int p = 0; p < len0; p++) sum += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[0]), p));
-  for (int p = 64; p < len; p++) sum += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v[1]), p - 64));
+  int p = 0;
+  for (; p + 4 <= len; p += 4) {
+    const float4 q = *(const float4 *)(V + p);
+    sum += q.x;
+    sum += q.y;
+    sum += q.z;
+    sum += q.w;
+  }
+  for (; p < len; p++) sum += V[p];
   float *pr = f.smf + (size_t)81 * b;
   const bool norm = (double)sum > 0.0;
 #pragma unroll
   for (int h = 0; h < 2; h++) {
-    const int p = lane + 64 * h;
-    if (p < s.len) pr[p] = norm ? v[h] / sum : (p == s.self ? 1.0f : v[h]);
+    const int q = lane + 64 * h;
+    if (q < s.len) pr[q] = norm ? v[h] / sum : (q == s.self ? 1.0f : v[h]);
   }
 }
 
 // update_si_ML (motionLearning.cpp:129-160): SMF-weighted over-complete
 // coefficients for the bands after the frame's current band, accumulated in
-// window raster order and truncated.  One lane per (4x4 block, coefficient of
-// a band not yet decoded): the 16 - (band + 1) lanes of a block read the same
-// 64-byte coefficient record per window position.  A window row's (at most 9)
-// weights and coefficients are loaded before they are added in order.
-// Workgroups are dispatched to the 8 XCDs in turn, each XCD with its own L2;
-// the 1-D grid is renumbered so that XCD k runs the k-th eighth of the
-// (frame, row segment) list in order.  Vertically adjacent blocks, whose
-// windows share 5 of 9 rows, then run on one XCD close together in time and
-// find each other's records in its L2.
-__global__ __launch_bounds__(256) void k_update_si_ml(const FrameDev *F, const int *list, int perFrame, int total) {
-  const int per = gridDim.x >> 3;  // the grid is a multiple of 8
-  const int logical = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  if (logical >= total) return;
-  const int frame = logical / perFrame, seg = logical - frame * perFrame;
-  const FrameDev &f = F[list[frame]];
+// window raster order and truncated.  Each lane owns one (block column,
+// coefficient still to predict) pair, a wave owns ML_ROWS block rows of
+// 64 / cnt adjacent block columns, and the wave streams down the over-complete
+// rows its blocks' windows cover, reading each row once: the windows of block
+// rows R-1, R and R+1 share position row 4R, so while position rows
+// 4j..4j+3 stream by, block row j-1 takes its last row (at 4j), block row j
+// its rows 5-8 and block row j+1 its rows 1-4, each in its own accumulator.
+// Every block's sum still runs over its window in raster order.  A window
+// position's coefficients are one 64-byte record, which the lanes of a block
+// column share; the SMF weights are shared the same way.
+#define ML_ROWS 8
+__global__ __launch_bounds__(256) void k_update_si_ml(const FrameDev *F, const int *list) {
+  const FrameDev &f = F[list[blockIdx.y]];
   const int bandStart = f.band + 1;  // update_si_ML(..., band+1, ...) (DVC_Decoder.cpp:369)
   const int cnt = 16 - bandStart;    // coefficients still to predict per block
   if (cnt <= 0) return;
-  const int t = seg * blockDim.x + threadIdx.x;
-  const int b = t / cnt;
-  if (b >= f.N || !wz_block(f, b)) return;  // update_si_ML walks the SIR's WZ blocks
-  const int index = c_zigzag[bandStart + t % cnt];
-  const int W = f.W, ow = W - 3;
-  const SmfWin s = smf_window(f, b);
-  const float *pr = f.smf + (size_t)81 * b;
-  float acc = 0.0f;
-  int p = 0;
-  for (int m = s.sT; m <= s.sB; m++) {
-    const int *o = f.oct + (size_t)16 * (m * ow + s.sL) + index;
-    float pv[9] = {}, ov[9] = {};
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int W = f.W, H = f.H, bw = W / 4, bh = H / 4, ow = W - 3;
+  const int C = 64 / cnt;  // block columns per wave
+  const int colTiles = (bw + C - 1) / C, rowTiles = (bh + ML_ROWS - 1) / ML_ROWS;
+  const int tile = blockIdx.x * 4 + wave;
+  if (tile >= colTiles * rowTiles) return;  // uniform over the wave
+  const int ty = tile / colTiles, tx = tile - ty * colTiles;
+  const int c = lane / cnt, bc = tx * C + c;
+  const bool live = c < C && bc < bw;
+  const int index = c_zigzag[bandStart + (live ? lane - c * cnt : 0)];
+  const int left = 4 * (live ? bc : 0), sL = imax(0, left - 4), w = imin(W - 4, left + 4) - sL + 1;
+  const int R0 = ty * ML_ROWS, R1 = imin(bh, R0 + ML_ROWS);
+  const int mS = imax(0, 4 * R0 - 4), mE = imin(H - 4, 4 * (R1 - 1) + 4);
+  // one position row m for block row R (R0 <= R < R1): acc += p * o over the row
+  auto rowTerms = [&](float acc, int R, int m, const float *o) -> float {
+    const int b = R * bw + bc;
+    if (live && wz_block(f, b)) {
+      const float *pr = f.smf + (size_t)81 * b + (m - imax(0, 4 * R - 4)) * w;
 #pragma unroll
-    for (int q = 0; q < 9; q++)
-      if (q < s.w) {
-        pv[q] = pr[p + q];
-        ov[q] = (float)o[16 * q];
+      for (int q = 0; q < 9; q++)
+        if (q < w) acc += pr[q] * o[q];
+    }
+    return acc;
+  };
+  auto finish = [&](float acc, int R) {
+    const int b = R * bw + bc;
+    if (live && wz_block(f, b))
+      f.sideTrans[(4 * R + (index >> 2)) * W + left + (index & 3)] = (int)acc;
+  };
+  float accP = 0.0f, accQ = 0.0f;  // block rows j-1 and j, carried into chunk j
+  for (int j = mS >> 2; 4 * j <= mE; j++) {
+    float accN = 0.0f;  // block row j+1, whose window starts at position row 4j
+    for (int t = 0; t < 4; t++) {
+      const int m = 4 * j + t;
+      if (m < mS || m > mE) continue;  // uniform
+      float o[9];
+      const int *op = f.oct + (size_t)16 * ((size_t)m * ow + sL) + index;
+#pragma unroll
+      for (int q = 0; q < 9; q++) o[q] = live && q < w ? (float)op[16 * q] : 0.0f;
+      if (t == 0 && j - 1 >= R0) {  // the last row of block row j-1
+        accP = rowTerms(accP, j - 1, m, o);
+        finish(accP, j - 1);
       }
-#pragma unroll
-    for (int q = 0; q < 9; q++)
-      if (q < s.w) acc += pv[q] * ov[q];
-    p += s.w;
+      if (j >= R0 && j < R1) {
+        accQ = rowTerms(accQ, j, m, o);
+        if (m == H - 4) finish(accQ, j);  // the bottom block row's window ends at H-4
+      }
+      if (j + 1 < R1) accN = rowTerms(accN, j + 1, m, o);
+    }
+    accP = accQ;
+    accQ = accN;
   }
-  f.sideTrans[(s.top + (index >> 2)) * W + s.left + (index & 3)] = (int)acc;
 }
 
 // ------------------------------------------------------------ launchers
@@ -1097,7 +1215,7 @@ hipError_t launch_oct_plane(const FrameDev *F, const int *list, int n, int maxPo
 }
 hipError_t launch_noise_dist(const FrameDev *F, const int *list, int n, int maxN, int maxStride, hipStream_t s) {
   (void)maxStride;  // one wave per block whatever the band's value count
-  hipLaunchKernelGGL(k_noise_dist, dim3(maxN, n), dim3(64), 0, s, F, list);
+  hipLaunchKernelGGL(k_noise_dist, dim3(maxN, n), dim3(ND_T), 0, s, F, list);
   return hipGetLastError();
 }
 hipError_t launch_condbit(const FrameDev *F, const int *list, int n, int maxN, hipStream_t s) {
@@ -1129,10 +1247,10 @@ hipError_t launch_update_smf(const FrameDev *F, const int *list, int n, int maxN
   hipLaunchKernelGGL(k_update_smf, dim3((maxN + 3) / 4, n), dim3(256), 0, s, F, list);
   return hipGetLastError();
 }
-hipError_t launch_update_si_ml(const FrameDev *F, const int *list, int n, int maxN, int bandStart, hipStream_t s) {
-  (void)bandStart;  // each frame's own band (FrameDev::band) sets its coefficient count
-  const int perFrame = (16 * maxN + 255) / 256, total = perFrame * n;
-  hipLaunchKernelGGL(k_update_si_ml, dim3((total + 7) / 8 * 8), dim3(256), 0, s, F, list, perFrame, total);
+hipError_t launch_update_si_ml(const FrameDev *F, const int *list, int n, int maxBw, int maxBh, hipStream_t s) {
+  // tiles of the widest case: 15 coefficients, 4 block columns per wave
+  const int tiles = (maxBw + 3) / 4 * ((maxBh + ML_ROWS - 1) / ML_ROWS);
+  hipLaunchKernelGGL(k_update_si_ml, dim3((tiles + 3) / 4, n), dim3(256), 0, s, F, list);
   return hipGetLastError();
 }
 
