@@ -500,28 +500,59 @@ __global__ __launch_bounds__(1024) void k_noise_param(const FrameDev *F, const i
 // (band 0 at the frame's start, else band + 1) also goes to the octBand plane,
 // so that kernel reads 4 contiguous bytes per window position instead of one
 // int out of each 64-byte record.
+// A workgroup's 256 consecutive positions span at most 3 position rows of a
+// frame at least 176 wide, so the side-information rows under them (at most
+// 6) are staged in LDS as words with aligned loads, and each position's 4x4
+// pixels are taken from there (4 byte alignments) instead of 16 byte loads.
+// The 64-byte records of a wave's 64 positions go out through LDS so that each
+// store instruction writes 1 KB of consecutive bytes.
+#define OCT_ROWS 6
+#define OCT_LW 89  // words per staged row: 352 / 4 + one for the byte alignment
 __global__ __launch_bounds__(256) void k_oct(const FrameDev *F, const int *list, int initSideTrans) {
   const FrameDev &f = F[list[blockIdx.y]];
-  const int ow = f.W - 3, npos = ow * (f.H - 3);
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npos) return;
-  const int top = p / ow, left = p % ow, W = f.W;
+  const int ow = f.W - 3, npos = ow * (f.H - 3), W = f.W, nw = W >> 2;
+  const int p0 = blockIdx.x * 256;
+  if (p0 >= npos) return;  // uniform over the workgroup
+  __shared__ uint32_t s_rows[OCT_ROWS * OCT_LW];
+  __shared__ int4 s_rec[256 * 4];  // the workgroup's records, position-major
+  const int top0 = p0 / ow, rows = imin(f.H, (imin(p0 + 255, npos - 1) / ow) + 4) - top0;  // <= OCT_ROWS
+  for (int i = threadIdx.x; i < rows * OCT_LW; i += 256) {
+    const int r = i / OCT_LW, w = i - r * OCT_LW;
+    s_rows[i] = w < nw ? *(const uint32_t *)(f.si + (size_t)(top0 + r) * W + 4 * w) : 0u;
+  }
+  __syncthreads();
+  const int p = p0 + threadIdx.x;
+  const bool valid = p < npos;
+  const int top = valid ? p / ow : top0, left = valid ? p % ow : 0, j = left >> 2, sh = left & 3;
+  const uint32_t *R = s_rows + (top - top0) * OCT_LW + j;
   int blk[16];
 #pragma unroll
-  for (int m = 0; m < 4; m++)
+  for (int m = 0; m < 4; m++) {
+    const uint32_t v = __builtin_amdgcn_alignbyte(R[m * OCT_LW + 1], R[m * OCT_LW], sh);
 #pragma unroll
-    for (int n = 0; n < 4; n++) blk[4 * m + n] = f.si[(top + m) * W + left + n];
+    for (int n = 0; n < 4; n++) blk[4 * m + n] = (v >> (8 * n)) & 0xff;
+  }
   forward44(blk);
-  int4 *o = (int4 *)(f.oct + (size_t)16 * p);
-  o[0] = make_int4(blk[0], blk[1], blk[2], blk[3]);
-  o[1] = make_int4(blk[4], blk[5], blk[6], blk[7]);
-  o[2] = make_int4(blk[8], blk[9], blk[10], blk[11]);
-  o[3] = make_int4(blk[12], blk[13], blk[14], blk[15]);
-  const int nb = initSideTrans ? 0 : f.band + 1;
-  if (nb < 16) f.octBand[p] = blk[c_zigzag[nb]];
-  if (initSideTrans && (top & 3) == 0 && (left & 3) == 0 && top <= f.H - 4 && left <= f.W - 4)
-    for (int m = 0; m < 4; m++)
-      for (int n = 0; n < 4; n++) f.sideTrans[(top + m) * W + left + n] = blk[4 * m + n];
+  int4 *rec = s_rec + 4 * threadIdx.x;
+  rec[0] = make_int4(blk[0], blk[1], blk[2], blk[3]);
+  rec[1] = make_int4(blk[4], blk[5], blk[6], blk[7]);
+  rec[2] = make_int4(blk[8], blk[9], blk[10], blk[11]);
+  rec[3] = make_int4(blk[12], blk[13], blk[14], blk[15]);
+  if (valid) {
+    const int nb = initSideTrans ? 0 : f.band + 1;
+    if (nb < 16) f.octBand[p] = blk[c_zigzag[nb]];
+    if (initSideTrans && (top & 3) == 0 && (left & 3) == 0 && top <= f.H - 4 && left <= f.W - 4)
+      for (int m = 0; m < 4; m++)
+        for (int n = 0; n < 4; n++) f.sideTrans[(top + m) * W + left + n] = blk[4 * m + n];
+  }
+  __syncthreads();
+  // store i writes the workgroup's 16-byte pieces 256 i + t: position p0 + 64 i + t / 4
+  int4 *dst = (int4 *)(f.oct + (size_t)16 * p0);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int piece = 256 * i + threadIdx.x;
+    if (p0 + (piece >> 2) < npos) dst[piece] = s_rec[piece];
+  }
 }
 
 // octBand from a complete over-complete transform (coefficient `index`).
@@ -1094,73 +1125,105 @@ __global__ __launch_bounds__(256) void k_update_smf(const FrameDev *F, const int
 
 // update_si_ML (motionLearning.cpp:129-160): SMF-weighted over-complete
 // coefficients for the bands after the frame's current band, accumulated in
-// window raster order and truncated.  Each lane owns one (block column,
-// coefficient still to predict) pair, a wave owns ML_ROWS block rows of
-// 64 / cnt adjacent block columns, and the wave streams down the over-complete
-// rows its blocks' windows cover, reading each row once: the windows of block
-// rows R-1, R and R+1 share position row 4R, so while position rows
-// 4j..4j+3 stream by, block row j-1 takes its last row (at 4j), block row j
-// its rows 5-8 and block row j+1 its rows 1-4, each in its own accumulator.
-// Every block's sum still runs over its window in raster order.  A window
-// position's coefficients are one 64-byte record, which the lanes of a block
-// column share; the SMF weights are shared the same way.
+// window raster order and truncated.  A wave owns ML_ROWS block rows of 16
+// adjacent block columns; lane 4c + g owns row g (coefficients 4g..4g+3) of
+// block column c and reads it from each window position's 64-byte record with
+// one 16-byte load.  The wave streams down the over-complete rows its blocks'
+// windows cover, reading each row once: the windows of block rows R-1, R and
+// R+1 share position row 4R, so while position rows 4j..4j+3 stream by, block
+// row j-1 takes its last row (at 4j), block row j its rows 5-8 and block row
+// j+1 its rows 1-4, each in its own accumulators.  Every coefficient's sum
+// still runs over its block's window in raster order; coefficients of bands
+// already decoded are summed too but not stored.  A window row's weights are
+// loaded three per lane by the four lanes of a block column and passed
+// between them with quad permutes.
 #define ML_ROWS 8
+#define ML_COLS 16  // block columns per wave
+template <int L>
+__device__ __forceinline__ float quad_bcast(float v) {
+  // quad_perm [L, L, L, L]: every lane of a quad takes the quad's lane L
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), L * 0x55, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float quad_bcast(float v, int l) {  // l is a constant after unrolling
+  switch (l) {
+    case 0: return quad_bcast<0>(v);
+    case 1: return quad_bcast<1>(v);
+    case 2: return quad_bcast<2>(v);
+    default: return quad_bcast<3>(v);
+  }
+}
 __global__ __launch_bounds__(256) void k_update_si_ml(const FrameDev *F, const int *list) {
   const FrameDev &f = F[list[blockIdx.y]];
   const int bandStart = f.band + 1;  // update_si_ML(..., band+1, ...) (DVC_Decoder.cpp:369)
-  const int cnt = 16 - bandStart;    // coefficients still to predict per block
-  if (cnt <= 0) return;
+  if (bandStart >= 16) return;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int W = f.W, H = f.H, bw = W / 4, bh = H / 4, ow = W - 3;
-  const int C = 64 / cnt;  // block columns per wave
-  const int colTiles = (bw + C - 1) / C, rowTiles = (bh + ML_ROWS - 1) / ML_ROWS;
+  const int colTiles = (bw + ML_COLS - 1) / ML_COLS, rowTiles = (bh + ML_ROWS - 1) / ML_ROWS;
   const int tile = blockIdx.x * 4 + wave;
   if (tile >= colTiles * rowTiles) return;  // uniform over the wave
   const int ty = tile / colTiles, tx = tile - ty * colTiles;
-  const int c = lane / cnt, bc = tx * C + c;
-  const bool live = c < C && bc < bw;
-  const int index = c_zigzag[bandStart + (live ? lane - c * cnt : 0)];
+  const int c = lane >> 2, g = lane & 3, bc = tx * ML_COLS + c;
+  const bool live = bc < bw;
+  int need = 0;  // coefficients 4g + jj still to predict
+  for (int k = bandStart; k < 16; k++)
+    if ((c_zigzag[k] >> 2) == g) need |= 1 << (c_zigzag[k] & 3);
   const int left = 4 * (live ? bc : 0), sL = imax(0, left - 4), w = imin(W - 4, left + 4) - sL + 1;
   const int R0 = ty * ML_ROWS, R1 = imin(bh, R0 + ML_ROWS);
   const int mS = imax(0, 4 * R0 - 4), mE = imin(H - 4, 4 * (R1 - 1) + 4);
-  // one position row m for block row R (R0 <= R < R1): acc += p * o over the row
-  auto rowTerms = [&](float acc, int R, int m, const float *o) -> float {
+  // one position row m for block row R (R0 <= R < R1): acc[jj] += p_q * o_q[jj]
+  // over the row.  Positions past a narrow window's edge (q >= w) have p and o
+  // zero: adding their +0 products leaves every sum's value unchanged (the
+  // reference never adds them; a zero's sign cannot survive the truncation).
+  auto rowTerms = [&](float *acc, int R, int m, const float (*o)[4]) {
+    const float *pr = f.smf + (size_t)81 * (live ? R * bw + bc : 0) + (m - imax(0, 4 * R - 4)) * w;
+    // lane g holds the weights q = g, g + 4, g + 8 of its block column's row
+    float wv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) wv[i] = live && g + 4 * i < w ? pr[g + 4 * i] : 0.0f;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      const float p = quad_bcast(wv[q >> 2], q & 3);
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) acc[jj] += p * o[q][jj];
+    }
+  };
+  auto finish = [&](const float *acc, int R) {
     const int b = R * bw + bc;
     if (live && wz_block(f, b)) {
-      const float *pr = f.smf + (size_t)81 * b + (m - imax(0, 4 * R - 4)) * w;
+      int *dst = f.sideTrans + (4 * R + g) * W + left;
 #pragma unroll
-      for (int q = 0; q < 9; q++)
-        if (q < w) acc += pr[q] * o[q];
+      for (int jj = 0; jj < 4; jj++)
+        if (need >> jj & 1) dst[jj] = (int)acc[jj];
     }
-    return acc;
   };
-  auto finish = [&](float acc, int R) {
-    const int b = R * bw + bc;
-    if (live && wz_block(f, b))
-      f.sideTrans[(4 * R + (index >> 2)) * W + left + (index & 3)] = (int)acc;
-  };
-  float accP = 0.0f, accQ = 0.0f;  // block rows j-1 and j, carried into chunk j
+  float accP[4] = {}, accQ[4] = {};  // block rows j-1 and j, carried into chunk j
   for (int j = mS >> 2; 4 * j <= mE; j++) {
-    float accN = 0.0f;  // block row j+1, whose window starts at position row 4j
+    float accN[4] = {};  // block row j+1, whose window starts at position row 4j
     for (int t = 0; t < 4; t++) {
       const int m = 4 * j + t;
       if (m < mS || m > mE) continue;  // uniform
-      float o[9];
-      const int *op = f.oct + (size_t)16 * ((size_t)m * ow + sL) + index;
+      float o[9][4];
+      const int4 *op = (const int4 *)(f.oct + (size_t)16 * ((size_t)m * ow + sL)) + g;
 #pragma unroll
-      for (int q = 0; q < 9; q++) o[q] = live && q < w ? (float)op[16 * q] : 0.0f;
+      for (int q = 0; q < 9; q++) {
+        const int4 v = live && q < w ? op[4 * q] : make_int4(0, 0, 0, 0);
+        o[q][0] = (float)v.x; o[q][1] = (float)v.y; o[q][2] = (float)v.z; o[q][3] = (float)v.w;
+      }
       if (t == 0 && j - 1 >= R0) {  // the last row of block row j-1
-        accP = rowTerms(accP, j - 1, m, o);
+        rowTerms(accP, j - 1, m, o);
         finish(accP, j - 1);
       }
       if (j >= R0 && j < R1) {
-        accQ = rowTerms(accQ, j, m, o);
+        rowTerms(accQ, j, m, o);
         if (m == H - 4) finish(accQ, j);  // the bottom block row's window ends at H-4
       }
-      if (j + 1 < R1) accN = rowTerms(accN, j + 1, m, o);
+      if (j + 1 < R1) rowTerms(accN, j + 1, m, o);
     }
-    accP = accQ;
-    accQ = accN;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      accP[jj] = accQ[jj];
+      accQ[jj] = accN[jj];
+    }
   }
 }
 
@@ -1248,8 +1311,7 @@ hipError_t launch_update_smf(const FrameDev *F, const int *list, int n, int maxN
   return hipGetLastError();
 }
 hipError_t launch_update_si_ml(const FrameDev *F, const int *list, int n, int maxBw, int maxBh, hipStream_t s) {
-  // tiles of the widest case: 15 coefficients, 4 block columns per wave
-  const int tiles = (maxBw + 3) / 4 * ((maxBh + ML_ROWS - 1) / ML_ROWS);
+  const int tiles = (maxBw + ML_COLS - 1) / ML_COLS * ((maxBh + ML_ROWS - 1) / ML_ROWS);
   hipLaunchKernelGGL(k_update_si_ml, dim3((tiles + 3) / 4, n), dim3(256), 0, s, F, list);
   return hipGetLastError();
 }
