@@ -20,7 +20,11 @@ struct BandState {
   float *dist = nullptr;   // the band's noise distribution, N * stride floats
   size_t distBytes = 0;
   int distIndex = -1, distRange = -1;  // band of the distribution in `dist`
-  dvc_smf *smfRef = nullptr;           // reference-layout SMF staging (N entries)
+  dvc_smf *smfRef = nullptr;           // reference-layout SMF staging (one per SIR entry)
+  int nsir = 0;                        // SIR entries (WZ 4x4 blocks)
+  int *d_blk = nullptr;                // blkIdx of each SIR entry (device)
+  uint8_t *gat = nullptr;              // the distribution in noiseDistribution's SIR-entry layout
+  size_t gatBytes = 0;
   FrameDev fd{};
   FrameDev *d_fd = nullptr;
   int *d_list = nullptr;
@@ -33,6 +37,7 @@ void band_free(dvc_ctx *c) {
   if (c->band->mem) (void)hipFree(c->band->mem);
   if (c->band->dist) (void)hipFree(c->band->dist);
   if (c->band->fmeMem) (void)hipFree(c->band->fmeMem);
+  if (c->band->gat) (void)hipFree(c->band->gat);
   delete c->band;
   c->band = nullptr;
 }
@@ -52,18 +57,28 @@ static int grow(uint8_t **p, size_t *have, size_t bytes) {
 
 static bool valid_size(int W, int H) { return (W == 176 && H == 144) || (W == 352 && H == 288); }
 
-// prob[0..probLen) of the reference-layout SMFs to the kernels' N x 81 layout and back.
-__global__ void k_smf_unpack(const dvc_smf *src, float *dst, int n) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
-  const int len = min(src[b].probLen, 81);
-  for (int p = 0; p < 81; p++) dst[(size_t)81 * b + p] = p < len ? src[b].prob[p] : 0.0f;
+// prob[0..probLen) of the reference-layout SMFs (one per SIR entry i) to the
+// kernels' N x 81 layout (block blk[i]) and back.
+__global__ void k_smf_unpack(const dvc_smf *src, float *dst, int n, const int *blk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int len = min(src[i].probLen, 81);
+  float *d = dst + (size_t)81 * blk[i];
+  for (int p = 0; p < 81; p++) d[p] = p < len ? src[i].prob[p] : 0.0f;
 }
-__global__ void k_smf_pack(const float *src, dvc_smf *dst, int n) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
-  const int len = min(dst[b].probLen, 81);
-  for (int p = 0; p < len; p++) dst[b].prob[p] = src[(size_t)81 * b + p];
+__global__ void k_smf_pack(const float *src, dvc_smf *dst, int n, const int *blk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int len = min(dst[i].probLen, 81);
+  const float *s = src + (size_t)81 * blk[i];
+  for (int p = 0; p < len; p++) dst[i].prob[p] = s[p];
+}
+// The block-major distribution to noiseDistribution's layout: SIR entry i's
+// `stride` values at i * stride.
+__global__ void k_dist_gather(const float *src, float *dst, int n, const int *blk, int stride) {
+  const int i = blockIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || k >= stride) return;
+  dst[(size_t)i * stride + k] = src[(size_t)blk[i] * stride + k];
 }
 
 static int upload_fd(dvc_ctx *c, BandState *B) {
@@ -89,12 +104,35 @@ extern "C" int dvc_band_frame_init(dvc_ctx *c, int W, int H, int qindex, const u
   DVC_CHECK(qindex >= 1 && qindex <= 8, DVC_EINVAL, "dvc_band_frame_init: qindex must be 1..8");
   DVC_CHECK(past && fut, DVC_EINVAL, "dvc_band_frame_init: NULL key frame");
   DVC_CHECK(past_idx < cur_idx && cur_idx < fut_idx, DVC_EINVAL, "dvc_band_frame_init: need past_idx < cur_idx < fut_idx");
-  const int N = W * H / 16;
+  const int N = W * H / 16, bw = W / 4, n8 = N / 4;
+  // initSIR (sideInfoCreation.cpp:554-596) lists the 4x4 blocks of the WZ 8x8
+  // blocks in raster order; the blocks it leaves out (skip and intra) are
+  // marked non-WZ in an 8x8 mode map, which every per-block kernel tests
+  std::vector<int> blk;
+  std::vector<uint8_t> mode;
   if (sir) {
-    DVC_CHECK(nsir == N, DVC_EINVAL, "dvc_band_frame_init: skip and intra blocks are not supported (the SIR must list every block)");
-    for (int i = 0; i < N; i++)
-      DVC_CHECK(sir[i].blkIdx == i, DVC_EINVAL, "dvc_band_frame_init: the SIR must list the blocks in raster order");
+    DVC_CHECK(nsir >= 0 && nsir <= N, DVC_EINVAL, "dvc_band_frame_init: more SIR entries than blocks");
+    mode.assign(n8, 2);
+    std::vector<int> per8(n8, 0);
+    for (int i = 0; i < nsir; i++) {
+      const int b = sir[i].blkIdx;
+      DVC_CHECK(b >= 0 && b < N && (i == 0 || b > sir[i - 1].blkIdx) && sir[i].top == (b / bw) * 4 &&
+                    sir[i].left == (b % bw) * 4,
+                DVC_EINVAL, "dvc_band_frame_init: the SIR must list 4x4 blocks in raster order (blkIdx, top, left)");
+      const int b8 = (b / bw / 2) * (bw / 2) + (b % bw) / 2;
+      mode[b8] = 0;
+      per8[b8]++;
+      blk.push_back(b);
+    }
+    for (int k = 0; k < n8; k++)
+      DVC_CHECK(per8[k] == 0 || per8[k] == 4, DVC_EINVAL,
+                "dvc_band_frame_init: the SIR lists part of an 8x8 block (block modes are per 8x8 block)");
+  } else {
+    nsir = N;
   }
+  if (blk.empty() && nsir == N)
+    for (int b = 0; b < N; b++) blk.push_back(b);
+  if (nsir == N) mode.clear();  // every block is WZ
   DVC_HIP(hipSetDevice(c->device));
   int rc = ensure_cost_tables(c);
   if (rc) return rc;
@@ -109,7 +147,7 @@ extern "C" int dvc_band_frame_init(dvc_ctx *c, int W, int H, int qindex, const u
                oOct = take((size_t)64 * (W - 3) * (H - 3)),
                oOb = take((size_t)4 * (W - 3) * (H - 3)), oSmf = take((size_t)4 * 81 * N),
                oSmfRef = take(sizeof(dvc_smf) * (size_t)N), oLR = take(4 * (size_t)N), oFd = take(sizeof(FrameDev)),
-               oList = take(sizeof(int));
+               oList = take(sizeof(int)), oBlk = take(sizeof(int) * (size_t)N), oMode = take((size_t)n8);
   if ((rc = grow(&B->mem, &B->bytes, off))) return rc;
   uint8_t *m = B->mem;
   FrameDev &f = B->fd;
@@ -122,6 +160,9 @@ extern "C" int dvc_band_frame_init(dvc_ctx *c, int W, int H, int qindex, const u
   f.si = m + oSi; f.recons = m + oRec; f.reconsTrans = (int *)(m + oRt); f.alpha = (float *)(m + oAl);
   f.oct = (int *)(m + oOct); f.octBand = (int *)(m + oOb); f.smf = (float *)(m + oSmf); f.LR = (float *)(m + oLR);
   B->smfRef = (dvc_smf *)(m + oSmfRef);
+  B->d_blk = (int *)(m + oBlk);
+  B->nsir = nsir;
+  f.mode = mode.empty() ? nullptr : m + oMode;
   B->d_fd = (FrameDev *)(m + oFd);
   B->d_list = (int *)(m + oList);
   B->W = W; B->H = H; B->N = N; B->qindex = qindex;
@@ -129,6 +170,8 @@ extern "C" int dvc_band_frame_init(dvc_ctx *c, int W, int H, int qindex, const u
   hipStream_t s = c->stream;
   const int zero = 0;
   DVC_HIP(hipMemcpyAsync(B->d_list, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+  if (nsir > 0) DVC_HIP(hipMemcpyAsync(B->d_blk, blk.data(), sizeof(int) * (size_t)nsir, hipMemcpyHostToDevice, s));
+  if (!mode.empty()) DVC_HIP(hipMemcpyAsync((void *)f.mode, mode.data(), (size_t)n8, hipMemcpyHostToDevice, s));
   DVC_HIP(hipMemcpyAsync((void *)f.past, past, FS, hipMemcpyHostToDevice, s));
   DVC_HIP(hipMemcpyAsync((void *)f.fut, fut, FS, hipMemcpyHostToDevice, s));
   DVC_HIP(launch_keyframe_filters(f.past, (uint8_t *)f.past_lp, (uint8_t *)f.past_half, W, H, s));
@@ -154,14 +197,26 @@ extern "C" int dvc_noise_distribution(dvc_ctx *c, int range, int index, const fl
   if ((rc = grow((uint8_t **)&B->dist, &B->distBytes, need))) return rc;
   DVC_HIP(hipMemcpyAsync(f.alpha, alpha, sizeof(float) * W * H, hipMemcpyHostToDevice, s));
   DVC_HIP(hipMemcpyAsync(f.oct, oct, sizeof(int) * 16 * (size_t)(W - 3) * (H - 3), hipMemcpyHostToDevice, s));
-  DVC_HIP(hipMemcpyAsync(B->smfRef, smf, sizeof(dvc_smf) * (size_t)N, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_smf_unpack, dim3((N + 255) / 256), dim3(256), 0, s, B->smfRef, f.smf, N);
-  DVC_HIP(hipGetLastError());
+  const int ns = B->nsir;
+  if (ns > 0) {
+    DVC_HIP(hipMemcpyAsync(B->smfRef, smf, sizeof(dvc_smf) * (size_t)ns, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_smf_unpack, dim3((ns + 255) / 256), dim3(256), 0, s, B->smfRef, f.smf, ns, B->d_blk);
+    DVC_HIP(hipGetLastError());
+  }
   f.index = index; f.range = range; f.distStride = stride; f.dist = B->dist;
   if ((rc = upload_fd(c, B))) return rc;
   DVC_HIP(launch_oct_plane(B->d_fd, B->d_list, 1, (W - 3) * (H - 3), index, s));
   DVC_HIP(launch_noise_dist(B->d_fd, B->d_list, 1, N, stride, s));
-  if (dist) DVC_HIP(hipMemcpyAsync(dist, B->dist, need, hipMemcpyDeviceToHost, s));
+  if (dist && ns == N) {
+    DVC_HIP(hipMemcpyAsync(dist, B->dist, need, hipMemcpyDeviceToHost, s));
+  } else if (dist && ns > 0) {  // SIR entries only, in their order
+    const size_t out = sizeof(float) * (size_t)ns * stride;
+    if ((rc = grow(&B->gat, &B->gatBytes, out))) return rc;
+    hipLaunchKernelGGL(k_dist_gather, dim3((stride + 255) / 256, ns), dim3(256), 0, s, B->dist, (float *)B->gat, ns,
+                       B->d_blk, stride);
+    DVC_HIP(hipGetLastError());
+    DVC_HIP(hipMemcpyAsync(dist, B->gat, out, hipMemcpyDeviceToHost, s));
+  }
   DVC_HIP(hipStreamSynchronize(s));
   B->distIndex = index;
   B->distRange = range;
@@ -214,16 +269,17 @@ extern "C" int dvc_update_smf(dvc_ctx *c, dvc_smf *smf, const uint8_t *recons, c
   FrameDev &f = B->fd;
   hipStream_t s = c->stream;
   const size_t FS = (size_t)B->W * B->H;
-  const int N = B->N;
-  DVC_HIP(hipMemcpyAsync(B->smfRef, smf, sizeof(dvc_smf) * (size_t)N, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_smf_unpack, dim3((N + 255) / 256), dim3(256), 0, s, B->smfRef, f.smf, N);
+  const int N = B->N, ns = B->nsir;
+  if (ns == 0) return DVC_OK;  // no WZ block: updateSMF's loop is empty
+  DVC_HIP(hipMemcpyAsync(B->smfRef, smf, sizeof(dvc_smf) * (size_t)ns, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_smf_unpack, dim3((ns + 255) / 256), dim3(256), 0, s, B->smfRef, f.smf, ns, B->d_blk);
   DVC_HIP(hipGetLastError());
   DVC_HIP(hipMemcpyAsync(f.si, side_info, FS, hipMemcpyHostToDevice, s));
   DVC_HIP(hipMemcpyAsync(f.recons, recons, FS, hipMemcpyHostToDevice, s));
   DVC_HIP(launch_update_smf(B->d_fd, B->d_list, 1, N, s));
-  hipLaunchKernelGGL(k_smf_pack, dim3((N + 255) / 256), dim3(256), 0, s, f.smf, B->smfRef, N);
+  hipLaunchKernelGGL(k_smf_pack, dim3((ns + 255) / 256), dim3(256), 0, s, f.smf, B->smfRef, ns, B->d_blk);
   DVC_HIP(hipGetLastError());
-  DVC_HIP(hipMemcpyAsync(smf, B->smfRef, sizeof(dvc_smf) * (size_t)N, hipMemcpyDeviceToHost, s));
+  DVC_HIP(hipMemcpyAsync(smf, B->smfRef, sizeof(dvc_smf) * (size_t)ns, hipMemcpyDeviceToHost, s));
   DVC_HIP(hipStreamSynchronize(s));
   return DVC_OK;
 }

@@ -328,9 +328,10 @@ int dvc_side_info_batch(dvc_ctx *ctx, int n, const dvc_si_request *reqs);
  * reference keeps implicit device state between calls (the band's noise
  * distribution between noiseDistribution_CUDA and condBitProb_CUDA, the key
  * frames uploaded by ME_CUDA_Init_Variable) the context keeps it.  Host
- * pointers; every call returns when its outputs are written.  Skip and intra
- * blocks are not supported: the SIR must list every 4x4 block in raster order
- * (blkIdx i at entry i), as initSIR does when no block is skipped. */
+ * pointers; every call returns when its outputs are written.  The SIR lists
+ * the WZ 4x4 blocks in raster order, as initSIR (sideInfoCreation.cpp:554-596)
+ * does: skip and intra blocks (whole 8x8 blocks) are left out, and the SMF
+ * arrays and the distribution are per SIR entry, as the reference's are. */
 typedef struct {                 /* SMF, DvcDec/inc/global.h:146-152 */
   float prob[81];
   int searchTop, searchLeft, searchRight, searchBottom;
@@ -352,16 +353,18 @@ int dvc_band_frame_init(dvc_ctx *ctx, int width, int height, int qindex, const u
                         int past_idx, int cur_idx, int fut_idx, const dvc_search_info *sir, int search_candidate);
 /* init_noiseDst_CUDA + noiseDistribution_CUDA (noiseModel_kernel.cu:84, 140;
  * noiseDistribution, noiseModel.cpp:77-132): the band's noise distribution
- * from alpha (W*H), the SMF (one per block) and the SI's over-complete
+ * from alpha (W*H), the SMF (one per SIR entry) and the SI's over-complete
  * transform (Trans oct[(H-3)*(W-3)], 16 ints each), kept in the context for
  * dvc_cond_bit_prob.  dist (may be NULL) also receives it in noiseDistribution's
- * layout: per block `range` (DC) or 2*range+1 (AC) floats. */
+ * layout: per SIR entry `range` (DC) or 2*range+1 (AC) floats. */
 int dvc_noise_distribution(dvc_ctx *ctx, int range, int index_in_blk, const float *alpha, const dvc_smf *smf,
                            const int *oct, float *dist);
 /* condBitProb_CUDA (noiseModel_kernel.cu:346; condBitProb, noiseModel.cpp:191-310):
  * the N likelihood ratios of bit `bit` of coefficient index_in_blk from the
  * decoded indices so far (recons_trans, W*H) and the distribution of the last
- * dvc_noise_distribution call for the same band (else DVC_ESTATE). */
+ * dvc_noise_distribution call for the same band (else DVC_ESTATE).  Blocks
+ * outside the SIR get 0.9999f / 0.0001f, the value decodeGOP gives them
+ * before the call when skip or intra blocks exist (DVC_Decoder.cpp:203-207). */
 int dvc_cond_bit_prob(dvc_ctx *ctx, float *LR, int index_in_blk, int bit, int step_size, const int *recons_trans,
                       int range);
 /* updateSI_CUDA (updateSI_kernel.cu:298; updateSI + subPelRefine,

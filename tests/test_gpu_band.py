@@ -156,3 +156,51 @@ def test_band_calls_check_their_order(ctx):
     with pytest.raises(_lib.DvcError) as e:
         _lib.BandSession(ctx, s["vid"][0], s["vid"][2], 4, (0, 1, 2), sir=bad)
     assert e.value.rc == _lib.DVC_EINVAL
+
+
+@pytest.mark.parametrize("w,h,q,seed", CASES)
+def test_band_calls_on_a_sir_subset_match_oracle(ctx, w, h, q, seed):
+    """Skip and intra blocks: initSIR lists only the WZ 8x8 blocks' 4x4 blocks
+    (sideInfoCreation.cpp:554-596), so the SMF arrays and the distribution are
+    per SIR entry and condBitProb writes only those blocks' likelihood ratios
+    (the others keep decodeGOP's 0.9999/0.0001, DVC_Decoder.cpp:203-207)."""
+    from dvc_amd import _lib
+    s = _setup(w, h, q, seed)
+    O, g, lib, W = s["O"], s["g"], s["lib"], s["W"]
+    past, _, fut = s["vid"]
+    flags = (np.random.default_rng(seed + 7).random(g.b88n) < 0.3).astype(np.uint8) * 2  # about 30% skip blocks
+    sir = np.zeros(g.b44n, O.SEARCH_INFO_DTYPE)
+    nsir = lib.orc_init_sir(ctypes.byref(g), O.P(flags), O.P(sir))
+    assert 0 < nsir < g.b44n
+    smf = np.zeros(nsir, O.SMF_DTYPE)
+    lib.orc_init_smf(ctypes.byref(g), O.P(smf), O.P(sir), nsir)
+    lib.orc_update_smf(ctypes.byref(g), O.P(smf), O.P(s["recons"]), O.P(s["si"]), O.P(sir), nsir, 0)
+    sess = _lib.BandSession(ctx, past, fut, q, (0, 1, 2), sir=sir[:nsir])
+    for band_pos in (0, 1):
+        index = int(W.ZIGZAG[band_pos])
+        step, rng = _band_params(q, s["wz"].ac_range, index)
+        per = 2 * rng + 1 if index else rng
+        odist = np.zeros((nsir, per), np.float32)
+        lib.orc_noise_distribution(ctypes.byref(g), rng, index, O.P(s["alpha"]), O.P(smf), O.P(s["oct"]),
+                                   O.P(sir), nsir, O.P(odist), 0)
+        gd = sess.noise_distribution(rng, index, s["alpha"], smf, s["oct"])
+        assert gd.view(np.uint32).tobytes() == odist.view(np.uint32).tobytes(), f"band {band_pos}"
+        qbits = int(W.QBITS[q][index])
+        for bit in sorted({qbits, 1}, reverse=True):
+            rt = _recons_trans(s, q, band_pos, index, bit)
+            olr = np.full(g.b44n, np.float32(0.9999) / np.float32(0.0001), np.float32)
+            lib.orc_cond_bit_prob(ctypes.byref(g), q, O.P(olr), index, bit, step, O.P(rt), rng, O.P(odist),
+                                  O.P(sir), nsir, 0)
+            glr = sess.cond_bit_prob(index, bit, step, rt, rng)
+            assert glr.view(np.uint32).tobytes() == olr.view(np.uint32).tobytes(), f"band {band_pos} bit {bit}"
+    osi = s["si"].copy()
+    lib.orc_update_si(ctypes.byref(g), O.P(s["recons"]), O.P(past), O.P(fut), O.P(s["ph"]), O.P(s["fh"]), 0, 1, 2,
+                      O.P(sir), nsir, O.P(osi), None, 0)
+    gsi = sess.update_si(s["si"], s["recons"])
+    assert np.array_equal(gsi, osi)
+    osmf = smf.copy()
+    lib.orc_update_smf(ctypes.byref(g), O.P(osmf), O.P(s["recons"]), O.P(osi), O.P(sir), nsir, 0)
+    gsmf = sess.update_smf(smf, s["recons"], osi)
+    for i in range(nsir):
+        n = int(osmf["probLen"][i])
+        assert gsmf["prob"][i, :n].view(np.uint32).tobytes() == osmf["prob"][i, :n].view(np.uint32).tobytes(), i
