@@ -191,9 +191,19 @@ FRAME_CASES = [
 ]
 
 
-def frames_fixture(mode=0, dynstep=False):
+# CIF at the Qindex values FRAME_CASES leaves out (bitplanes per frame: Q1 10, Q3 17, Q5 36, Q6 45, Q7 50)
+FRAME_CASES_CIF_Q = [
+    (352, 288, 1, 1021, 6, 3, 1),
+    (352, 288, 3, 1023, 6, 3, 1),
+    (352, 288, 5, 1025, 6, 3, 1),
+    (352, 288, 6, 1026, 6, 3, 1),
+    (352, 288, 7, 1027, 6, 3, 1),
+]
+
+
+def frames_fixture(mode=0, dynstep=False, cases=None):
     out = []
-    for (w, h, q, seed, shift, nf, fi) in FRAME_CASES:
+    for (w, h, q, seed, shift, nf, fi) in (FRAME_CASES if cases is None else cases):
         vid = W.synthetic_video(w, h, nf, shift, seed)
         wz = W.encode_wz_frame(vid[fi], q)
         R = O.RefDecoder(w, h, q, mode=mode, nthreads=1, dynstep=dynstep)
@@ -676,6 +686,9 @@ def main():
     if os.environ.get("DVC_GOLDEN_ONLY") == "intra_mode1":
         k = int(os.environ["DVC_GOLDEN_CASE"])
         Path(f"/tmp/dvc_intra_m1_case{k}.json").write_text(json.dumps(intra_streams_mode1_fixture()))
+        return
+    if os.environ.get("DVC_GOLDEN_ONLY") == "cif_q":
+        (HERE / "frames_cif_q.json").write_text(json.dumps(frames_fixture(cases=FRAME_CASES_CIF_Q), indent=1))
         return
     if os.environ.get("DVC_GOLDEN_ONLY") == "skip":
         (HERE / "jbig.json").write_text(json.dumps(jbig_fixture(), indent=1))

@@ -49,6 +49,17 @@ def test_frame_matches_reference_golden(ctx, case, batched):
     _check_against_golden(rec, st, c)
 
 
+@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("batched", [False, True])
+def test_cif_qindex_matches_reference_golden(ctx, case, batched):
+    """CIF at Qindex 1, 3, 5, 6 and 7 (tests/golden/frames_cif_q.json, the
+    reference's decodeGOP): the quantiser tables, band plans and bitplane
+    counts frames.json does not reach."""
+    c = json.loads((GOLD / "frames_cif_q.json").read_text())["cases"][case]
+    (rec, st), = ctx.decode_wz_frames([_case_inputs(c)], batched=batched)
+    _check_against_golden(rec, st, c)
+
+
 def test_mixed_batch_equals_solo(ctx):
     cases = json.loads((GOLD / "frames.json").read_text())["cases"]
     inputs = [_case_inputs(c) for c in cases]
