@@ -206,6 +206,22 @@ def test_frame_decode_matches_reference_fixture(case, fixture):
     assert sha(rec) == c["recons_sha256"]
 
 
+@pytest.mark.parametrize("case", [0, 1])
+def test_cif_low_qindex_frames_match_reference_fixture(case):
+    """The oracle on the CIF Qindex 1 and 3 frames of frames_cif_q.json (the
+    higher Qindex cases of that file run on the GPU only, to keep this suite
+    short)."""
+    c = _frame_cases("frames_cif_q.json")[case]
+    vid = W.synthetic_video(c["W"], c["H"], c["frames"], c["shift"], c["seed"])
+    wz = W.encode_wz_frame(vid[c["wz"]], c["qindex"])
+    rec, info = O.decode_wz_frame(vid[0], vid[2], vid[c["wz"]], wz, c["qindex"], mode=0, nthreads=1)
+    assert info["levels"] == c["levels"]
+    assert [fbits(e) for e in info["entropy"]] == c["entropy_bits"]
+    assert info["bitrate"] == c["bitrate"]
+    assert info["psnr"] == c["psnr"]
+    assert sha(rec) == c["recons_sha256"]
+
+
 @pytest.mark.parametrize("fixture", ["ldpca_nan.json", "ldpca_nan_mode1.json"])
 def test_ldpca_nan_bitplanes_match_reference_fixture(fixture):
     """The oracle's LdpcaDecoder on bitplanes with NaN likelihood ratios (x86's
