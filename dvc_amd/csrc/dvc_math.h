@@ -378,10 +378,12 @@ DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *logtab) {
   const float ln2_hi = 6.9313812256e-01f, ln2_lo = 9.0580006145e-06f, invln2 = 1.4426950216e+00f,
               Q1 = -3.3333335072e-02f, Q2 = 1.5873016091e-03f, Q3 = -7.9365076090e-05f,
               Q4 = 4.0082177293e-06f;
-  const float y0 = (x + (float)0.000000001) / 2.0f;
-  const float y = fminf(y0, 9.75f);
-  const int big = dvcm_asu(y0) >= 0x3f800000u;
-  const float a = big ? 2.0f * y : -2.0f * y;
+  // y0 = (x + 1e-9f)/2 and a = +-2*min(y0, 9.75): halving and doubling s =
+  // x + 1e-9f (a normal float) are exact, so a = min(s, 19.5) when y0 >= 1
+  // (s >= 2) and -s below, with the same bits as the halved-then-doubled form
+  const float s = x + (float)0.000000001;
+  const int big = s >= 2.0f;
+  const float a = big ? fminf(s, 19.5f) : -s;
   // ---- expm1f(a), a in (-2, -1e-9] or [2, 19.5]
   const int k = (int)(invln2 * a + (big ? 0.5f : -0.5f));
   const float tk = (float)k;
