@@ -559,22 +559,28 @@ def cpu_baseline_wz(jobs, args):
     cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     t = time.perf_counter()
     n = 0
+    split = {}
     for j in jobs:
-        O.decode_wz_frame(j["past"], j["fut"], j["orig"], j["wz"], 8, mode=1, nthreads=cores,
-                          max_iter=args.max_iter, idx=j["idx"])
+        _, info = O.decode_wz_frame(j["past"], j["fut"], j["orig"], j["wz"], 8, mode=1, nthreads=cores,
+                                    max_iter=args.max_iter, idx=j["idx"])
+        for k, v in info["times_ms"].items():
+            split[k] = split.get(k, 0.0) + v
         n += 1
         if time.perf_counter() - t > args.cpu_seconds:
             break
     dt = time.perf_counter() - t
     j = jobs[0]
     t0 = time.perf_counter()
-    O.decode_wz_frame(j["past"], j["fut"], j["orig"], j["wz"], 8, mode=0, nthreads=1, max_iter=args.max_iter,
-                      idx=j["idx"])
+    _, info0 = O.decode_wz_frame(j["past"], j["fut"], j["orig"], j["wz"], 8, mode=0, nthreads=1,
+                                 max_iter=args.max_iter, idx=j["idx"])
     dt0 = time.perf_counter() - t0
     return {"value": n / dt, "unit": "WZ frames/s", "cores": cores, "kind": "port",
             "sample": f"{n} CIF Q8 WZ frames of the same workload, ParallelMode 1 (OpenMP, {cores} threads), "
                       f"{dt:.1f}s; ParallelMode 0 on 1 core: 1 frame in {dt0:.1f}s",
-            "mode0_1core_fps": 1.0 / dt0}
+            "mode0_1core_fps": 1.0 / dt0,
+            # the decodeGOP components the reference times under SHOW_TIME_INFO, ms per frame
+            "split_ms_per_frame": {k: v / n for k, v in split.items()},
+            "mode0_split_ms_per_frame": dict(info0["times_ms"])}
 
 
 def cpu_baseline_ldpca(planes, args):
