@@ -8,6 +8,10 @@
 #include "../../include/dvc_accel.h"
 
 namespace dvc {
+struct ColEdges;  // ldpca_kernels.h
+}
+
+namespace dvc {
 
 void set_error(const std::string &msg);
 
@@ -45,14 +49,14 @@ struct Ladder {
   std::vector<uint8_t> crcBasis;   // N: CRC-8 of the message with only bit k set
 
   // Device image, per BP code c (0..numCodes-2):
-  //   d_colRows[c*N + k]       column k's edges 3k..3k+2 as 21-bit fields
-  //                            f = row | rank << 13 (rank: position of the
-  //                            edge among its row's edges in ascending edge
-  //                            order): x = f0 | f1 << 21, y = f1 >> 11 | f2 << 10
+  //   d_colRows[c*N + k]       column k's edges 3k..3k+2, one word each:
+  //                            row | slot << 13, slot = the edge's message
+  //                            position in row order (rowPtr[row] + its rank
+  //                            among the row's edges in ascending edge order)
   //   d_rowPtr[c*(N+1) + r]    number of edges in rows below r (r <= m): row
   //                            r's messages sit at rowPtr[r] + rank
   //   d_tx[c*N + k]            accumulated-syndrome position of syndrome bit k
-  uint2 *d_colRows = nullptr;
+  ColEdges *d_colRows = nullptr;
   // Irregular ladders (column degrees differ) instead of d_colRows:
   //   d_colEdges[c*E + l]   edge l (column order) as row | rank << 16
   //   d_jc[k]               column pointers (N+1)

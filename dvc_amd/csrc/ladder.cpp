@@ -170,7 +170,7 @@ static void column_slots(const Ladder &L, std::vector<int16_t> &slots, int &perT
 
 int upload_ladder(Ladder &L) {
   const int N = L.N, E = L.E, C = L.numCodes;
-  std::vector<uint32_t> colRows(L.regular ? (size_t)C * N * 2 : 0), colEdges(L.regular ? 0 : (size_t)C * E);
+  std::vector<uint32_t> colRows(L.regular ? (size_t)C * N * 3 : 0), colEdges(L.regular ? 0 : (size_t)C * E);
   std::vector<uint16_t> rowPtr((size_t)C * (N + 1)), tx((size_t)C * N);
   DVC_CHECK(E <= 0xffff && N <= 0x2000, DVC_EIO, "ladder: more than 65535 edges or 8192 checks");
   for (int c = 0; c < C; c++) {
@@ -189,14 +189,10 @@ int upload_ladder(Ladder &L) {
         colEdges[(size_t)c * E + e] = (uint32_t)dir[e] | ((uint32_t)rank << 16);
         continue;
       }
-      DVC_CHECK(rank <= 0xff, DVC_EIO, "ladder: check degree above 256");
-      field[e] = (uint32_t)dir[e] | ((uint32_t)rank << 13);
+      field[e] = (uint32_t)dir[e] | ((uint32_t)(cnt[dir[e]] + rank) << 13);  // row | message slot << 13
     }
-    for (int k = 0; L.regular && k < N; k++) {  // every column has degree 3: edges 3k..3k+2
-      const uint32_t f0 = field[3 * k], f1 = field[3 * k + 1], f2 = field[3 * k + 2];
-      colRows[((size_t)c * N + k) * 2] = f0 | (f1 << 21);
-      colRows[((size_t)c * N + k) * 2 + 1] = (f1 >> 11) | (f2 << 10);
-    }
+    for (int k = 0; L.regular && k < N; k++)  // every column has degree 3: edges 3k..3k+2
+      for (int q = 0; q < 3; q++) colRows[((size_t)c * N + k) * 3 + q] = field[3 * k + q];
     const int *ix = &L.idx[(size_t)c * L.period];
     for (int k = 0; k < m; k++) tx[(size_t)c * N + k] = (uint16_t)(ix[k % b] + (k / b) * L.period);
   }

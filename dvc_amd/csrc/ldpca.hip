@@ -22,8 +22,8 @@
 // rowPtr[r+1]-1, ascending edge order), so a check sums a contiguous run;
 // rowTot[m] holds each check's sum of |phi| with the check's sign product
 // (including the syndrome bit) in the sign bit.  Thread t owns columns t,
-// t+T, ...; a column's three edges are found from its packed (row, rank in
-// row) fields, so the variable-node update, the next check-node input and
+// t+T, ...; a column's three edges come as (row, message slot) words, the
+// slot precomputed per code, so the variable-node update, the next check-node input and
 // the decision are computed in registers without further exchange.  Two
 // barriers per iteration: after the check (row) pass and after the variable
 // (column) pass.  The syndrome test's decision parities are XORed into
@@ -60,14 +60,13 @@ __device__ __forceinline__ gfloat_t *as_global(const float *p) { return (gfloat_
 __device__ __forceinline__ gbyte_t *as_global(const uint8_t *p) { return (gbyte_t *)p; }
 __device__ __forceinline__ gword_t *as_global(const uint32_t *p) { return (gword_t *)p; }
 
-// A column's three edges, packed by the ladder upload as 21-bit fields
-// (row | rank-in-row << 13): x = f0 | f1 << 21, y = f1 >> 11 | f2 << 10.
-__device__ __forceinline__ void col_edges(uint2 c, int row[3], int rank[3]) {
-  const uint32_t f[3] = {c.x & 0x1fffffu, (c.x >> 21) | ((c.y & 0x3ffu) << 11), c.y >> 10};
+// A column's three edges as the ladder upload packs them: row | slot << 13,
+// slot = the edge's message position in row order.
+__device__ __forceinline__ void col_edges(const ColEdges &c, int row[3], int slot[3]) {
 #pragma unroll
   for (int q = 0; q < 3; q++) {
-    row[q] = (int)(f[q] & 0x1fffu);
-    rank[q] = (int)(f[q] >> 13);
+    row[q] = (int)(c.e[q] & 0x1fffu);
+    slot[q] = (int)(c.e[q] >> 13);
   }
 }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
@@ -206,7 +205,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
     // differ from them and records the one case where that matters
     const bool mode1 = MODE1, exact = level == run;
     const int m = L.syndLength[level];
-    const uint2 *gcol = L.colRows + (size_t)level * N;
+    const ColEdges *gcol = L.colRows + (size_t)level * N;
 
     // ---- stage the code's row starts and the syndrome (LdpcaDecoder.cpp:684-693)
     const uint16_t *growPtr = L.rowPtr + (size_t)level * (N + 1);
@@ -232,10 +231,10 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         uint32_t s = f2u(v) >> 31;
         float mag = (s ? -1.0f : 1.0f) * v;
         uint32_t f = (f2u(phi_of(mag)) & 0x7fffffffu) | (s << 31);
-        int row[3], rank[3];
-        col_edges(gcol[k], row, rank);
+        int row[3], slot[3];
+        col_edges(gcol[k], row, slot);
 #pragma unroll
-        for (int q = 0; q < 3; q++) phi[growPtr[row[q]] + rank[q]] = u2f(f);
+        for (int q = 0; q < 3; q++) phi[slot[q]] = u2f(f);
       }
     }
     __syncthreads();
@@ -306,12 +305,12 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
       // The next column's rows and channel LLR are fetched while this
       // column's six phi evaluations run.
       bool changed = false;
-      uint2 rowsNext = tid < N ? gcol[tid] : make_uint2(0u, 0u);
+      ColEdges rowsNext = tid < N ? gcol[tid] : ColEdges{{0u, 0u, 0u}};
       gfloat_t *gLLR = as_global(pd.LLR);
       float llrNext = tid < N ? gLLR[tid] : 0.0f;
       for (int j = 0; j < S::CPT; j++) {
         const int k = tid + j * T;
-        const uint2 rows = rowsNext;
+        const ColEdges rows = rowsNext;
         const float llr = llrNext;
         if (j + 1 < S::CPT && k + T < N) {
           rowsNext = gcol[k + T];
@@ -326,8 +325,6 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
           uint32_t R[3];
           float x[3], v[3];
 #pragma unroll
-          for (int q = 0; q < 3; q++) pos[q] += rowPtr[row[q]];
-#pragma unroll
           for (int q = 0; q < 3; q++) {
             R[q] = f2u(rowTot[row[q]]);
             ph[q] = f2u(phi[pos[q]]);
@@ -337,7 +334,11 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
 #pragma unroll
           for (int q = 0; q < 3; q++) v[q] = phi_of(x[q]);
 #pragma unroll
-          for (int q = 0; q < 3; q++) c2v[q] = ((R[q] ^ ph[q]) >> 31) ? v[q] * -1.0f : v[q] * 1.0f;
+          for (int q = 0; q < 3; q++)
+            // times -1 or 1 by the sign product; with no NaN (ParallelMode 0
+            // never runs a NaN bitplane) that is the sign bit flipped
+            c2v[q] = MODE1 ? (((R[q] ^ ph[q]) >> 31) ? v[q] * -1.0f : v[q] * 1.0f)
+                           : u2f((f2u(v[q]) & 0x7fffffffu) | ((f2u(v[q]) ^ R[q] ^ ph[q]) & 0x80000000u));
           float tmp = llr;
           tmp += c2v[0];
           tmp += c2v[1];
