@@ -50,7 +50,7 @@ struct Ladder {
 
   // Device image, per BP code c (0..numCodes-2):
   //   d_colRows[c*N + k]       column k's edges 3k..3k+2, one word each:
-  //                            row | slot << 13, slot = the edge's message
+  //                            row | slot << 15, slot = the edge's message
   //                            position in row order (rowPtr[row] + its rank
   //                            among the row's edges in ascending edge order)
   //   d_rowPtr[c*(N+1) + r]    number of edges in rows below r (r <= m): row

@@ -189,7 +189,7 @@ int upload_ladder(Ladder &L) {
         colEdges[(size_t)c * E + e] = (uint32_t)dir[e] | ((uint32_t)rank << 16);
         continue;
       }
-      field[e] = (uint32_t)dir[e] | ((uint32_t)(cnt[dir[e]] + rank) << 13);  // row | message slot << 13
+      field[e] = (uint32_t)dir[e] | ((uint32_t)(cnt[dir[e]] + rank) << 15);  // row | message slot << 15
     }
     for (int k = 0; L.regular && k < N; k++)  // every column has degree 3: edges 3k..3k+2
       for (int q = 0; q < 3; q++) colRows[((size_t)c * N + k) * 3 + q] = field[3 * k + q];

@@ -60,14 +60,18 @@ __device__ __forceinline__ gfloat_t *as_global(const float *p) { return (gfloat_
 __device__ __forceinline__ gbyte_t *as_global(const uint8_t *p) { return (gbyte_t *)p; }
 __device__ __forceinline__ gword_t *as_global(const uint32_t *p) { return (gword_t *)p; }
 
-// A column's three edges as the ladder upload packs them: row | slot << 13,
-// slot = the edge's message position in row order.
-__device__ __forceinline__ void col_edges(const ColEdges &c, int row[3], int slot[3]) {
+// A column's three edges as the ladder upload packs them: row | slot << 15,
+// slot = the edge's message position in row order; off = slot * 4, the
+// message's byte offset in the phi array, is the word shifted right by 13.
+__device__ __forceinline__ void col_edges(const ColEdges &c, int row[3], uint32_t off[3]) {
 #pragma unroll
   for (int q = 0; q < 3; q++) {
     row[q] = (int)(c.e[q] & 0x1fffu);
-    slot[q] = (int)(c.e[q] >> 13);
+    off[q] = c.e[q] >> 13;
   }
+}
+__device__ __forceinline__ float &at_byte(float *base, uint32_t off) {
+  return *(float *)((uint8_t *)base + off);
 }
 __device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
 
@@ -231,10 +235,11 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         uint32_t s = f2u(v) >> 31;
         float mag = (s ? -1.0f : 1.0f) * v;
         uint32_t f = (f2u(phi_of(mag)) & 0x7fffffffu) | (s << 31);
-        int row[3], slot[3];
-        col_edges(gcol[k], row, slot);
+        int row[3];
+        uint32_t off[3];
+        col_edges(gcol[k], row, off);
 #pragma unroll
-        for (int q = 0; q < 3; q++) phi[slot[q]] = u2f(f);
+        for (int q = 0; q < 3; q++) at_byte(phi, off[q]) = u2f(f);
       }
     }
     __syncthreads();
@@ -320,14 +325,15 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         if (k < N) {
           float c2v[3];
           uint32_t ph[3];
-          int row[3], pos[3];
+          int row[3];
+          uint32_t pos[3];
           col_edges(rows, row, pos);
           uint32_t R[3];
           float x[3], v[3];
 #pragma unroll
           for (int q = 0; q < 3; q++) {
             R[q] = f2u(rowTot[row[q]]);
-            ph[q] = f2u(phi[pos[q]]);
+            ph[q] = f2u(at_byte(phi, pos[q]));
           }
 #pragma unroll
           for (int q = 0; q < 3; q++) x[q] = u2f(R[q] & 0x7fffffffu) - u2f(ph[q] & 0x7fffffffu);
@@ -364,7 +370,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
 #pragma unroll
           for (int q = 0; q < 3; q++) v[q] = phi_of(x[q]);
 #pragma unroll
-          for (int q = 0; q < 3; q++) phi[pos[q]] = u2f((f2u(v[q]) & 0x7fffffffu) | (sg[q] << 31));
+          for (int q = 0; q < 3; q++) at_byte(phi, pos[q]) = u2f((f2u(v[q]) & 0x7fffffffu) | (sg[q] << 31));
         }
         const uint64_t bal = __ballot(dec);
         if (lane == 0) {

@@ -47,7 +47,7 @@ struct BpFrameLink {
 // ladder has (the irregular QCIF ladder's 66; the regular ladders have 65).
 constexpr int kMaxCodes = 66;
 
-// A degree-3 column's edges for the BP kernel: row | message slot << 13 each.
+// A degree-3 column's edges for the BP kernel: row | message slot << 15 each.
 struct ColEdges {
   uint32_t e[3];
 };
@@ -55,7 +55,7 @@ struct ColEdges {
 struct LadderDev {
   int N, E, numCodes;
   int regular;           // every column has degree 3 (colRows), else colEdges/jc/colSlots
-  const ColEdges *colRows;  // per code and column: its three edges as row | message slot << 13
+  const ColEdges *colRows;  // per code and column: its three edges as row | message slot << 15
   const uint32_t *colEdges;  // irregular: per code and edge (column order) row | rank << 16
   const int *jc;             // irregular: column pointers
   const int16_t *colSlots;   // irregular: columns of each of kIrregThreads threads (-1: none)
