@@ -87,8 +87,8 @@ def bench_ldpca(args):
 
     # Roofline of the dominant kernel (bp_6336): algorithmic HBM bytes per
     # launch = per accepted (bitplane, code) item 4N (LLR) + ceil(m/8)
-    # (syndrome) + ceil(N/8) (decisions) + 8N (packed column edges of the
-    # code), summed over every code the sequential search runs.
+    # (syndrome) + ceil(N/8) (decisions) + 12N (the code's column table: one
+    # word per edge), summed over every code the sequential search runs.
     ladders = {n: W.ladder(n) for n in (1584, 6336)}
     m_of = {n: (n // 66) * ladders[n].bits_per_period for n in ladders}
     hbm_bytes = 0
@@ -100,7 +100,7 @@ def bench_ldpca(args):
         for lvl in range(st["start_level"], st["level"] + 1):
             if lvl == 64:
                 continue
-            hbm_bytes += 4 * n + (int(m_of[n][lvl]) + 7) // 8 + (n + 7) // 8 + 8 * n
+            hbm_bytes += 4 * n + (int(m_of[n][lvl]) + 7) // 8 + (n + 7) // 8 + 12 * n
         lds_bytes += 16 * st["iterations"] * 3 * n
     bp_avg_ms = bp_ms / max(bp_n, 1)
     per_step = max(bp_n // max(args.steps, 1), 1)  # bp_6336 launches per step
@@ -497,7 +497,7 @@ def bp_roofline(kernel_ms_avg, hbm_bytes_per_launch, edge_updates_per_launch, wo
     launch duration measured here; peak = the issue rate measured by
     valu_calibration (dependent-free chains built with the library's flags).
     Also reported: HBM (per (bitplane, code) the sequential search runs, 4N
-    LLR + ceil(m/8) syndrome + ceil(N/8) decisions + 8N packed column edges)
+    LLR + ceil(m/8) syndrome + ceil(N/8) decisions + 12N column table, a word per edge)
     against 8 TB/s, with the profiled traffic, and LDS (16 bytes per
     edge-update) against 256 CUs x 128 B/clk x 2.4 GHz."""
     t = kernel_ms_avg * 1e-3
@@ -544,7 +544,7 @@ def roofline_wz(dom, kern, args, nframes, outs, valu=None):
         for b in range(st["nbitplanes"]):
             for lvl in range(st["start_levels"][b], st["levels"][b] + 1):
                 if lvl < 64:
-                    byts += 4 * 6336 + (96 * (lvl + 2) + 7) // 8 + 792 + 8 * 6336
+                    byts += 4 * 6336 + (96 * (lvl + 2) + 7) // 8 + 792 + 12 * 6336
             eu += st["iterations"][b] * 3 * 6336
     return bp_roofline(d["total_ms"] / launches, byts / launches, eu / launches, "wz_cif_q8", valu)
 
