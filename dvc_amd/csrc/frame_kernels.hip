@@ -899,6 +899,11 @@ __global__ void k_inverse_transform(const FrameDev *F, const int *list) {
 // lanes afterwards.  Ties resolve to the first candidate in raster order: the
 // wave's minimum cost first, then the least (row, column) key among the lanes
 // that hold it.
+// A cost (double)sad * fac is formed as fma(2^52 + sad, fac, -2^52 * fac): the
+// double with high word 0x43300000 and low word sad is 2^52 + sad exactly, the
+// fused product-sum is sad * fac before its one rounding, and -2^52 * fac is
+// exact, so the result has the bits of the rounded product (sad = 0 gives +0
+// both ways) without a conversion.
 #define USI_LW 21   // staged words per window row: 20 + one zero word
 #define USI_ROWS 68
 __device__ __forceinline__ uint32_t usi_word(const uint32_t *R, int r, int j, int sh) {
@@ -910,6 +915,13 @@ __device__ __forceinline__ uint32_t sad4(const uint32_t *bk, uint32_t r0, uint32
   s = __builtin_amdgcn_sad_u8(bk[2], r2, s);
   return __builtin_amdgcn_sad_u8(bk[3], r3, s);
 }
+// v_min_f64 of two values known to be numbers (the compiler would otherwise
+// canonicalise a loop-carried operand first)
+__device__ __forceinline__ double dmin_num(double a, double b) {
+  double r;
+  __asm__("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
 __device__ __forceinline__ void argmin_take(double c, int i, double &best, int &bi) {
   if (c < best || (c == best && i < bi)) { best = c; bi = i; }
 }
@@ -917,7 +929,7 @@ __device__ __forceinline__ void argmin_take(double c, int i, double &best, int &
 // least key; every lane gets both.  Costs are finite or DBL_MAX, never NaN.
 __device__ __forceinline__ void wave_argmin_key(double &cost, int &key) {
   double m = cost;
-  for (int o = 32; o > 0; o >>= 1) m = fmin(m, __shfl_xor(m, o));
+  for (int o = 32; o > 0; o >>= 1) m = dmin_num(m, __shfl_xor(m, o));
   int k = cost == m ? key : 0x7fffffff;
   for (int o = 32; o > 0; o >>= 1) k = imin(k, __shfl_xor(k, o));
   cost = m;
@@ -930,7 +942,7 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
   if (b0 >= f.N) return;  // uniform over the workgroup
   const int W = f.W, H = f.H, W2 = 2 * W, H2 = 2 * H, bw = W / 4;
   __shared__ uint32_t s_win[USI_ROWS * 2 * USI_LW];  // [row][past, future][word]
-  __shared__ double s_fac[33 * 33];
+  __shared__ double s_fac[33 * 33];  // by (|dy|, |dx|)
   // union window of the 4 blocks: rows [wT, wB], byte columns [cL, cR]
   const int row0 = (b0 / bw) * 4, left0 = (b0 % bw) * 4;
   const int wT = imax(0, row0 - 32), wB = imin(H - 4, row0 + 32) + 3;
@@ -966,26 +978,46 @@ __global__ __launch_bounds__(256) void k_update_si(const FrameDev *F, const int 
     int r = sT - wT, pRow = 0, fRow = 0;
     uint32_t p0 = usi_word(P, r, j, sh), p1 = usi_word(P, r + 1, j, sh), p2 = usi_word(P, r + 2, j, sh);
     uint32_t f0 = usi_word(Fu, r, j, sh), f1 = usi_word(Fu, r + 1, j, sh), f2 = usi_word(Fu, r + 2, j, sh);
+    // the high words of 2^52 + sad, one register per frame so that each SAD
+    // can land in the low half of its own pair
+    int hiP = 0x43300000, hiF = 0x43300000;
+    __asm__ volatile("" : "+v"(hiP));
+    __asm__ volatile("" : "+v"(hiF));
     auto step = [&](int yr, double fac) {
       const uint32_t p3 = usi_word(P, r + 3, j, sh), f3 = usi_word(Fu, r + 3, j, sh);
       const uint32_t ps = sad4(bk, p0, p1, p2, p3), fs = sad4(bk, f0, f1, f2, f3);
-      const double pc = (double)ps * fac, fc = (double)fs * fac;
-      // costs are positive and never NaN: the running minimum is fmin, and
-      // only the row needs the strict comparison
+      const double facn = fac * -4503599627370496.0;  // -2^52 fac
+      const double pc = fma(__hiloint2double(hiP, (int)ps), fac, facn);
+      const double fc = fma(__hiloint2double(hiF, (int)fs), fac, facn);
+      // costs are positive and never NaN: the running minimum is a plain
+      // v_min_f64, and only the row needs the strict comparison
       pRow = pc < pBest ? yr : pRow;
       fRow = fc < fBest ? yr : fRow;
-      pBest = fmin(pBest, pc);
-      fBest = fmin(fBest, fc);
+      pBest = dmin_num(pBest, pc);
+      fBest = dmin_num(fBest, fc);
       p0 = p1; p1 = p2; p2 = p3;
       f0 = f1; f1 = f2; f2 = f3;
       r++;
     };
     int y = sT;
-    const double *fp = facCol + 33 * (top - sT);  // rows above the block: |y - top| falls
-#pragma unroll 4
-    for (; y < top; y++, fp -= 33) step(y - sT, *fp);
-#pragma unroll 4
-    for (; y <= sB; y++, fp += 33) step(y - sT, *fp);  // fp == facCol at y == top
+    // rows above the block: |y - top| falls, so four rows at a time read the
+    // table upwards from the lowest of their entries (LDS offsets are positive)
+    const double *lo = facCol + 33 * (top - sT - 3);
+    for (; y + 4 <= top; y += 4, lo -= 4 * 33) {
+      step(y - sT, lo[99]);
+      step(y + 1 - sT, lo[66]);
+      step(y + 2 - sT, lo[33]);
+      step(y + 3 - sT, lo[0]);
+    }
+    for (; y < top; y++) step(y - sT, facCol[33 * (top - y)]);
+    const double *fp = facCol;  // rows from the block down: |y - top| rises
+    for (; y + 4 <= sB + 1; y += 4, fp += 4 * 33) {
+      step(y - sT, fp[0]);
+      step(y + 1 - sT, fp[33]);
+      step(y + 2 - sT, fp[66]);
+      step(y + 3 - sT, fp[99]);
+    }
+    for (; y <= sB; y++, fp += 33) step(y - sT, *fp);
     pKey = pRow * 128 + lane;
     fKey = fRow * 128 + lane;
   }
