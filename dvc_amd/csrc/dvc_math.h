@@ -256,6 +256,45 @@ DVC_MATH_FN float dvc_logf_normal(float x, const double *tab) {
   return (float)y;
 }
 
+// dvc_logf_normal for the arguments phi reaches, z = tanhf(y) with y >=
+// 5e-10, so z >= 2^-31 and k = (bits(z) - 0x3f330000) >> 23 lies in [-31, 0].
+// Everything of the general path that depends on (k, i) only comes from one
+// 16-byte table entry e = (k + 31) * 16 + i, which a kernel keeps in LDS:
+//   * invc[i] * 2^-k: the general path's reduced argument is z * 2^-k (z with
+//     its exponent moved, exact for normal z), so fma(z, invc * 2^-k, -1) has
+//     the same exact value before its one rounding as fma(z * 2^-k, invc, -1),
+//     and the power-of-two scaling of invc is exact;
+//   * fma((double)k, Ln2, logc[i]), the same expression, so the same bits.
+// With t = bits(z) - 0x2fb30000 (= bits(z) - 0x3f330000 + 31 * 2^23), e is
+// t >> 19, masked to the table, so an argument outside that range (a NaN lane
+// whose value is discarded) still reads inside it.
+#define DVC_PHI_LOGTAB 512  // entries of two doubles each (9 index bits)
+DVC_MATH_FN void dvc_phi_logtab_entry(int e, double *tab) {
+  const double Ln2 = 0x1.62e42fefa39efp-1;
+  const int i = e & 15, k = (e >> 4) - 31;
+  tab[2 * e] = ldexp(dvcm_logf_invc[i], -k);
+  tab[2 * e + 1] = fma((double)k, Ln2, dvcm_logf_logc[i]);
+}
+DVC_MATH_FN float dvc_logf_phi(float x, const double *tab) {
+  const double A0 = -0x1.00ea348b88334p-2, A1 = 0x1.5575b0be00b6ap-2, A2 = -0x1.ffffef20a4123p-2;
+  const uint32_t t = dvcm_asu(x) - 0x2fb30000u;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // one v_bfe_u32 (kept as written: the compiler would otherwise fold the
+  // field extraction and the address scaling into a shift, a mask and an add)
+  uint32_t e;
+  __asm__("v_bfe_u32 %0, %1, 19, 9" : "=v"(e) : "v"(t));
+#else
+  const uint32_t e = (t >> 19) & (DVC_PHI_LOGTAB - 1);
+#endif
+  const double invcs = tab[2 * e], y0 = tab[2 * e + 1];
+  const double r = fma((double)x, invcs, -1.0);
+  const double r2 = r * r;
+  double y = fma(A1, r, A2);
+  y = fma(A0, r2, y);
+  y = fma(y, r2, y0 + r);
+  return (float)y;
+}
+
 // dvc_bp_phi for x >= 0 (the magnitudes BP applies it to), bit-identical to
 // it, without data-dependent branches, so the lanes of a wave never diverge.
 // With y = (x + 1e-9f)/2 >= 5e-10:
@@ -363,7 +402,13 @@ DVC_MATH_FN float dvcm_div_phi(float a, float b) {
 //     case of its own (x - (x*e - hxs) is x there);
 //   * expm1f's polynomial stops at Q4: on the reduced arguments here
 //     (|r| <= 0.35) the Q5 term never changes the final result;
-//   * both quotients use dvcm_div_phi.
+//   * both quotients use dvcm_div_phi;
+//   * a - k*ln2_hi is one fused multiply-add: k*ln2_hi is exact (|k| <= 28
+//     and ln2_hi has 17 significant bits), so the single rounding is the
+//     subtraction's; likewise 0.5*(xx - e2) - 0.5, whose product is exact
+//     (and xx - e2 is the negated e2 - xx, which the k <= -2 and k >= 3
+//     cases use as well);
+//   * logf is dvc_logf_phi with its (k, i) table in `phitab`.
 // Pins a value in a register at this point, so the compiler cannot sink its
 // computation into a branch (device builds; a no-op on the host).
 DVC_MATH_FN float dvcm_pin(float v) {
@@ -373,7 +418,7 @@ DVC_MATH_FN float dvcm_pin(float v) {
   return v;
 }
 
-DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *logtab) {
+DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *phitab) {
   DVCM_NOCONTRACT
   const float ln2_hi = 6.9313812256e-01f, ln2_lo = 9.0580006145e-06f, invln2 = 1.4426950216e+00f,
               Q1 = -3.3333335072e-02f, Q2 = 1.5873016091e-03f, Q3 = -7.9365076090e-05f,
@@ -387,7 +432,7 @@ DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *logtab) {
   // ---- expm1f(a), a in (-2, -1e-9] or [2, 19.5]
   const int k = (int)(invln2 * a + (big ? 0.5f : -0.5f));
   const float tk = (float)k;
-  const float hi = a - tk * ln2_hi;
+  const float hi = fmaf(-tk, ln2_hi, a);
   const float lo = tk * ln2_lo;
   const float xx = hi - lo;
   const float c = (hi - xx) - lo;
@@ -399,9 +444,9 @@ DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *logtab) {
   const float r_k0 = xx - (xx * e - hxs);
   float e2 = (xx * (e - c) - c);
   e2 -= hxs;
-  const float r_km1 = 0.5f * (xx - e2) - 0.5f;
-  const int kneg = k <= -2;
   const float u = e2 - xx;
+  const float r_km1 = fmaf(-0.5f, u, -0.5f);  // 0.5f * (xx - e2) - 0.5f: xx - e2 is -u exactly
+  const int kneg = k <= -2;
   const float Yneg = 1.0f - u;
   const float Ypos = dvcm_pin(dvcm_asf(0x3f800000u - (0x1000000u >> (k & 31))) - u);
   const float Y = kneg ? Yneg : Ypos;
@@ -412,7 +457,7 @@ DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *logtab) {
   // ---- tanhf(y)
   const float q = dvcm_div_phi(big ? 2.0f : -em1, em1 + 2.0f);
   const float z = big ? 1.0f - q : q;
-  return -dvc_logf_normal(z, logtab);
+  return -dvc_logf_phi(z, phitab);
 }
 
 // ---------------------------------------------------------------- exp (double)

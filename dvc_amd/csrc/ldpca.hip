@@ -149,6 +149,8 @@ template <int N, int T, bool MODE1>
 __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
   using S = BpShape<N, T>;
   constexpr int E = S::E;
+  // phi's logf table first, so that its LDS address fits a read's offset field
+  __shared__ __align__(16) double s_phitab[2 * DVC_PHI_LOGTAB];
   __shared__ float phi[E];  // by check row
   __shared__ float rowTot[N];
   __shared__ uint16_t rowPtr[N + 2];
@@ -156,16 +158,12 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
   __shared__ uint32_t synBits[S::SYN_WORDS];
   __shared__ int s_item, s_cancel;
   __shared__ uint32_t s_crc[T / 64];
-  __shared__ double s_logtab[32];  // logf's (invc, logc) table, read with per-lane indices
   __shared__ uint32_t parBits[2][S::SYN_WORDS];  // decision parity per check, written by alternate iterations
   __shared__ int s_changed[2];  // ParallelMode 1: some decision changed in the variable pass (alternate iterations)
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const LadderDev L = a.lad;
-  if (tid < 16) {
-    s_logtab[tid] = dvcm_logf_invc[tid];
-    s_logtab[16 + tid] = dvcm_logf_logc[tid];
-  }
+  for (int e = tid; e < DVC_PHI_LOGTAB; e += T) dvc_phi_logtab_entry(e, s_phitab);
 
   for (;;) {
     // ---- grab the next item (lane 0 of the workgroup), broadcast through LDS
@@ -201,7 +199,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
     }
     // phi of the kernel, NaN-preserving on a tainted ParallelMode 1 bitplane
     auto phi_of = [&](float x) {
-      const float v = dvc_bp_phi_fast(x, s_logtab);
+      const float v = dvc_bp_phi_fast(x, s_phitab);
       return (MODE1 && taint && x != x) ? x : v;
     };
     // ParallelMode 1: the decisions carried into this code are known only for
@@ -234,7 +232,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
         float v = as_global(pd.LLR)[k];
         uint32_t s = f2u(v) >> 31;
         float mag = (s ? -1.0f : 1.0f) * v;
-        uint32_t f = (f2u(phi_of(mag)) & 0x7fffffffu) | (s << 31);
+        uint32_t f = f2u(copysignf(phi_of(mag), v));  // |phi| with the LLR's sign bit
         int row[3];
         uint32_t off[3];
         col_edges(gcol[k], row, off);
@@ -370,7 +368,7 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
 #pragma unroll
           for (int q = 0; q < 3; q++) v[q] = phi_of(x[q]);
 #pragma unroll
-          for (int q = 0; q < 3; q++) at_byte(phi, pos[q]) = u2f((f2u(v[q]) & 0x7fffffffu) | (sg[q] << 31));
+          for (int q = 0; q < 3; q++) at_byte(phi, pos[q]) = copysignf(v[q], u2f(sg[q] << 31));
         }
         const uint64_t bal = __ballot(dec);
         if (lane == 0) {

@@ -43,6 +43,8 @@ __device__ __forceinline__ uint32_t ir_acc_bit(const uint8_t *acc, int i) {
 template <bool MODE1>
 __global__ __launch_bounds__(kIrregThreads) void bp_irregular_kernel(BpArgs a) {
   constexpr int T = kIrregThreads;
+  // phi's logf table first, so that its LDS address fits a read's offset field
+  __shared__ __align__(16) double s_phitab[2 * DVC_PHI_LOGTAB];
   __shared__ float phi[IR_EMAX];  // by check row; during the variable pass an edge's c2v
   __shared__ float rowTot[IR_NMAX];
   __shared__ uint16_t rowPtr[IR_NMAX + 2];
@@ -51,17 +53,13 @@ __global__ __launch_bounds__(kIrregThreads) void bp_irregular_kernel(BpArgs a) {
   __shared__ uint32_t parBits[2][(IR_NMAX + 31) / 32];
   __shared__ int s_item, s_cancel;
   __shared__ uint32_t s_crc[T / 64];
-  __shared__ double s_logtab[32];
   __shared__ int s_changed[2];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const LadderDev L = a.lad;
   const int N = L.N, E = L.E, S = L.colSlotsPerThread, synWords = (N + 31) / 32;
   const int16_t *mySlots = L.colSlots + (size_t)tid * S;
-  if (tid < 16) {
-    s_logtab[tid] = dvcm_logf_invc[tid];
-    s_logtab[16 + tid] = dvcm_logf_logc[tid];
-  }
+  for (int e = tid; e < DVC_PHI_LOGTAB; e += T) dvc_phi_logtab_entry(e, s_phitab);
 
   for (;;) {
     // ---- grab the next item (thread 0), broadcast through LDS
@@ -96,7 +94,7 @@ __global__ __launch_bounds__(kIrregThreads) void bp_irregular_kernel(BpArgs a) {
       continue;
     }
     auto phi_of = [&](float x) {
-      const float v = dvc_bp_phi_fast(x, s_logtab);
+      const float v = dvc_bp_phi_fast(x, s_phitab);
       return (MODE1 && taint && x != x) ? x : v;
     };
     const bool mode1 = MODE1, exact = level == run;

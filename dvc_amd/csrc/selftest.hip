@@ -9,11 +9,13 @@
 namespace dvc {
 
 __global__ void k_selftest(int func, const void *in, void *out, int n) {
-  __shared__ double s_logtab[32];  // as the BP kernel stages it
+  __shared__ double s_logtab[32];  // dvc_logf_normal's (invc, logc) table
+  __shared__ __align__(16) double s_phitab[2 * DVC_PHI_LOGTAB];  // as the BP kernels stage it
   if (threadIdx.x < 16) {
     s_logtab[threadIdx.x] = dvcm_logf_invc[threadIdx.x];
     s_logtab[16 + threadIdx.x] = dvcm_logf_logc[threadIdx.x];
   }
+  for (int e = threadIdx.x; e < DVC_PHI_LOGTAB; e += blockDim.x) dvc_phi_logtab_entry(e, s_phitab);
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -25,7 +27,7 @@ __global__ void k_selftest(int func, const void *in, void *out, int n) {
     case 4: ((double *)out)[i] = sqrt((double)((const int *)in)[i]); break;
     case 5: ((double *)out)[i] = dvc_exp(((const double *)in)[i]); break;
     case 6: ((float *)out)[i] = dvc_bp_phi_nonneg(((const float *)in)[i], s_logtab); break;
-    case 7: ((float *)out)[i] = dvc_bp_phi_fast(((const float *)in)[i], s_logtab); break;
+    case 7: ((float *)out)[i] = dvc_bp_phi_fast(((const float *)in)[i], s_phitab); break;
     default: break;
   }
 }
@@ -35,11 +37,8 @@ __global__ void k_selftest(int func, const void *in, void *out, int n) {
 // counts[0] = patterns compared, counts[1] = mismatches; counts[2] = lowest
 // mismatching pattern (0xffffffff when none).
 __global__ void k_phi_sweep(uint32_t lo, uint32_t hi, unsigned long long *counts) {
-  __shared__ double s_logtab[32];
-  if (threadIdx.x < 16) {
-    s_logtab[threadIdx.x] = dvcm_logf_invc[threadIdx.x];
-    s_logtab[16 + threadIdx.x] = dvcm_logf_logc[threadIdx.x];
-  }
+  __shared__ __align__(16) double s_phitab[2 * DVC_PHI_LOGTAB];
+  for (int e = threadIdx.x; e < DVC_PHI_LOGTAB; e += blockDim.x) dvc_phi_logtab_entry(e, s_phitab);
   __syncthreads();
   unsigned long long n = 0, bad = 0;
   uint32_t first = 0xffffffffu;
@@ -47,7 +46,7 @@ __global__ void k_phi_sweep(uint32_t lo, uint32_t hi, unsigned long long *counts
   for (uint64_t b = (uint64_t)lo + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b <= hi; b += stride) {
     const float x = __uint_as_float((uint32_t)b);
     if (x != x) continue;
-    const uint32_t f = __float_as_uint(dvc_bp_phi_fast(x, s_logtab));
+    const uint32_t f = __float_as_uint(dvc_bp_phi_fast(x, s_phitab));
     const uint32_t g = __float_as_uint(dvc_bp_phi(x));
     n++;
     if (f != g) {
