@@ -430,7 +430,9 @@ DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *phitab) {
   const int big = s >= 2.0f;
   const float a = big ? fminf(s, 19.5f) : -s;
   // ---- expm1f(a), a in (-2, -1e-9] or [2, 19.5]
-  const int k = (int)(invln2 * a + (big ? 0.5f : -0.5f));
+  // fused: trunc(invln2*a +- 0.5) equals the separately rounded form's on
+  // every float a of this range (checked exhaustively)
+  const int k = (int)fmaf(invln2, a, big ? 0.5f : -0.5f);
   const float tk = (float)k;
   const float hi = fmaf(-tk, ln2_hi, a);
   const float lo = tk * ln2_lo;
@@ -441,19 +443,21 @@ DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *phitab) {
   const float r1 = 1.0f + hxs * (Q1 + hxs * (Q2 + hxs * (Q3 + hxs * Q4)));
   const float t3 = 3.0f - r1 * hfx;
   const float e = hxs * dvcm_div_phi(r1 - t3, 6.0f - xx * t3);
-  const float r_k0 = xx - (xx * e - hxs);
   float e2 = (xx * (e - c) - c);
   e2 -= hxs;
   const float u = e2 - xx;
+  // k = 0: c is +0 (hi = a, lo = +0), so e2 is xx*e - hxs with the same two
+  // roundings as the k = 0 result's inner term, and xx - (xx*e - hxs) is -u
+  const float r_k0 = -u;
   const float r_km1 = fmaf(-0.5f, u, -0.5f);  // 0.5f * (xx - e2) - 0.5f: xx - e2 is -u exactly
-  const int kneg = k <= -2;
-  const float Yneg = 1.0f - u;
-  const float Ypos = dvcm_pin(dvcm_asf(0x3f800000u - (0x1000000u >> (k & 31))) - u);
-  const float Y = kneg ? Yneg : Ypos;
+  // k >= 3: Y = (1 - 2^-k) - u; k <= -2: Y = 1 - u, which the same expression
+  // gives, since k & 31 is then 29 or 30 and 2^24 >> 29 (or 30) is 0
+  const float Y = dvcm_pin(dvcm_asf(0x3f800000u - (0x1000000u >> (k & 31))) - u);
   const float R = dvcm_asf(dvcm_asu(Y) + ((uint32_t)k << 23));
-  float em1 = kneg ? R - 1.0f : R;
-  em1 = (k == -1) ? r_km1 : em1;
-  em1 = (k == 0) ? r_k0 : em1;
+  // k >= 3 exactly on the lanes with a >= 2 (big); the others have k in -3..0
+  float em1 = (k == -1) ? r_km1 : r_k0;
+  em1 = (k <= -2) ? R - 1.0f : em1;
+  em1 = big ? R : em1;
   // ---- tanhf(y)
   const float q = dvcm_div_phi(big ? 2.0f : -em1, em1 + 2.0f);
   const float z = big ? 1.0f - q : q;
