@@ -356,7 +356,14 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
           }
           if (dec) {
 #pragma unroll
-            for (int q = 0; q < 3; q++) atomicXor(&parBits[t & 1][row[q] >> 5], 1u << (row[q] & 31));
+            for (int q = 0; q < 3; q++) {
+              // word row >> 5 = bits 5..12 of the column word, extracted by one
+              // v_bfe_u32 (kept as written, so that the word's address is one
+              // shift-and-add); the bit's shift takes the row's low 5 bits
+              uint32_t w;
+              __asm__("v_bfe_u32 %0, %1, 5, 8" : "=v"(w) : "v"(rows.e[q]));
+              atomicXor(&parBits[t & 1][w], 1u << (row[q] & 31));
+            }
           }
           uint32_t sg[3];
 #pragma unroll
