@@ -430,10 +430,12 @@ DVC_MATH_FN float dvc_bp_phi_fast(float x, const double *phitab) {
   const int big = s >= 2.0f;
   const float a = big ? fminf(s, 19.5f) : -s;
   // ---- expm1f(a), a in (-2, -1e-9] or [2, 19.5]
-  // fused: trunc(invln2*a +- 0.5) equals the separately rounded form's on
-  // every float a of this range (checked exhaustively)
-  const int k = (int)fmaf(invln2, a, big ? 0.5f : -0.5f);
-  const float tk = (float)k;
+  // k = rint(invln2*a) in place of trunc(invln2*a +- 0.5): the two differ on
+  // 14 floats a of this range, and phi's result is the same on those (every
+  // argument is checked exhaustively); rint leaves tk a float and k one
+  // conversion away
+  const float tk = rintf(invln2 * a);
+  const int k = (int)tk;
   const float hi = fmaf(-tk, ln2_hi, a);
   const float lo = tk * ln2_lo;
   const float xx = hi - lo;
