@@ -391,10 +391,12 @@ DVC_MATH_FN float dvcm_div_phi(float a, float b) {
 //     is below half an ulp of 1), so every larger argument gives the same
 //     result, every intermediate is finite, and expm1f's reduction index k
 //     stays in [-3, 28];
-//   * with k = (int)(a/ln2 +- 0.5) taken on every argument, the reduction
+//   * with the reduction index k taken on every argument, the reduction
 //     a - k*ln2_hi - k*ln2_lo reproduces expm1f's special cases by itself:
 //     k = 0 leaves a unchanged (the unreduced case, c = 0) and k = -1 is
-//     exactly the hx < 1.5 ln2 case's a + ln2_hi, -ln2_lo;
+//     exactly the hx < 1.5 ln2 case's a + ln2_hi, -ln2_lo; k is rint(a/ln2)
+//     where expm1f has (int)(a/ln2 +- 0.5): they differ on 14 floats a of
+//     this range, on which phi's result is the same;
 //   * k takes the values -3..0 and 3..28; the result cases k = 0, k = -1,
 //     k <= -2 and k >= 3 are all computed and one selected (no branches);
 //     for k = 23..28 (y >= 7.8) the k <= 22 form stands in for expm1f's
@@ -408,6 +410,8 @@ DVC_MATH_FN float dvcm_div_phi(float a, float b) {
 //     subtraction's; likewise 0.5*(xx - e2) - 0.5, whose product is exact
 //     (and xx - e2 is the negated e2 - xx, which the k <= -2 and k >= 3
 //     cases use as well);
+//   * the k = 0 result is -u (u = e2 - xx, below), and one expression gives
+//     the k <= -2 and k >= 3 cases' Y;
 //   * logf is dvc_logf_phi with its (k, i) table in `phitab`.
 // Pins a value in a register at this point, so the compiler cannot sink its
 // computation into a branch (device builds; a no-op on the host).
