@@ -151,7 +151,9 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
   constexpr int E = S::E;
   // phi's logf table first, so that its LDS address fits a read's offset field
   __shared__ __align__(16) double s_phitab[2 * DVC_PHI_LOGTAB];
-  __shared__ float phi[E];  // by check row
+  // by check row; 3 spare slots at the end, so that the check pass may read
+  // a row's last group of four unconditionally
+  __shared__ float phi[E + 3];
   __shared__ float rowTot[N];
   __shared__ uint16_t rowPtr[N + 2];
   __shared__ uint32_t decBits[S::DEC_WORDS];
@@ -255,18 +257,32 @@ __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
       for (int r = tid; r < m; r += T) {
         const int e0 = rowPtr[r], e1 = rowPtr[r + 1];
         const uint32_t syn = (synBits[r >> 5] >> (r & 31)) & 1u;
-        uint32_t sgn = syn;
+        uint32_t sx = 0u;  // XOR of the row's message words: the sign product in bit 31
         float sum = 0.0f;
-        // four loads in flight per step; a slot past the row's end reads as
-        // +0, and sum + 0.0f is sum bit for bit (sum is never -0)
-        for (int j = e0; j < e1; j += 4) {
+        // groups of four loads without conditions, then the last 1..3 slots
+        // read unconditionally too (the rows are contiguous and phi has spare
+        // slots at its end) with those past the row's end taken as +0: sum +
+        // 0.0f is sum bit for bit (sum is never -0)
+        int j = e0;
+        for (; j + 4 <= e1; j += 4) {
           uint32_t f[4];
 #pragma unroll
-          for (int i = 0; i < 4; i++) f[i] = j + i < e1 ? f2u(phi[j + i]) : 0u;
+          for (int i = 0; i < 4; i++) f[i] = f2u(phi[j + i]);
 #pragma unroll
           for (int i = 0; i < 4; i++) sum += u2f(f[i] & 0x7fffffffu);
-          sgn ^= (f[0] ^ f[1] ^ f[2] ^ f[3]) >> 31;
+          sx ^= f[0] ^ f[1] ^ f[2] ^ f[3];
         }
+        if (j < e1) {
+          uint32_t f[3];
+#pragma unroll
+          for (int i = 0; i < 3; i++) f[i] = f2u(phi[j + i]);
+          f[1] = j + 1 < e1 ? f[1] : 0u;
+          f[2] = j + 2 < e1 ? f[2] : 0u;
+#pragma unroll
+          for (int i = 0; i < 3; i++) sum += u2f(f[i] & 0x7fffffffu);
+          sx ^= f[0] ^ f[1] ^ f[2];
+        }
+        const uint32_t sgn = syn ^ (sx >> 31);
         rowTot[r] = u2f(f2u(sum) | (sgn << 31));
         if (t > 0 && (syn ^ ((parPrev[r >> 5] >> (r & 31)) & 1u))) fail = 1;
       }
