@@ -45,7 +45,9 @@ __global__ __launch_bounds__(kIrregThreads) void bp_irregular_kernel(BpArgs a) {
   constexpr int T = kIrregThreads;
   // phi's logf table first, so that its LDS address fits a read's offset field
   __shared__ __align__(16) double s_phitab[2 * DVC_PHI_LOGTAB];
-  __shared__ float phi[IR_EMAX];  // by check row; during the variable pass an edge's c2v
+  // by check row; during the variable pass an edge's c2v.  3 spare slots at
+  // the end, so that the check pass may read a row's last slots unconditionally
+  __shared__ float phi[IR_EMAX + 3];
   __shared__ float rowTot[IR_NMAX];
   __shared__ uint16_t rowPtr[IR_NMAX + 2];
   __shared__ uint8_t dec[IR_NMAX];
@@ -143,16 +145,32 @@ __global__ __launch_bounds__(kIrregThreads) void bp_irregular_kernel(BpArgs a) {
       for (int r = tid; r < m; r += T) {
         const int e0 = rowPtr[r], e1 = rowPtr[r + 1];
         const uint32_t syn = (synBits[r >> 5] >> (r & 31)) & 1u;
-        uint32_t sgn = syn;
+        uint32_t sx = 0u;  // XOR of the row's message words: the sign product in bit 31
         float sum = 0.0f;
-        for (int j = e0; j < e1; j += 4) {
+        // groups of four loads without conditions, then the last 1..3 slots
+        // read unconditionally too (rows are contiguous, phi has spare slots)
+        // with those past the row's end taken as +0: sum + 0.0f is sum bit for
+        // bit (sum is never -0)
+        int j = e0;
+        for (; j + 4 <= e1; j += 4) {
           uint32_t f[4];
 #pragma unroll
-          for (int i = 0; i < 4; i++) f[i] = j + i < e1 ? ir_f2u(phi[j + i]) : 0u;
+          for (int i = 0; i < 4; i++) f[i] = ir_f2u(phi[j + i]);
 #pragma unroll
           for (int i = 0; i < 4; i++) sum += ir_u2f(f[i] & 0x7fffffffu);
-          sgn ^= (f[0] ^ f[1] ^ f[2] ^ f[3]) >> 31;
+          sx ^= f[0] ^ f[1] ^ f[2] ^ f[3];
         }
+        if (j < e1) {
+          uint32_t f[3];
+#pragma unroll
+          for (int i = 0; i < 3; i++) f[i] = ir_f2u(phi[j + i]);
+          f[1] = j + 1 < e1 ? f[1] : 0u;
+          f[2] = j + 2 < e1 ? f[2] : 0u;
+#pragma unroll
+          for (int i = 0; i < 3; i++) sum += ir_u2f(f[i] & 0x7fffffffu);
+          sx ^= f[0] ^ f[1] ^ f[2];
+        }
+        const uint32_t sgn = syn ^ (sx >> 31);
         rowTot[r] = ir_u2f(ir_f2u(sum) | (sgn << 31));
         if (t > 0 && (syn ^ ((parPrev[r >> 5] >> (r & 31)) & 1u))) fail = 1;
       }
