@@ -149,7 +149,8 @@ template <int N, int T, bool MODE1>
 __global__ __launch_bounds__(T) void bp_persistent_kernel(BpArgs a) {
   using S = BpShape<N, T>;
   constexpr int E = S::E;
-  // phi's logf table first, so that its LDS address fits a read's offset field
+  // phi's logf table: (invc * 2^-k, k*ln2 + logc) for each (k, i) that phi's
+  // arguments reach, read with per-lane indices
   __shared__ __align__(16) double s_phitab[2 * DVC_PHI_LOGTAB];
   // by check row; 3 spare slots at the end, so that the check pass may read
   // a row's last group of four unconditionally

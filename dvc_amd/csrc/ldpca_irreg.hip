@@ -43,7 +43,8 @@ __device__ __forceinline__ uint32_t ir_acc_bit(const uint8_t *acc, int i) {
 template <bool MODE1>
 __global__ __launch_bounds__(kIrregThreads) void bp_irregular_kernel(BpArgs a) {
   constexpr int T = kIrregThreads;
-  // phi's logf table first, so that its LDS address fits a read's offset field
+  // phi's logf table: (invc * 2^-k, k*ln2 + logc) for each (k, i) that phi's
+  // arguments reach, read with per-lane indices
   __shared__ __align__(16) double s_phitab[2 * DVC_PHI_LOGTAB];
   // by check row; during the variable pass an edge's c2v.  3 spare slots at
   // the end, so that the check pass may read a row's last slots unconditionally
